@@ -1,0 +1,218 @@
+/* smgpu.h -- C ABI of libsmgpu, the MI355X (gfx950) DistMesh force loop.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns an int status (SM_OK = 0, negative on error; sm_last_error()
+ * describes the last failure of the calling thread).  No C++ exception
+ * crosses this boundary.
+ *
+ * Device entry points take device pointers (caller-owned) and a HIP stream
+ * (passed as void*; NULL = the default stream) and only enqueue work: results
+ * are ready when the stream is synchronised.  Host entry points (sm_host_*)
+ * run synchronously on the calling thread.  A handle (sm_mesher2d) is not
+ * thread-safe; use one per stream.
+ *
+ * Reference interfaces replaced (paths relative to the SeismicMesh 3.6.2
+ * source root):
+ *   sm_unique_edges / sm_cell_edges  <- geometry/cpp/fast_geometry.cpp:49-77
+ *                                       (unique_edges) and
+ *                                       generation/mesh_generator.py:680-688
+ *   sm_sdf_eval                       <- geometry/signed_distance_functions.py
+ *                                       (Disk.eval ... Repeat.eval, :283-623),
+ *                                       drectangle_fast / dblock_fast
+ *                                       (fast_geometry.cpp:104-214)
+ *   sm_sizing_eval                    <- sizing/mesh_size_function.py:391-408
+ *                                       (RegularGridInterpolator __call__)
+ *   sm_centroid_filter                <- generation/mesh_generator.py:734-738
+ *   sm_bar_forces                     <- generation/mesh_generator.py:691-712
+ *                                       and generation/utils.py:48-68 (dense)
+ *   sm_update_project                 <- generation/mesh_generator.py:499-506,
+ *                                       :762-784 (_project_points_back_newton)
+ *   sm_host_delaunay2d, sm_flip_repair2d
+ *                                     <- generation/cpp/delaunay_class.cpp
+ *                                       :108-121, :232-298 (CGAL insert and
+ *                                       finite cells/vertices)
+ *   sm_mesher2d_*                     <- generation/mesh_generator.py:460-527
+ *                                       (the force loop itself)
+ *   sm_dihedral_angles                <- fast_geometry.cpp:351-452
+ *   sm_circumsphere_grad              <- fast_geometry.cpp:580-703
+ *   sm_limgrad                        <- sizing/cpp/FastHJ.cpp:63-187
+ */
+#ifndef SMGPU_H
+#define SMGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SM_OK 0
+#define SM_ERR_ARG (-1)       /* invalid argument (null pointer, bad size) */
+#define SM_ERR_HIP (-2)       /* a HIP runtime call failed */
+#define SM_ERR_CAPACITY (-3)  /* an output or workspace buffer is too small */
+#define SM_ERR_STATE (-4)     /* handle used in the wrong state */
+#define SM_ERR_NUMERIC (-5)   /* divergence: max movement >= 1000 h0 */
+
+/* ------------------------------------------------------------------ misc */
+int sm_version(void);
+/* Copies the last error message of the calling thread (NUL-terminated). */
+int sm_last_error(char *buf, size_t len);
+/* Number of usable gfx950 devices (0 when no GPU is present). */
+int sm_device_count(int *count);
+
+/* ---------------------------------------------------------- SDF programs
+ * A signed distance function is lowered to a postfix program of nodes
+ * ("tape").  Primitive nodes push the distance of the current point; CSG
+ * nodes pop `nargs` values and push their combination; SM_OP_PUSH_MOD pushes
+ * a transformed evaluation point (Repeat) and SM_OP_POP_POINT restores it.
+ */
+enum {
+  SM_OP_DISK = 1,      /* prm: xc, yc, r                                     */
+  SM_OP_RECT = 2,      /* prm: x1, x2, y1, y2 (bbox0)                        */
+  SM_OP_BALL = 3,      /* prm: xc, yc, zc, r                                 */
+  SM_OP_CUBE = 4,      /* prm: x1, x2, y1, y2, z1, z2 (bbox0)               */
+  SM_OP_TORUS = 5,     /* prm: r1, r2                                        */
+  SM_OP_PRISM = 6,     /* prm: b, h                                          */
+  SM_OP_CYLINDER = 7,  /* prm: r, h/2                                        */
+  SM_OP_UNION = 16,    /* nargs children; prm[0] = smoothness k (0: sharp)   */
+  SM_OP_INTERSECT = 17,
+  SM_OP_DIFFERENCE = 18,
+  SM_OP_SDIFF_STEP = 19, /* pops b, a; pushes the smooth difference step
+                            max(-a, b) + h^2/(4k), h = max(k - |-a - b|, 0) */
+  SM_OP_PUSH_MOD = 32, /* prm: period[3]: q = mod(x + p/2, p) - p/2         */
+  SM_OP_POP_POINT = 33
+};
+/* transform flags of a primitive node, applied in this order to the point
+ * before the primitive is evaluated (signed_distance_functions.py:89-101) */
+enum {
+  SM_XF_TRANSLATE = 1, /* x -= xf[0:3]                                       */
+  SM_XF_ROT_X = 2,     /* x = Rx_inv x, xf[3:12] row-major (2D: xf[3:7])     */
+  SM_XF_ROT_Y = 4,     /* x = Ry_inv x, xf[12:21]                            */
+  SM_XF_ROT_Z = 8,     /* x = Rz_inv x, xf[21:30]                            */
+  SM_XF_STRETCH = 16   /* v = xf[30:33] (unit), alpha = xf[33]               */
+};
+typedef struct {
+  int32_t op;
+  int32_t nargs;
+  int32_t flags;
+  int32_t dim;
+  double prm[8];
+  double xf[34];
+  double pad[2];
+} sm_node; /* 368 bytes */
+
+/* d[i] = sdf(x[i]) in FP64; x is (n, dim) row-major.  `tape` is a device
+ * copy of `ntape` nodes. */
+int sm_sdf_eval(const sm_node *tape, int ntape, int dim, const double *x, int64_t n,
+                double *d, void *stream);
+
+/* --------------------------------------------------------- sizing grids */
+typedef struct {
+  int32_t dim;        /* 2 or 3; axes are (z, x[, y]) = point columns 0,1,2 */
+  int32_t n[3];       /* grid points per axis                               */
+  const double *axis[3]; /* device: grid coordinates per axis (ascending)   */
+  const float *values;   /* device: n0*n1[*n2] values, C order              */
+  double hconst;      /* used when values == NULL: uniform sizing           */
+} sm_grid;
+
+/* h[i] = linear interpolation of the grid at x[i] (linear extrapolation
+ * outside), FP64 index/weight arithmetic, result in FP32. */
+int sm_sizing_eval(const sm_grid *grid, const double *x, int64_t n, float *h,
+                   void *stream);
+
+/* ----------------------------------------------------------- unique edges
+ * Sorted unique undirected edges: each pair becomes (min, max); the output
+ * is lexicographically sorted without duplicates, exactly like
+ * std::sort + std::unique in the reference.  Vertex ids must lie in
+ * [0, nvert).  `count` is a device int64 receiving the number of edges;
+ * `edges` needs room for npairs pairs. */
+size_t sm_unique_edges_workspace(int64_t npairs, int64_t nvert);
+int sm_unique_edges(const int32_t *pairs, int64_t npairs, int64_t nvert, int32_t *edges,
+                    int64_t *count, void *ws, size_t ws_bytes, void *stream);
+/* Same, taking the local edges of cells (dim+1 ids each) in the reference's
+ * order (0,1),(1,2),(2,0)[,(0,3),(1,3),(2,3)]. */
+size_t sm_cell_edges_workspace(int64_t ncells, int dim, int64_t nvert);
+int sm_cell_edges(const int32_t *cells, int64_t ncells, int dim, int64_t nvert,
+                  int32_t *edges, int64_t *count, void *ws, size_t ws_bytes, void *stream);
+
+/* ----------------------------------------------------------- the loop
+ * A 2D/3D force-loop handle owning device state for one domain:
+ * points (FP64), triangulation, edges, per-vertex edge lists, forces.
+ */
+typedef struct sm_mesher sm_mesher;
+
+/* Host sizing callback: fills h[i] = edge_length(mid[i]) for n midpoints
+ * (host arrays, (n, dim) FP64).  Returns 0 on success.  Used when the sizing
+ * is a user function that only the host can evaluate. */
+typedef int (*sm_host_sizing_fn)(void *ctx, const double *mid, int64_t n, double *h);
+
+typedef struct {
+  int32_t dim;            /* 2 (3D handled by the 3D entry points)           */
+  int32_t precision;      /* 64: FP64 forces (bit-exact with the reference's
+                             order of operations); 32: FP32 forces            */
+  int32_t nfix;           /* the first nfix points are fixed (pfix)          */
+  int32_t nlevels;        /* number of level sets (domain + subdomains)      */
+  double h0, delta_t, geps, deps, L0mult;
+} sm_mesher_params;
+
+/* Creates a handle on device `device`.  `tapes`/`tape_lens` give one SDF
+ * program per level (host arrays; copied).  `sizing` describes the sizing
+ * grid with HOST pointers (copied to the device), or a constant when
+ * values == NULL; `host_sizing` (may be NULL) overrides it with a callback. */
+int sm_mesher_create(const sm_mesher_params *prm, const sm_node *const *tapes,
+                     const int *tape_lens, const sm_grid *sizing,
+                     sm_host_sizing_fn host_sizing, void *host_sizing_ctx, int device,
+                     sm_mesher **out);
+int sm_mesher_destroy(sm_mesher *m);
+/* Uploads points (host, (n, dim) FP64) and builds the first triangulation. */
+int sm_mesher_set_points(sm_mesher *m, const double *p, int64_t n);
+/* Runs `iters` force iterations (each: triangulation repair, centroid
+ * filter, unique edges, bar forces, update, projection).  `maxdp_out`
+ * (host, may be NULL) receives the last iteration's max movement. */
+int sm_mesher_run(sm_mesher *m, int iters, double *maxdp_out);
+/* Makes the triangulation of the current points Delaunay and filters cells
+ * (the first half of an iteration), without moving points. */
+int sm_mesher_triangulate(sm_mesher *m);
+/* Reads back sizes of the current state. */
+int sm_mesher_sizes(sm_mesher *m, int64_t *npoints, int64_t *ncells, int64_t *nedges);
+/* Copies current points / kept cells / unique edges to host buffers. */
+int sm_mesher_get_points(sm_mesher *m, double *p, int64_t cap);
+int sm_mesher_get_cells(sm_mesher *m, int32_t *cells, int64_t cap);
+int sm_mesher_get_edges(sm_mesher *m, int32_t *edges, int64_t cap);
+/* Statistics since creation: [0] iterations, [1] flip rounds, [2] flips,
+ * [3] host rebuilds, [4] exact predicate fallbacks. */
+int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats);
+/* Device pointers of the state, for zero-copy access from the host side. */
+int sm_mesher_device_points(sm_mesher *m, double **p);
+/* Stream used by the handle (hipStream_t as void*). */
+int sm_mesher_stream(sm_mesher *m, void **stream);
+
+/* ------------------------------------------------------- host triangulation */
+/* Delaunay triangulation of the convex hull of p (n, 2), exact predicates.
+ * Writes real and ghost triangles (ghost vertex = -1), neighbour j opposite
+ * vertex j; returns the triangle count (>= 0) or an error code.  inserted[i]
+ * is 0 for exact duplicates that were skipped. */
+int64_t sm_host_delaunay2d(const double *p, int64_t n, int32_t *tri, int32_t *adj,
+                           int64_t cap, uint8_t *inserted);
+
+/* ------------------------------------------------------- 3D quality kernels */
+/* angles: (6*ncells) dihedral angles in radians, FP64 (fast_geometry.cpp
+ * edge table {{2,3},{1,3},{1,2},{0,3},{0,2},{0,1}}). */
+int sm_dihedral_angles(const double *p, const int32_t *cells, int64_t ncells,
+                       double *angles, void *stream);
+/* grad: (k, 3) gradient of the circumradius w.r.t. p0 for k tetrahedra. */
+int sm_circumsphere_grad(const double *p0, const double *p1, const double *p2,
+                         const double *p3, int64_t k, double *grad, void *stream);
+
+/* ------------------------------------------------------- gradient limiting */
+/* Host-callable, device-executed: ffun (nz*nx*ny, Fortran order, FP64) is
+ * limited in place so that neighbouring values differ by at most elen*dfdx
+ * (the fixed point of FastHJ.cpp's active-set relaxation). */
+int sm_limgrad(double *ffun_dev, int nz, int nx, int ny, double elen, double dfdx,
+               void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SMGPU_H */

@@ -1,0 +1,66 @@
+// Shared device helpers for libsmgpu (gfx950, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "smgpu_internal.h"
+
+#define SM_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return sm_set_error(SM_ERR_HIP, "%s:%d %s -> %s", __FILE__, __LINE__, #call,   \
+                          hipGetErrorString(e_));                                     \
+  } while (0)
+
+#define SM_CHECK(cond, code, ...)                   \
+  do {                                              \
+    if (!(cond)) return sm_set_error(code, __VA_ARGS__); \
+  } while (0)
+
+#define SM_LAUNCH_CHECK() SM_HIP(hipGetLastError())
+
+namespace smk {
+
+constexpr int kWave = 64;
+
+inline hipStream_t stream_of(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline unsigned blocks_for(int64_t n, int block) {
+  int64_t b = (n + block - 1) / block;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+// Fill kernel used instead of hipMemsetAsync so the sequence stays safe to
+// capture in a graph.
+template <typename T>
+__global__ void k_fill(T *p, int64_t n, T v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+template <typename T>
+inline int fill(T *p, int64_t n, T v, hipStream_t s) {
+  if (n <= 0) return SM_OK;
+  unsigned g = blocks_for(n, 256);
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(k_fill<T>, dim3(g), dim3(256), 0, s, p, n, v);
+  SM_LAUNCH_CHECK();
+  return SM_OK;
+}
+
+// numpy's minimum / maximum (np.minimum(a, b) returns b on ties)
+__device__ inline double np_min(double a, double b) { return (a < b || a != a) ? a : b; }
+__device__ inline double np_max(double a, double b) { return (a > b || a != a) ? a : b; }
+
+// Exclusive scan of int32 counts; out may alias in.  total (device, may be
+// null) receives the sum.  Workspace bytes from scan_workspace(n).
+size_t scan_workspace(int64_t n);
+int scan_exclusive(const int32_t *in, int32_t *out, int64_t n, int32_t *total, void *ws,
+                   hipStream_t s);
+
+}  // namespace smk

@@ -1,0 +1,300 @@
+// Sorted unique edges and the per-vertex edge lists the force scatter uses.
+//
+// Replaces SeismicMesh/geometry/cpp/fast_geometry.cpp:49-77 (unique_edges:
+// (min,max) per pair, std::sort, std::unique) as called by
+// generation/mesh_generator.py:680-688 (_get_edges).
+//
+// Method: the sort key of an edge is (min, max) with min a vertex id, so the
+// lexicographic sort is a counting sort on min (one bucket per vertex: a
+// histogram, an exclusive scan and a scatter) followed by a sort of each
+// bucket's max ids.  Buckets hold a vertex's higher neighbours (a handful in
+// a mesh), sorted in registers; long buckets (arbitrary user input) are
+// sorted by a whole workgroup in LDS or, beyond that, in global memory.
+// Duplicates are dropped inside each sorted bucket; a second scan turns the
+// per-bucket unique counts into output offsets.  The output is exactly the
+// reference's sorted unique edge list.
+#include "common.h"
+#include "edges.h"
+
+namespace smk {
+
+namespace {
+
+__device__ inline void pair_of(const int32_t *pairs, int64_t i, int32_t &a, int32_t &b) {
+  int32_t u = pairs[2 * i], v = pairs[2 * i + 1];
+  a = u < v ? u : v;
+  b = u < v ? v : u;
+}
+
+// local edge j of a cell in the reference's order (mesh_generator.py:683-687)
+__device__ inline void cell_edge(const int32_t *cells, int64_t c, int dim, int j, int32_t &a,
+                                 int32_t &b) {
+  const int32_t *t = cells + c * (dim + 1);
+  int u, v;
+  switch (j) {
+    case 0: u = t[0]; v = t[1]; break;
+    case 1: u = t[1]; v = t[2]; break;
+    case 2: u = t[2]; v = t[0]; break;
+    case 3: u = t[0]; v = t[3]; break;
+    case 4: u = t[1]; v = t[3]; break;
+    default: u = t[2]; v = t[3]; break;
+  }
+  a = u < v ? u : v;
+  b = u < v ? v : u;
+}
+
+struct PairSrc {
+  const int32_t *pairs;
+  const int32_t *cells;
+  const uint8_t *keep;  // optional cell mask
+  int dim;
+  int64_t n;            // pairs, or cells
+  __device__ inline int per() const { return pairs ? 1 : (dim == 2 ? 3 : 6); }
+  __device__ inline bool get(int64_t i, int j, int32_t &a, int32_t &b) const {
+    if (pairs) {
+      pair_of(pairs, i, a, b);
+      return true;
+    }
+    if (keep && !keep[i]) return false;
+    cell_edge(cells, i, dim, j, a, b);
+    return true;
+  }
+};
+
+__global__ void k_count(PairSrc src, int32_t *cnt) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= src.n) return;
+  const int per = src.per();
+  for (int j = 0; j < per; ++j) {
+    int32_t a, b;
+    if (src.get(i, j, a, b)) atomicAdd(&cnt[a], 1);
+  }
+}
+
+__global__ void k_scatter(PairSrc src, const int32_t *rowptr, int32_t *cur, int32_t *col) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= src.n) return;
+  const int per = src.per();
+  for (int j = 0; j < per; ++j) {
+    int32_t a, b;
+    if (src.get(i, j, a, b)) {
+      int32_t pos = rowptr[a] + atomicAdd(&cur[a], 1);
+      col[pos] = b;
+    }
+  }
+}
+
+constexpr int kRegSort = 32;    // buckets up to this size: per-thread sort
+constexpr int kLdsSort = 4096;  // up to this size: one workgroup, LDS bitonic
+
+// Per-bucket sort + unique count for short buckets (one thread each); long
+// buckets are flagged for the workgroup pass.
+__global__ void k_row_sort(int64_t nrows, const int32_t *rowptr, int32_t *col, int32_t *ucnt,
+                           int32_t *long_rows, int32_t *nlong) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  int32_t s = rowptr[r], e = rowptr[r + 1];
+  int32_t len = e - s;
+  if (len > kRegSort) {
+    int32_t slot = atomicAdd(nlong, 1);
+    long_rows[slot] = (int32_t)r;
+    ucnt[r] = 0;  // filled by the long-bucket pass
+    return;
+  }
+  int32_t v[kRegSort];
+  for (int k = 0; k < len; ++k) v[k] = col[s + k];
+  for (int k = 1; k < len; ++k) {
+    int32_t x = v[k];
+    int m = k - 1;
+    while (m >= 0 && v[m] > x) {
+      v[m + 1] = v[m];
+      --m;
+    }
+    v[m + 1] = x;
+  }
+  int32_t u = 0;
+  for (int k = 0; k < len; ++k) {
+    col[s + k] = v[k];
+    u += (k == 0 || v[k] != v[k - 1]);
+  }
+  ucnt[r] = u;
+}
+
+// Long buckets, one workgroup each: a bitonic sort in LDS (padded with
+// INT32_MAX) for buckets up to kLdsSort ids (pass `big` = false, many
+// workgroups), or in the global staging area for longer ones (pass `big` =
+// true, launched with a single workgroup so the staging area is private).
+__global__ void k_long_sort(const int32_t *long_rows, const int32_t *nlong,
+                            const int32_t *rowptr, int32_t *col, int32_t *ucnt, int32_t *stage,
+                            int big) {
+  __shared__ int32_t sh[kLdsSort];
+  __shared__ int32_t red[256];
+  const int nl = *nlong;
+  for (int li = blockIdx.x; li < nl; li += gridDim.x) {
+    const int32_t r = long_rows[li];
+    const int32_t s = rowptr[r], e = rowptr[r + 1];
+    const int32_t len = e - s;
+    int32_t p2 = 1;
+    while (p2 < len) p2 <<= 1;
+    const bool lds = p2 <= kLdsSort;
+    if (lds == (big != 0)) continue;
+    int32_t *buf = lds ? sh : stage;
+    for (int32_t k = threadIdx.x; k < p2; k += blockDim.x) buf[k] = k < len ? col[s + k] : INT32_MAX;
+    __threadfence_block();
+    __syncthreads();
+    for (int32_t size = 2; size <= p2; size <<= 1) {
+      for (int32_t stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int32_t k = threadIdx.x; k < p2; k += blockDim.x) {
+          int32_t j = k ^ stride;
+          if (j > k) {
+            bool up = (k & size) == 0;
+            int32_t x = buf[k], y = buf[j];
+            if ((x > y) == up) {
+              buf[k] = y;
+              buf[j] = x;
+            }
+          }
+        }
+        __threadfence_block();
+        __syncthreads();
+      }
+    }
+    int32_t u = 0;
+    for (int32_t k = threadIdx.x; k < len; k += blockDim.x) {
+      col[s + k] = buf[k];
+      u += (k == 0 || buf[k] != buf[k - 1]);
+    }
+    red[threadIdx.x] = u;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int32_t t = 0;
+      for (int k = 0; k < (int)blockDim.x; ++k) t += red[k];
+      ucnt[r] = t;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_emit(int64_t nrows, const int32_t *rowptr, const int32_t *col,
+                       const int32_t *eptr, int32_t *edges) {
+  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  int32_t s = rowptr[r], e = rowptr[r + 1];
+  int32_t o = eptr[r];
+  for (int32_t k = s; k < e; ++k) {
+    if (k == s || col[k] != col[k - 1]) {
+      edges[2 * (int64_t)o] = (int32_t)r;
+      edges[2 * (int64_t)o + 1] = col[k];
+      ++o;
+    }
+  }
+}
+
+__global__ void k_count_to_i64(const int32_t *c, int64_t *out) { *out = (int64_t)*c; }
+
+inline size_t al(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace
+
+size_t unique_edges_workspace(int64_t npairs, int64_t nvert) {
+  size_t b = 0;
+  b += al((nvert + 1) * 4);  // cnt -> rowptr
+  b += al((nvert + 1) * 4);  // cur
+  b += al(npairs * 4);       // col
+  b += al((nvert + 1) * 4);  // ucnt -> eptr
+  b += al((nvert + 1) * 4);  // long rows
+  b += al(4 * 4);            // counters
+  b += al((size_t)npairs * 2 * 4 + 4096 * 4);  // staging for very long buckets
+  b += 2 * scan_workspace(nvert + 1);
+  return b;
+}
+
+int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *keep, int dim,
+                     int64_t n, int64_t npairs, int64_t nvert, int32_t *edges, int32_t *count32,
+                     int64_t *count64, void *ws, size_t ws_bytes, hipStream_t s) {
+  SM_CHECK(ws_bytes >= unique_edges_workspace(npairs, nvert), SM_ERR_CAPACITY,
+           "unique edges: workspace %zu < %zu bytes", ws_bytes,
+           unique_edges_workspace(npairs, nvert));
+  char *w = reinterpret_cast<char *>(ws);
+  int32_t *rowptr = reinterpret_cast<int32_t *>(w); w += al((nvert + 1) * 4);
+  int32_t *cur = reinterpret_cast<int32_t *>(w); w += al((nvert + 1) * 4);
+  int32_t *col = reinterpret_cast<int32_t *>(w); w += al(npairs * 4);
+  int32_t *eptr = reinterpret_cast<int32_t *>(w); w += al((nvert + 1) * 4);
+  int32_t *longr = reinterpret_cast<int32_t *>(w); w += al((nvert + 1) * 4);
+  int32_t *ctr = reinterpret_cast<int32_t *>(w); w += al(4 * 4);
+  int32_t *stage = reinterpret_cast<int32_t *>(w);
+  w += al((size_t)npairs * 2 * 4 + 4096 * 4);
+  void *sws = w;
+
+  int rc;
+  if ((rc = fill<int32_t>(rowptr, nvert + 1, 0, s))) return rc;
+  if ((rc = fill<int32_t>(cur, nvert + 1, 0, s))) return rc;
+  if ((rc = fill<int32_t>(ctr, 4, 0, s))) return rc;
+  PairSrc src{pairs, cells, keep, dim, n};
+  const int B = 256;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_count, dim3(blocks_for(n, B)), dim3(B), 0, s, src, rowptr);
+    SM_LAUNCH_CHECK();
+  }
+  if ((rc = scan_exclusive(rowptr, rowptr, nvert + 1, ctr + 1, sws, s))) return rc;
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, B)), dim3(B), 0, s, src, rowptr, cur, col);
+    SM_LAUNCH_CHECK();
+  }
+  if (nvert > 0) {
+    hipLaunchKernelGGL(k_row_sort, dim3(blocks_for(nvert, B)), dim3(B), 0, s, nvert, rowptr,
+                       col, eptr, longr, ctr + 2);
+    SM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_long_sort, dim3(64), dim3(256), 0, s, longr, ctr + 2, rowptr, col, eptr,
+                       stage, 0);
+    SM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_long_sort, dim3(1), dim3(256), 0, s, longr, ctr + 2, rowptr, col, eptr,
+                       stage, 1);
+    SM_LAUNCH_CHECK();
+  }
+  if ((rc = fill<int32_t>(eptr + nvert, 1, 0, s))) return rc;
+  if ((rc = scan_exclusive(eptr, eptr, nvert + 1, count32 ? count32 : ctr + 3, sws, s))) return rc;
+  if (nvert > 0) {
+    hipLaunchKernelGGL(k_emit, dim3(blocks_for(nvert, B)), dim3(B), 0, s, nvert, rowptr, col, eptr,
+                       edges);
+    SM_LAUNCH_CHECK();
+  }
+  if (count64) {
+    hipLaunchKernelGGL(k_count_to_i64, dim3(1), dim3(1), 0, s, count32 ? count32 : ctr + 3,
+                       count64);
+    SM_LAUNCH_CHECK();
+  }
+  return SM_OK;
+}
+
+}  // namespace smk
+
+extern "C" size_t sm_unique_edges_workspace(int64_t npairs, int64_t nvert) {
+  return smk::unique_edges_workspace(npairs, nvert);
+}
+
+extern "C" int sm_unique_edges(const int32_t *pairs, int64_t npairs, int64_t nvert,
+                               int32_t *edges, int64_t *count, void *ws, size_t ws_bytes,
+                               void *stream) {
+  SM_CHECK(npairs >= 0 && nvert >= 0, SM_ERR_ARG, "unique_edges: negative size");
+  SM_CHECK(npairs == 0 || (pairs && edges), SM_ERR_ARG, "unique_edges: null pointer");
+  SM_CHECK(count && ws, SM_ERR_ARG, "unique_edges: null count/workspace");
+  return smk::unique_edges_run(pairs, nullptr, nullptr, 2, npairs, npairs, nvert, edges, nullptr,
+                               count, ws, ws_bytes, smk::stream_of(stream));
+}
+
+extern "C" size_t sm_cell_edges_workspace(int64_t ncells, int dim, int64_t nvert) {
+  return smk::unique_edges_workspace(ncells * (dim == 2 ? 3 : 6), nvert);
+}
+
+extern "C" int sm_cell_edges(const int32_t *cells, int64_t ncells, int dim, int64_t nvert,
+                             int32_t *edges, int64_t *count, void *ws, size_t ws_bytes,
+                             void *stream) {
+  SM_CHECK(dim == 2 || dim == 3, SM_ERR_ARG, "cell_edges: dim must be 2 or 3");
+  SM_CHECK(ncells >= 0 && nvert >= 0, SM_ERR_ARG, "cell_edges: negative size");
+  SM_CHECK(ncells == 0 || (cells && edges), SM_ERR_ARG, "cell_edges: null pointer");
+  SM_CHECK(count && ws, SM_ERR_ARG, "cell_edges: null count/workspace");
+  int64_t np = ncells * (dim == 2 ? 3 : 6);
+  return smk::unique_edges_run(nullptr, cells, nullptr, dim, ncells, np, nvert, edges, nullptr,
+                               count, ws, ws_bytes, smk::stream_of(stream));
+}

@@ -1,0 +1,337 @@
+// Device repair of the 2D Delaunay triangulation after the points move.
+//
+// Replaces the per-iteration CGAL re-triangulation of
+// SeismicMesh/generation/mesh_generator.py:464-469 (delaunay_class.cpp
+// insert + get_finite_cells).  The previous iteration's triangulation of the
+// whole convex hull is kept on the device as a triangulated sphere: real
+// triangles plus one ghost triangle (u, v, INF) per hull edge.  After the
+// points move, rounds of parallel edge flips restore it:
+//
+//  * untangling flips first: an edge next to an inverted (orientation <= 0)
+//    triangle is flipped when both new triangles are valid -- for a thin hull
+//    sliver whose middle vertex crossed the hull edge this turns the vertex
+//    into a hull vertex;
+//  * then Lawson flips: an edge is flipped when the opposite vertex lies
+//    strictly inside the circumcircle of the other triangle, where a ghost's
+//    circle is the open half-plane outside its hull edge (so reflex hull
+//    vertices are filled in).
+//
+// Every flip claims its two triangles with a 64-bit priority (class, then a
+// per-round scramble of the edge id); only edges owning both triangles flip,
+// so a round's flips are disjoint.  Neighbour links of the triangles around
+// a flipped pair are repaired in a separate pass.  Predicates are filtered in
+// FP64; signs the filter cannot certify are queued and settled by an exact
+// expansion pass (Shewchuk's method, see predicates.h).  When the rounds end
+// with no flip and no inverted triangle, every edge is locally Delaunay and
+// the triangulation is the Delaunay triangulation of the points; otherwise
+// the host rebuilds it from scratch with exact predicates.
+#include "common.h"
+#include "mesher.h"
+#include "predicates.h"
+
+namespace smk {
+
+namespace {
+
+constexpr int kGhostOri = 2;
+
+__device__ inline const double *PT(const double *p, int v) { return p + 2 * (int64_t)v; }
+
+__global__ void k_orient(int64_t T, const int32_t *tri, const double *p, int32_t *ori,
+                         int32_t *queue, int32_t *ctr) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  int a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
+  if (a < 0 || b < 0 || c < 0) {
+    ori[t] = kGhostOri;
+    return;
+  }
+  int o = smpred::orient2d_filter(PT(p, a), PT(p, b), PT(p, c));
+  if (o == SM_PRED_UNRESOLVED) {
+    int slot = atomicAdd(&ctr[2], 1);
+    queue[slot] = (int32_t)t;
+    ori[t] = 0;  // provisional, settled by k_orient_exact
+    return;
+  }
+  ori[t] = o;
+  if (o <= 0) atomicAdd(&ctr[1], 1);
+}
+
+__global__ void k_orient_exact(const int32_t *tri, const double *p, int32_t *ori,
+                               const int32_t *queue, int32_t *ctr) {
+  const int n = ctr[2];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int t = queue[i];
+    int o = smpred::orient2d_exact(PT(p, tri[3 * t]), PT(p, tri[3 * t + 1]),
+                                   PT(p, tri[3 * t + 2]));
+    if (o == SM_PRED_UNRESOLVED) {
+      o = 0;
+      atomicAdd(&ctr[5], 1);
+    }
+    ori[t] = o;
+    if (o <= 0) atomicAdd(&ctr[1], 1);
+  }
+}
+
+__device__ inline uint32_t scramble(uint32_t id, int round) {
+  return (id * 2654435761u) ^ ((uint32_t)round * 0x9E3779B9u);
+}
+
+// quad of edge k of triangle t: t = (x, a, b), o = (b, a, y)
+struct Quad {
+  int x, a, b, y;
+  int o;
+};
+
+__device__ inline Quad quad_of(const int32_t *tri, const int32_t *adj, int64_t t, int k) {
+  Quad q;
+  q.x = tri[3 * t + k];
+  q.a = tri[3 * t + (k + 1) % 3];
+  q.b = tri[3 * t + (k + 2) % 3];
+  q.o = adj[3 * t + k];
+  int y = -2;
+  for (int j = 0; j < 3; ++j) {
+    int v = tri[3 * (int64_t)q.o + j];
+    if (v != q.a && v != q.b) y = v;
+  }
+  q.y = y;
+  return q;
+}
+
+// Lawson test "y inside the circle of t = (x, a, b)"; t may be a ghost.
+// Returns 1/0, or SM_PRED_UNRESOLVED.
+__device__ inline int in_circle_of(const double *p, int x, int a, int b, int y) {
+  if (y < 0) return 0;  // INF is never inside a finite circle
+  if (x >= 0 && a >= 0 && b >= 0) {
+    int s = smpred::incircle_filter(PT(p, x), PT(p, a), PT(p, b), PT(p, y));
+    if (s == SM_PRED_UNRESOLVED) return s;
+    return s > 0;
+  }
+  // ghost: rotate so INF is last -> (u, v, INF); inside = strictly left of u->v
+  int u, v;
+  if (x < 0) { u = a; v = b; }
+  else if (a < 0) { u = b; v = x; }
+  else { u = x; v = a; }
+  int s = smpred::orient2d_filter(PT(p, u), PT(p, v), PT(p, y));
+  if (s == SM_PRED_UNRESOLVED) return s;
+  return s > 0;
+}
+
+// orientation of a would-be triangle; INF triangles count as valid
+__device__ inline int valid_new(const double *p, int a, int b, int c) {
+  if (a < 0 || b < 0 || c < 0) return 1;
+  int s = smpred::orient2d_filter(PT(p, a), PT(p, b), PT(p, c));
+  if (s == SM_PRED_UNRESOLVED) return 0;
+  return s > 0;
+}
+
+__global__ void k_mark(int64_t T, const int32_t *tri, const int32_t *adj, const int32_t *ori,
+                       const double *p, uint64_t *claim, uint64_t *cand, int32_t *queue,
+                       int32_t *ctr, int round) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  for (int k = 0; k < 3; ++k) {
+    uint64_t prio = 0;
+    int64_t o = adj[3 * t + k];
+    if (o > t) {
+      Quad q = quad_of(tri, adj, t, k);
+      if (q.y == -2) {  // inconsistent link: leave it to the validity check
+        cand[3 * t + k] = 0;
+        continue;
+      }
+      const bool inv_t = ori[t] <= 0;
+      const bool inv_o = ori[o] <= 0;
+      if (inv_t || inv_o) {
+        // untangling flip: both new triangles (x, a, y) and (y, b, x) valid
+        bool ghost_new = q.x < 0 || q.y < 0;
+        bool ok;
+        if (ghost_new) ok = true;  // the real old triangle is the inverted one
+        else ok = valid_new(p, q.x, q.a, q.y) && valid_new(p, q.y, q.b, q.x);
+        if (q.a < 0 || q.b < 0) {
+          // edge through INF between two ghosts: the new real triangle
+          // (x, a, y) or (y, b, x) must be positive
+          ok = valid_new(p, q.x, q.a, q.y) && valid_new(p, q.y, q.b, q.x);
+        }
+        if (ok) prio = (2ull << 32) | scramble((uint32_t)(3 * t + k), round);
+      } else {
+        int s = in_circle_of(p, q.x, q.a, q.b, q.y);
+        if (s == SM_PRED_UNRESOLVED) {
+          int slot = atomicAdd(&ctr[3], 1);
+          queue[slot] = (int32_t)(3 * t + k);
+        } else if (s) {
+          prio = (1ull << 32) | scramble((uint32_t)(3 * t + k), round);
+        }
+      }
+      if (prio) {
+        atomicMax((unsigned long long *)&claim[t], (unsigned long long)prio);
+        atomicMax((unsigned long long *)&claim[o], (unsigned long long)prio);
+      }
+    }
+    cand[3 * t + k] = prio;
+  }
+}
+
+__global__ void k_mark_exact(const int32_t *tri, const int32_t *adj, const double *p,
+                             uint64_t *claim, uint64_t *cand, const int32_t *queue, int32_t *ctr,
+                             int round) {
+  const int n = ctr[3];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    int id = queue[i];
+    int64_t t = id / 3;
+    int k = id % 3;
+    Quad q = quad_of(tri, adj, t, k);
+    // only real-real quads reach here (ghost tests use orient2d_filter,
+    // whose uncertain cases are also settled exactly below)
+    int s;
+    if (q.x >= 0 && q.a >= 0 && q.b >= 0) {
+      s = smpred::incircle_exact(PT(p, q.x), PT(p, q.a), PT(p, q.b), PT(p, q.y));
+    } else {
+      int u, v;
+      if (q.x < 0) { u = q.a; v = q.b; }
+      else if (q.a < 0) { u = q.b; v = q.x; }
+      else { u = q.x; v = q.a; }
+      s = smpred::orient2d_exact(PT(p, u), PT(p, v), PT(p, q.y));
+    }
+    if (s == SM_PRED_UNRESOLVED) {
+      atomicAdd(&ctr[5], 1);
+      continue;
+    }
+    if (s > 0) {
+      uint64_t prio = (1ull << 32) | scramble((uint32_t)id, round);
+      cand[id] = prio;
+      atomicMax((unsigned long long *)&claim[t], (unsigned long long)prio);
+      atomicMax((unsigned long long *)&claim[q.o], (unsigned long long)prio);
+    }
+  }
+}
+
+__global__ void k_apply(int64_t T, int32_t *tri, int32_t *adj, const uint64_t *claim,
+                        const uint64_t *cand, int32_t *partner, int32_t *mark, int32_t *ctr,
+                        int round) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  for (int k = 0; k < 3; ++k) {
+    uint64_t pr = cand[3 * t + k];
+    if (!pr) continue;
+    int64_t o = adj[3 * t + k];
+    if (claim[t] != pr || claim[o] != pr) continue;
+    // t = (x, a, b) with neighbours: opp x -> o, opp a -> na, opp b -> nb
+    int x = tri[3 * t + k], a = tri[3 * t + (k + 1) % 3], b = tri[3 * t + (k + 2) % 3];
+    int na = adj[3 * t + (k + 1) % 3], nb = adj[3 * t + (k + 2) % 3];
+    // o = (y, b, a) rotated: find y's slot j
+    int j = 0;
+    for (int s = 0; s < 3; ++s)
+      if (adj[3 * o + s] == t) j = s;
+    int y = tri[3 * o + j];
+    // in o = (y, b, a): the vertex after y is b, then a
+    int mb = adj[3 * o + (j + 1) % 3];  // opposite b: edge (a, y)
+    int ma = adj[3 * o + (j + 2) % 3];  // opposite a: edge (y, b)
+    // t' = (x, a, y): opp x -> mb (a,y); opp a -> o' (y,x); opp y -> nb (x,a)
+    tri[3 * t] = x; tri[3 * t + 1] = a; tri[3 * t + 2] = y;
+    adj[3 * t] = mb; adj[3 * t + 1] = (int32_t)o; adj[3 * t + 2] = nb;
+    // o' = (y, b, x): opp y -> na (b,x); opp b -> t' (x,y); opp x -> ma (y,b)
+    tri[3 * o] = y; tri[3 * o + 1] = b; tri[3 * o + 2] = x;
+    adj[3 * o] = na; adj[3 * o + 1] = (int32_t)t; adj[3 * o + 2] = ma;
+    partner[t] = (int32_t)o;
+    partner[o] = (int32_t)t;
+    mark[t] = round;
+    mark[o] = round;
+    atomicAdd(&ctr[0], 1);
+    break;
+  }
+}
+
+__global__ void k_fix(int64_t T, const int32_t *tri, int32_t *adj, const int32_t *partner,
+                      const int32_t *mark, int round) {
+  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= T) return;
+  for (int k = 0; k < 3; ++k) {
+    int nb = adj[3 * s + k];
+    if (mark[nb] != round) continue;
+    int u = tri[3 * s + (k + 1) % 3], v = tri[3 * s + (k + 2) % 3];
+    bool hu = false, hv = false;
+    for (int j = 0; j < 3; ++j) {
+      int w = tri[3 * (int64_t)nb + j];
+      hu |= (w == u);
+      hv |= (w == v);
+    }
+    if (!(hu && hv)) adj[3 * s + k] = partner[nb];
+  }
+}
+
+}  // namespace
+
+int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
+  hipStream_t s = m->stream;
+  const int64_t T = m->T;
+  const int B = 256;
+  const unsigned g = blocks_for(T, B);
+  *need_rebuild = false;
+  for (int r = 0; r < max_rounds; ++r) {
+    const int round = ++m->round;
+    int rc;
+    if ((rc = fill<int32_t>(m->d_ctr, 8, 0, s))) return rc;
+    if ((rc = fill<uint64_t>(m->d_claim, T, 0ull, s))) return rc;
+    hipLaunchKernelGGL(k_orient, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_p, m->d_ori,
+                       m->d_queue, m->d_ctr);
+    SM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_orient_exact, dim3(32), dim3(64), 0, s, m->d_tri, m->d_p, m->d_ori,
+                       m->d_queue, m->d_ctr);
+    SM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, m->d_p,
+                       m->d_claim, m->d_cand, m->d_queue + T, m->d_ctr, round);
+    SM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_mark_exact, dim3(32), dim3(64), 0, s, m->d_tri, m->d_adj, m->d_p,
+                       m->d_claim, m->d_cand, m->d_queue + T, m->d_ctr, round);
+    SM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_apply, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_claim,
+                       m->d_cand, m->d_partner, m->d_mark, m->d_ctr, round);
+    SM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_fix, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_partner,
+                       m->d_mark, round);
+    SM_LAUNCH_CHECK();
+    SM_HIP(hipMemcpyAsync(m->h_ctr, m->d_ctr, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SM_HIP(hipStreamSynchronize(s));
+    const int flips = m->h_ctr[0], inverted = m->h_ctr[1];
+    m->stats[1] += 1;
+    m->stats[2] += flips;
+    m->stats[4] += m->h_ctr[2] + m->h_ctr[3];
+    if (m->h_ctr[5] > 0) {
+      *need_rebuild = true;
+      return SM_OK;
+    }
+    if (flips == 0) {
+      // no flip this round: every edge was locally Delaunay (or blocked);
+      // inverted triangles left mean the flips could not untangle
+      *need_rebuild = inverted > 0;
+      return SM_OK;
+    }
+  }
+  *need_rebuild = true;  // did not converge within the round budget
+  return SM_OK;
+}
+
+int host_rebuild2d(sm_mesher *m) {
+  const int64_t N = m->N;
+  std::vector<double> hp((size_t)N * 2);
+  SM_HIP(hipMemcpyAsync(hp.data(), m->d_p, N * 2 * sizeof(double), hipMemcpyDeviceToHost,
+                        m->stream));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  std::vector<int32_t> tri((size_t)m->capT * 3), adj((size_t)m->capT * 3);
+  std::vector<uint8_t> ins((size_t)N);
+  int64_t T = sm_host_delaunay2d(hp.data(), N, tri.data(), adj.data(), m->capT, ins.data());
+  if (T < 0) return sm_set_error((int)T, "host Delaunay rebuild failed (%lld)", (long long)T);
+  m->T = T;
+  SM_HIP(hipMemcpyAsync(m->d_tri, tri.data(), T * 3 * sizeof(int32_t), hipMemcpyHostToDevice,
+                        m->stream));
+  SM_HIP(hipMemcpyAsync(m->d_adj, adj.data(), T * 3 * sizeof(int32_t), hipMemcpyHostToDevice,
+                        m->stream));
+  int rc = fill<int32_t>(m->d_mark, m->capT, 0, m->stream);
+  if (rc) return rc;
+  SM_HIP(hipStreamSynchronize(m->stream));
+  m->stats[3] += 1;
+  m->have_tri = true;
+  return SM_OK;
+}
+
+}  // namespace smk

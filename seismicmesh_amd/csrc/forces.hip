@@ -1,0 +1,478 @@
+// One DistMesh position update on the device.
+//
+// Follows SeismicMesh/generation/mesh_generator.py:
+//   _remove_triangles_outside  :734-738  -> k_keep
+//   _get_edges / unique_edges  :680-688  -> build_edges (edges.hip)
+//   _compute_forces            :691-712  -> k_edge_geom, sums, k_edge_force,
+//                                           k_vertex_update (scatter)
+//   Ftot[ifix] = 0, p += dt*F  :499-502  -> k_vertex_update
+//   _project_points_back_newton :762-784 -> k_vertex_update
+//
+// Layout in HBM: points are (N, dim) FP64 row-major; edges (E, 2) int32 in
+// the reference's sorted order; per-edge L, h and F vectors in the compute
+// precision T.  The scatter of generation/utils.py:48-68 (coo_matrix
+// (...).toarray(), which adds duplicates in entry order) is a deterministic
+// gather: vertex v sums -Fvec over the edges where it is the larger id, in
+// increasing edge order, then +Fvec over the edges where it is the smaller
+// id -- exactly the reference's order of additions.  With T = double and the
+// L0 sums taken in numpy's summation order (pairwise blocks of 128 inside
+// sequential chunks of 8192) an iteration reproduces the reference's float64
+// arithmetic bit for bit; the one transcendental scalar, (sum L^d / sum h^d)
+// ** (1/d), is taken on the host with the C library's pow as numpy does.
+#include <cmath>
+
+#include "common.h"
+#include "edges.h"
+#include "mesher.h"
+#include "sdf_eval.h"
+#include "sizing_eval.h"
+
+namespace smk {
+
+namespace {
+
+template <int DIM>
+__global__ void k_keep(int64_t T, const int32_t *tri, const double *p, const sm_node *tape,
+                       int ntape, double geps, uint8_t *keep) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  const int32_t *c = tri + (DIM + 1) * t;
+  bool real = true;
+#pragma unroll
+  for (int k = 0; k <= DIM; ++k) real &= c[k] >= 0;
+  if (!real) {
+    keep[t] = 0;
+    return;
+  }
+  double x[3];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    double s = p[(int64_t)c[0] * DIM + d];
+#pragma unroll
+    for (int k = 1; k <= DIM; ++k) s = s + p[(int64_t)c[k] * DIM + d];
+    x[d] = s / (double)(DIM + 1);
+  }
+  keep[t] = sdf_eval<DIM>(tape, ntape, x) < -geps;
+}
+
+// second-endpoint lists: count, fill, order
+__global__ void k_count_second(int64_t E, const int32_t *edges, int32_t *cnt) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < E) atomicAdd(&cnt[edges[2 * e + 1]], 1);
+}
+__global__ void k_fill_second(int64_t E, const int32_t *edges, const int32_t *ptr, int32_t *cur,
+                              int32_t *list) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  int b = edges[2 * e + 1];
+  list[ptr[b] + atomicAdd(&cur[b], 1)] = (int32_t)e;
+}
+__global__ void k_sort_second(int64_t N, const int32_t *ptr, int32_t *list) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= N) return;
+  int32_t s = ptr[v], e = ptr[v + 1];
+  for (int32_t k = s + 1; k < e; ++k) {
+    int32_t x = list[k];
+    int32_t j = k - 1;
+    while (j >= s && list[j] > x) {
+      list[j + 1] = list[j];
+      --j;
+    }
+    list[j + 1] = x;
+  }
+}
+
+template <typename T> __device__ inline T zero_len();
+template <> __device__ inline double zero_len<double>() { return 2.220446049250313e-16; }
+template <> __device__ inline float zero_len<float>() { return 2.220446049250313e-16f; }
+
+// Per-edge bar length and sizing; per-block partial sums of L^d and h^d
+// (FP64) for the fast path, or the per-edge powers for the exact path.
+template <int DIM, typename T, bool EXACT>
+__global__ void k_edge_geom(int64_t E, const int32_t *edges, const double *p, GridView grid,
+                            const double *h_ext, T *L_out, T *h_out, double *Lp, double *hp,
+                            double *part) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double sl = 0.0, sh = 0.0;
+  if (e < E) {
+    const int a = edges[2 * e], b = edges[2 * e + 1];
+    T L2 = 0;
+    double mid[3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      double pa = p[(int64_t)a * DIM + d], pb = p[(int64_t)b * DIM + d];
+      T bar = (T)(pa - pb);
+      L2 = L2 + bar * bar;
+      mid[d] = (pa + pb) / 2.0;
+    }
+    T L = sqrt(L2);
+    if (L == (T)0) L = zero_len<T>();
+    T h;
+    if (h_ext) h = (T)h_ext[e];
+    else h = (T)grid_eval<DIM>(grid, mid);
+    L_out[e] = L;
+    h_out[e] = h;
+    if (EXACT) {
+      if (DIM == 2) {
+        Lp[e] = (double)L * (double)L;
+        hp[e] = (double)h * (double)h;
+      } else {
+        Lp[e] = (double)L * (double)L * (double)L;
+        hp[e] = (double)h * (double)h * (double)h;
+      }
+    } else {
+      double Ld = L, hd = h;
+      sl = DIM == 2 ? Ld * Ld : Ld * Ld * Ld;
+      sh = DIM == 2 ? hd * hd : hd * hd * hd;
+    }
+  }
+  if (!EXACT) {
+    // fixed-order block reduction (deterministic)
+    __shared__ double rl[256], rh[256];
+    rl[threadIdx.x] = sl;
+    rh[threadIdx.x] = sh;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) {
+        rl[threadIdx.x] += rl[threadIdx.x + w];
+        rh[threadIdx.x] += rh[threadIdx.x + w];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      part[2 * blockIdx.x] = rl[0];
+      part[2 * blockIdx.x + 1] = rh[0];
+    }
+  }
+}
+
+// sums the per-block partials in a fixed order; sums[2] = scale (fast path)
+__global__ void k_sum_partials(int64_t nb, const double *part, double *sums, int dim) {
+  __shared__ double rl[256], rh[256];
+  double sl = 0.0, sh = 0.0;
+  for (int64_t i = threadIdx.x; i < nb; i += blockDim.x) {
+    sl += part[2 * i];
+    sh += part[2 * i + 1];
+  }
+  rl[threadIdx.x] = sl;
+  rh[threadIdx.x] = sh;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      rl[threadIdx.x] += rl[threadIdx.x + w];
+      rh[threadIdx.x] += rh[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sums[0] = rl[0];
+    sums[1] = rh[0];
+    double r = rl[0] / rh[0];
+    sums[2] = dim == 2 ? sqrt(r) : cbrt(r);
+  }
+}
+
+// numpy's float64 add.reduce over a contiguous array: sequential over chunks
+// of 8192 elements; inside a chunk, pairwise with leaves of <= 128 elements
+// summed by 8 interleaved accumulators.  One workgroup per chunk computes
+// the leaves in parallel and combines them in the recursion's order.
+constexpr int kNpChunk = 8192;
+constexpr int kNpMaxLeaves = 256;
+
+__device__ double np_leaf(const double *a, int n) {
+  if (n < 8) {
+    double r = 0.0;
+    for (int i = 0; i < n; ++i) r += a[i];
+    return r;
+  }
+  double r[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) r[j] = a[j];
+  int i;
+  for (i = 8; i < n - (n % 8); i += 8) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r[j] += a[i + j];
+  }
+  double res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
+  for (; i < n; ++i) res += a[i];
+  return res;
+}
+
+__global__ void k_np_chunks(const double *a, int64_t n, double *chunk_sums) {
+  __shared__ int loff[kNpMaxLeaves], llen[kNpMaxLeaves];
+  __shared__ double lsum[kNpMaxLeaves];
+  __shared__ int nleaves;
+  const int64_t c0 = (int64_t)blockIdx.x * kNpChunk;
+  const int m = (int)((n - c0) < kNpChunk ? (n - c0) : kNpChunk);
+  if (threadIdx.x == 0) {
+    // enumerate leaves left to right with an explicit stack
+    int so[32], sn[32], sp = 0, L = 0;
+    so[sp] = 0;
+    sn[sp] = m;
+    ++sp;
+    while (sp > 0) {
+      --sp;
+      int o = so[sp], k = sn[sp];
+      if (k <= 128) {
+        loff[L] = o;
+        llen[L] = k;
+        ++L;
+      } else {
+        int k2 = k / 2;
+        k2 -= k2 % 8;
+        // push right first so the left half is processed first
+        so[sp] = o + k2; sn[sp] = k - k2; ++sp;
+        so[sp] = o; sn[sp] = k2; ++sp;
+      }
+    }
+    nleaves = L;
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < nleaves; l += blockDim.x)
+    lsum[l] = np_leaf(a + c0 + loff[l], llen[l]);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // combine: recursion pw(o,k) = pw(left) + pw(right), leaves consumed in
+    // order; iterative post-order with a value stack
+    int so[32], sn[32], st[32], sp = 0;
+    double vs[32];
+    int vp = 0, next = 0;
+    so[0] = 0; sn[0] = m; st[0] = 0; sp = 1;
+    while (sp > 0) {
+      int top = sp - 1;
+      int k = sn[top];
+      if (k <= 128) {
+        vs[vp++] = lsum[next++];
+        --sp;
+        continue;
+      }
+      int k2 = k / 2;
+      k2 -= k2 % 8;
+      if (st[top] == 0) {
+        st[top] = 1;
+        so[sp] = so[top]; sn[sp] = k2; st[sp] = 0; ++sp;
+      } else if (st[top] == 1) {
+        st[top] = 2;
+        so[sp] = so[top] + k2; sn[sp] = k - k2; st[sp] = 0; ++sp;
+      } else {
+        double r = vs[--vp];
+        double l = vs[--vp];
+        vs[vp++] = l + r;
+        --sp;
+      }
+    }
+    chunk_sums[blockIdx.x] = vs[0];
+  }
+}
+
+__global__ void k_np_total(const double *chunk_sums, int64_t nchunks, double *out) {
+  double r = chunk_sums[0];
+  for (int64_t i = 1; i < nchunks; ++i) r += chunk_sums[i];
+  *out = r;
+}
+
+template <int DIM, typename T>
+__global__ void k_edge_force(int64_t E, const int32_t *edges, const double *p, const T *L,
+                             const T *h, const double *scale_ptr, double L0mult, T *F) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const T scale = (T)scale_ptr[2];
+  const int a = edges[2 * e], b = edges[2 * e + 1];
+  const T Le = L[e];
+  T L0 = h[e] * (T)L0mult * scale;
+  T f = L0 - Le;
+  if (f < (T)0) f = (T)0;
+  T r = f / Le;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    T bar = (T)(p[(int64_t)a * DIM + d] - p[(int64_t)b * DIM + d]);
+    F[e * DIM + d] = r * bar;
+  }
+}
+
+template <int DIM>
+__device__ inline void project_level(double *x, const sm_node *tape, int ntape, double deps,
+                                     double hmin, int level) {
+  double d = sdf_eval<DIM>(tape, ntape, x);
+  bool in = level == 0 ? (d > 0.0) : (d > 0.0 && d < hmin / 1.5);
+  if (!in) return;
+  double g[DIM];
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) {
+    double y[3];
+#pragma unroll
+    for (int k = 0; k < DIM; ++k) y[k] = x[k] + (k == i ? deps : 0.0);
+    g[i] = (sdf_eval<DIM>(tape, ntape, y) - d) / deps;
+  }
+  double g2 = g[0] * g[0];
+#pragma unroll
+  for (int i = 1; i < DIM; ++i) g2 = g2 + g[i] * g[i];
+  if (g2 < deps) g2 = deps;
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) x[i] = x[i] - (d * g[i]) / g2;
+}
+
+template <int DIM, typename T>
+__global__ void k_vertex_update(int64_t N, double *p, const int32_t *first_ptr,
+                                const int32_t *second_ptr, const int32_t *second, const T *F,
+                                int nfix, double dt, const sm_node *tapes, const int *tape_off,
+                                const int *tape_len, int nlevels, double deps, double hmin,
+                                uint64_t *maxF) {
+  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= N) return;
+  T f[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) f[d] = (T)0;
+  for (int32_t k = second_ptr[v], ke = second_ptr[v + 1]; k < ke; ++k) {
+    int64_t e = second[k];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) f[d] = f[d] + (-F[e * DIM + d]);
+  }
+  for (int32_t e = first_ptr[v], ee = first_ptr[v + 1]; e < ee; ++e) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) f[d] = f[d] + F[(int64_t)e * DIM + d];
+  }
+  if (v < nfix) {
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) f[d] = (T)0;
+  }
+  double s2 = (double)f[0] * (double)f[0];
+#pragma unroll
+  for (int d = 1; d < DIM; ++d) s2 = s2 + (double)f[d] * (double)f[d];
+  double s = sqrt(s2);
+  atomicMax((unsigned long long *)maxF, (unsigned long long)__double_as_longlong(s));
+  double x[3];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) x[d] = p[v * DIM + d] + dt * (double)f[d];
+  for (int l = 0; l < nlevels; ++l)
+    project_level<DIM>(x, tapes + tape_off[l], tape_len[l], deps, hmin, l);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) p[v * DIM + d] = x[d];
+}
+
+}  // namespace
+
+int centroid_keep(sm_mesher *m) {
+  const int B = 256;
+  if (m->T == 0) return SM_OK;
+  if (m->dim == 2)
+    hipLaunchKernelGGL(k_keep<2>, dim3(blocks_for(m->T, B)), dim3(B), 0, m->stream, m->T,
+                       m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0], m->prm.geps, m->d_keep);
+  else
+    hipLaunchKernelGGL(k_keep<3>, dim3(blocks_for(m->T, B)), dim3(B), 0, m->stream, m->T,
+                       m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0], m->prm.geps, m->d_keep);
+  SM_LAUNCH_CHECK();
+  return SM_OK;
+}
+
+// first-endpoint offsets live in the unique-edge workspace (eptr)
+static int32_t *eptr_of(sm_mesher *m) {
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  int64_t nv = m->N;
+  int64_t np = m->capT * (m->dim == 2 ? 3 : 6);
+  char *w = reinterpret_cast<char *>(m->d_ews);
+  w += al((nv + 1) * 4);
+  w += al((nv + 1) * 4);
+  w += al(np * 4);
+  return reinterpret_cast<int32_t *>(w);
+}
+
+int build_edges(sm_mesher *m) {
+  hipStream_t s = m->stream;
+  const int dim = m->dim;
+  int64_t np = m->capT * (dim == 2 ? 3 : 6);
+  int rc = unique_edges_run(nullptr, m->d_tri, m->d_keep, dim, m->T, np, m->N, m->d_edges,
+                            m->d_ctr + 4, nullptr, m->d_ews, m->ews_bytes, s);
+  if (rc) return rc;
+  // the edge count is needed on the host to size the next launches
+  SM_HIP(hipMemcpyAsync(m->h_ctr + 4, m->d_ctr + 4, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  SM_HIP(hipStreamSynchronize(s));
+  m->E = m->h_ctr[4];
+  const int B = 256;
+  const int64_t N = m->N, E = m->E;
+  if ((rc = fill<int32_t>(m->d_sptr, N + 1, 0, s))) return rc;
+  if ((rc = fill<int32_t>(m->d_scur, N + 1, 0, s))) return rc;
+  if (E > 0) {
+    hipLaunchKernelGGL(k_count_second, dim3(blocks_for(E, B)), dim3(B), 0, s, E, m->d_edges,
+                       m->d_sptr);
+    SM_LAUNCH_CHECK();
+  }
+  if ((rc = scan_exclusive(m->d_sptr, m->d_sptr, N + 1, nullptr, m->d_scan_ws, s))) return rc;
+  if (E > 0) {
+    hipLaunchKernelGGL(k_fill_second, dim3(blocks_for(E, B)), dim3(B), 0, s, E, m->d_edges,
+                       m->d_sptr, m->d_scur, m->d_slist);
+    SM_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_sort_second, dim3(blocks_for(N, B)), dim3(B), 0, s, N, m->d_sptr,
+                     m->d_slist);
+  SM_LAUNCH_CHECK();
+  return SM_OK;
+}
+
+template <int DIM, typename T>
+static int forces_t(sm_mesher *m) {
+  hipStream_t s = m->stream;
+  const int B = 256;
+  const int64_t E = m->E, N = m->N;
+  const bool exact = m->prm.precision == 64;
+  const unsigned ge = blocks_for(E, B);
+  GridView g = m->grid;
+  if (E > 0) {
+    if (exact)
+      hipLaunchKernelGGL((k_edge_geom<DIM, T, true>), dim3(ge), dim3(B), 0, s, E, m->d_edges,
+                         m->d_p, g, m->d_hext, (T *)m->d_L, (T *)m->d_h, m->d_Lp,
+                         m->d_hp, m->d_part);
+    else
+      hipLaunchKernelGGL((k_edge_geom<DIM, T, false>), dim3(ge), dim3(B), 0, s, E, m->d_edges,
+                         m->d_p, g, m->d_hext, (T *)m->d_L, (T *)m->d_h, m->d_Lp,
+                         m->d_hp, m->d_part);
+    SM_LAUNCH_CHECK();
+  }
+  if (exact) {
+    int64_t nch = (E + kNpChunk - 1) / kNpChunk;
+    if (nch < 1) nch = 1;
+    if (E > 0) {
+      hipLaunchKernelGGL(k_np_chunks, dim3((unsigned)nch), dim3(64), 0, s, m->d_Lp, E,
+                         m->d_part);
+      SM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_np_total, dim3(1), dim3(1), 0, s, m->d_part, nch, m->d_sums);
+      SM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_np_chunks, dim3((unsigned)nch), dim3(64), 0, s, m->d_hp, E,
+                         m->d_part);
+      SM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_np_total, dim3(1), dim3(1), 0, s, m->d_part, nch, m->d_sums + 1);
+      SM_LAUNCH_CHECK();
+    }
+    // (sum L^d / sum h^d) ** (1/d) with the C library's pow, as numpy does
+    SM_HIP(hipMemcpyAsync(m->h_sums, m->d_sums, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    SM_HIP(hipStreamSynchronize(s));
+    m->h_sums[2] = std::pow(m->h_sums[0] / m->h_sums[1], 1.0 / DIM);
+    SM_HIP(hipMemcpyAsync(m->d_sums + 2, m->h_sums + 2, sizeof(double), hipMemcpyHostToDevice, s));
+  } else if (E > 0) {
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, (int64_t)ge, m->d_part,
+                       m->d_sums, DIM);
+    SM_LAUNCH_CHECK();
+  }
+  if (E > 0) {
+    hipLaunchKernelGGL((k_edge_force<DIM, T>), dim3(ge), dim3(B), 0, s, E, m->d_edges, m->d_p,
+                       (const T *)m->d_L, (const T *)m->d_h, m->d_sums, m->prm.L0mult,
+                       (T *)m->d_F);
+    SM_LAUNCH_CHECK();
+  }
+  int rc = fill<uint64_t>(m->d_maxF, 1, 0ull, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_vertex_update<DIM, T>), dim3(blocks_for(N, B)), dim3(B), 0, s, N,
+                     m->d_p, eptr_of(m), m->d_sptr, m->d_slist, (const T *)m->d_F,
+                     m->prm.nfix, m->prm.delta_t, m->d_tape_all, m->d_tape_off_dev,
+                     m->d_tape_len_dev, m->prm.nlevels, m->prm.deps, m->prm.h0, m->d_maxF);
+  SM_LAUNCH_CHECK();
+  return SM_OK;
+}
+
+int forces_and_update(sm_mesher *m) {
+  const bool f64 = m->prm.precision == 64;
+  if (m->dim == 2) return f64 ? forces_t<2, double>(m) : forces_t<2, float>(m);
+  return f64 ? forces_t<3, double>(m) : forces_t<3, float>(m);
+}
+
+}  // namespace smk

@@ -1,0 +1,83 @@
+// Device state of one force-loop handle (2D path).
+#pragma once
+
+#include <vector>
+
+#include "common.h"
+#include "sizing_eval.h"
+
+struct sm_mesher {
+  sm_mesher_params prm{};
+  int dim = 2;
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  // SDF programs, one per level (domain first, then subdomains)
+  std::vector<sm_node *> d_tapes;
+  std::vector<int> tape_len;
+  sm_node *d_tape_all = nullptr;  // concatenated, with offsets below
+  std::vector<int> tape_off;
+  int *d_tape_off_dev = nullptr, *d_tape_len_dev = nullptr;
+
+  // host sizing callback (for sizing given as a Python callable): called
+  // with the edge midpoints, fills h; the device path then uses d_hext
+  sm_host_sizing_fn host_sizing = nullptr;
+  void *host_sizing_ctx = nullptr;
+  double *d_mid = nullptr, *d_hext = nullptr;
+
+  // sizing grid (device copies of axes and values)
+  smk::GridView grid{};
+  double *d_axis[3] = {nullptr, nullptr, nullptr};
+  float *d_values = nullptr;
+
+  // points
+  int64_t N = 0, capN = 0;
+  double *d_p = nullptr;
+
+  // triangulation (real + ghost triangles; ghost vertex = -1)
+  int64_t T = 0, capT = 0;
+  int32_t *d_tri = nullptr, *d_adj = nullptr;
+  int32_t *d_ori = nullptr;
+  uint64_t *d_claim = nullptr, *d_cand = nullptr;
+  int32_t *d_partner = nullptr, *d_mark = nullptr;
+  int32_t *d_queue = nullptr;  // unresolved predicate queue
+  uint8_t *d_keep = nullptr;
+
+  // unique edges and per-vertex lists
+  int64_t E = 0, capE = 0;
+  int32_t *d_edges = nullptr;
+  void *d_ews = nullptr;
+  size_t ews_bytes = 0;
+  int32_t *d_sptr = nullptr, *d_scur = nullptr, *d_slist = nullptr;
+  void *d_scan_ws = nullptr;
+
+  // forces
+  void *d_L = nullptr, *d_h = nullptr, *d_F = nullptr;  // per edge (T = float|double)
+  double *d_Lp = nullptr, *d_hp = nullptr;              // exact-order sums
+  double *d_part = nullptr;                             // per-block partial sums
+  double *d_sums = nullptr;                             // [sumL, sumH, scale]
+  uint64_t *d_maxF = nullptr;
+
+  // counters and their pinned host mirror
+  // [0] flips, [1] inverted, [2] ori-queue, [3] inc-queue, [4] edge count,
+  // [5] unresolved left, [6] kept cells
+  int32_t *d_ctr = nullptr;
+  int32_t *h_ctr = nullptr;
+  double *h_sums = nullptr;
+  uint64_t *h_maxF = nullptr;
+
+  int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int round = 0;
+  bool have_tri = false;
+  double last_maxdp = 0.0;
+};
+
+namespace smk {
+// flip2d.hip
+int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild);
+int host_rebuild2d(sm_mesher *m);
+// forces.hip
+int centroid_keep(sm_mesher *m);
+int build_edges(sm_mesher *m);
+int forces_and_update(sm_mesher *m);
+}  // namespace smk

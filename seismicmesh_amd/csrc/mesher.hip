@@ -1,0 +1,359 @@
+// The force-loop handle: device state and the per-iteration sequence of
+// SeismicMesh/generation/mesh_generator.py:460-527 (re-triangulate, filter
+// cells, forces, update, project), for 2D domains.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "edges.h"
+#include "mesher.h"
+
+using namespace smk;
+
+namespace {
+
+template <typename T>
+int dalloc(T **p, int64_t n) {
+  *p = nullptr;
+  if (n <= 0) n = 1;
+  SM_HIP(hipMalloc((void **)p, (size_t)n * sizeof(T)));
+  return SM_OK;
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+void free_point_state(sm_mesher *m) {
+  dfree(m->d_p);
+  dfree(m->d_tri);
+  dfree(m->d_adj);
+  dfree(m->d_ori);
+  dfree(m->d_claim);
+  dfree(m->d_cand);
+  dfree(m->d_partner);
+  dfree(m->d_mark);
+  dfree(m->d_queue);
+  dfree(m->d_keep);
+  dfree(m->d_edges);
+  dfree(m->d_ews);
+  dfree(m->d_sptr);
+  dfree(m->d_scur);
+  dfree(m->d_slist);
+  dfree(m->d_scan_ws);
+  dfree(m->d_L);
+  dfree(m->d_h);
+  dfree(m->d_F);
+  dfree(m->d_Lp);
+  dfree(m->d_hp);
+  dfree(m->d_part);
+  dfree(m->d_mid);
+  dfree(m->d_hext);
+}
+
+int alloc_point_state(sm_mesher *m, int64_t N) {
+  free_point_state(m);
+  const int dim = m->dim;
+  m->capN = N;
+  m->capT = 2 * N + 16;
+  const int64_t T = m->capT;
+  const int64_t npairs = T * (dim == 2 ? 3 : 6);
+  m->capE = npairs;
+  int rc;
+  if ((rc = dalloc(&m->d_p, N * dim))) return rc;
+  if ((rc = dalloc(&m->d_tri, T * (dim + 1)))) return rc;
+  if ((rc = dalloc(&m->d_adj, T * (dim + 1)))) return rc;
+  if ((rc = dalloc(&m->d_ori, T))) return rc;
+  if ((rc = dalloc(&m->d_claim, T))) return rc;
+  if ((rc = dalloc(&m->d_cand, T * (dim + 1)))) return rc;
+  if ((rc = dalloc(&m->d_partner, T))) return rc;
+  if ((rc = dalloc(&m->d_mark, T))) return rc;
+  if ((rc = dalloc(&m->d_queue, 4 * T))) return rc;
+  if ((rc = dalloc(&m->d_keep, T))) return rc;
+  if ((rc = dalloc(&m->d_edges, 2 * m->capE))) return rc;
+  m->ews_bytes = unique_edges_workspace(npairs, N);
+  SM_HIP(hipMalloc(&m->d_ews, m->ews_bytes));
+  if ((rc = dalloc(&m->d_sptr, N + 1))) return rc;
+  if ((rc = dalloc(&m->d_scur, N + 1))) return rc;
+  if ((rc = dalloc(&m->d_slist, m->capE))) return rc;
+  SM_HIP(hipMalloc(&m->d_scan_ws, scan_workspace(N + 1)));
+  const size_t tsz = m->prm.precision == 64 ? 8 : 4;
+  SM_HIP(hipMalloc(&m->d_L, (size_t)m->capE * tsz));
+  SM_HIP(hipMalloc(&m->d_h, (size_t)m->capE * tsz));
+  SM_HIP(hipMalloc(&m->d_F, (size_t)m->capE * dim * tsz));
+  if ((rc = dalloc(&m->d_Lp, m->capE))) return rc;
+  if ((rc = dalloc(&m->d_hp, m->capE))) return rc;
+  if ((rc = dalloc(&m->d_part, 2 * (m->capE / 256 + 2) + 16))) return rc;
+  if (m->host_sizing) {
+    if ((rc = dalloc(&m->d_mid, m->capE * dim))) return rc;
+    if ((rc = dalloc(&m->d_hext, m->capE))) return rc;
+  }
+  if ((rc = fill<int32_t>(m->d_mark, T, 0, m->stream))) return rc;
+  return SM_OK;
+}
+
+__global__ void k_midpoints2(int64_t E, const int32_t *edges, const double *p, double *mid) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  int a = edges[2 * e], b = edges[2 * e + 1];
+  for (int d = 0; d < 2; ++d) mid[2 * e + d] = (p[2 * a + d] + p[2 * b + d]) / 2.0;
+}
+
+int host_sizing_pass(sm_mesher *m) {
+  // midpoints -> host -> user function -> device h
+  const int64_t E = m->E;
+  if (E == 0) return SM_OK;
+  hipLaunchKernelGGL(k_midpoints2, dim3(blocks_for(E, 256)), dim3(256), 0, m->stream, E,
+                     m->d_edges, m->d_p, m->d_mid);
+  SM_LAUNCH_CHECK();
+  std::vector<double> mid((size_t)E * 2), h((size_t)E);
+  SM_HIP(hipMemcpyAsync(mid.data(), m->d_mid, E * 2 * sizeof(double), hipMemcpyDeviceToHost,
+                        m->stream));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  int rc = m->host_sizing(m->host_sizing_ctx, mid.data(), E, h.data());
+  SM_CHECK(rc == 0, SM_ERR_ARG, "host sizing callback failed (%d)", rc);
+  SM_HIP(hipMemcpyAsync(m->d_hext, h.data(), E * sizeof(double), hipMemcpyHostToDevice,
+                        m->stream));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  return SM_OK;
+}
+
+int triangulate(sm_mesher *m) {
+  bool rebuild = !m->have_tri;
+  if (!rebuild) {
+    int rc = flip_repair2d(m, 64, &rebuild);
+    if (rc) return rc;
+  }
+  if (rebuild) {
+    int rc = host_rebuild2d(m);
+    if (rc) return rc;
+  }
+  return centroid_keep(m);
+}
+
+}  // namespace
+
+extern "C" int sm_mesher_create(const sm_mesher_params *prm, const sm_node *const *tapes,
+                                const int *tape_lens, const sm_grid *sizing,
+                                sm_host_sizing_fn host_sizing, void *host_sizing_ctx,
+                                int device, sm_mesher **out) {
+  SM_CHECK(prm && tapes && tape_lens && sizing && out, SM_ERR_ARG, "mesher_create: null argument");
+  SM_CHECK(prm->dim == 2, SM_ERR_ARG, "mesher_create: the device force loop is 2D (got dim=%d)",
+           prm->dim);
+  SM_CHECK(prm->precision == 32 || prm->precision == 64, SM_ERR_ARG,
+           "mesher_create: precision must be 32 or 64");
+  SM_CHECK(prm->nlevels >= 1 && prm->nlevels <= 64, SM_ERR_ARG, "mesher_create: bad nlevels");
+  SM_HIP(hipSetDevice(device));
+  sm_mesher *m = new sm_mesher();
+  m->prm = *prm;
+  m->dim = prm->dim;
+  m->device = device;
+  m->host_sizing = host_sizing;
+  m->host_sizing_ctx = host_sizing_ctx;
+  SM_HIP(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+  // SDF programs
+  int total = 0;
+  for (int l = 0; l < prm->nlevels; ++l) {
+    SM_CHECK(tapes[l] && tape_lens[l] > 0, SM_ERR_ARG, "mesher_create: empty SDF program");
+    m->tape_off.push_back(total);
+    m->tape_len.push_back(tape_lens[l]);
+    total += tape_lens[l];
+  }
+  SM_HIP(hipMalloc(&m->d_tape_all, total * sizeof(sm_node)));
+  for (int l = 0; l < prm->nlevels; ++l) {
+    SM_HIP(hipMemcpy(m->d_tape_all + m->tape_off[l], tapes[l], tape_lens[l] * sizeof(sm_node),
+                     hipMemcpyHostToDevice));
+    m->d_tapes.push_back(m->d_tape_all + m->tape_off[l]);
+  }
+  SM_HIP(hipMalloc(&m->d_tape_off_dev, prm->nlevels * sizeof(int)));
+  SM_HIP(hipMalloc(&m->d_tape_len_dev, prm->nlevels * sizeof(int)));
+  SM_HIP(hipMemcpy(m->d_tape_off_dev, m->tape_off.data(), prm->nlevels * sizeof(int),
+                   hipMemcpyHostToDevice));
+  SM_HIP(hipMemcpy(m->d_tape_len_dev, m->tape_len.data(), prm->nlevels * sizeof(int),
+                   hipMemcpyHostToDevice));
+  // sizing grid
+  m->grid.dim = m->dim;
+  m->grid.values = nullptr;
+  m->grid.hconst = sizing->hconst;
+  if (sizing->values) {
+    int64_t nv = 1;
+    for (int k = 0; k < m->dim; ++k) {
+      SM_CHECK(sizing->n[k] >= 2, SM_ERR_ARG, "mesher_create: grid axis %d has < 2 points", k);
+      m->grid.n[k] = sizing->n[k];
+      nv *= sizing->n[k];
+      SM_HIP(hipMalloc(&m->d_axis[k], sizing->n[k] * sizeof(double)));
+      SM_HIP(hipMemcpy(m->d_axis[k], sizing->axis[k], sizing->n[k] * sizeof(double),
+                       hipMemcpyHostToDevice));
+      m->grid.axis[k] = m->d_axis[k];
+    }
+    SM_HIP(hipMalloc(&m->d_values, nv * sizeof(float)));
+    SM_HIP(hipMemcpy(m->d_values, sizing->values, nv * sizeof(float), hipMemcpyHostToDevice));
+    m->grid.values = m->d_values;
+  }
+  SM_HIP(hipMalloc(&m->d_ctr, 16 * sizeof(int32_t)));
+  SM_HIP(hipMalloc(&m->d_sums, 4 * sizeof(double)));
+  SM_HIP(hipMalloc(&m->d_maxF, sizeof(uint64_t)));
+  SM_HIP(hipHostMalloc((void **)&m->h_ctr, 16 * sizeof(int32_t), hipHostMallocDefault));
+  SM_HIP(hipHostMalloc((void **)&m->h_sums, 4 * sizeof(double), hipHostMallocDefault));
+  SM_HIP(hipHostMalloc((void **)&m->h_maxF, sizeof(uint64_t), hipHostMallocDefault));
+  *out = m;
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_destroy(sm_mesher *m) {
+  if (!m) return SM_OK;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  free_point_state(m);
+  dfree(m->d_tape_all);
+  dfree(m->d_tape_off_dev);
+  dfree(m->d_tape_len_dev);
+  for (int k = 0; k < 3; ++k) dfree(m->d_axis[k]);
+  dfree(m->d_values);
+  dfree(m->d_ctr);
+  dfree(m->d_sums);
+  dfree(m->d_maxF);
+  if (m->h_ctr) (void)hipHostFree(m->h_ctr);
+  if (m->h_sums) (void)hipHostFree(m->h_sums);
+  if (m->h_maxF) (void)hipHostFree(m->h_maxF);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_set_points(sm_mesher *m, const double *p, int64_t n) {
+  SM_CHECK(m && p && n > 0, SM_ERR_ARG, "mesher_set_points: bad argument");
+  SM_HIP(hipSetDevice(m->device));
+  if (n != m->capN) {
+    int rc = alloc_point_state(m, n);
+    if (rc) return rc;
+  }
+  m->N = n;
+  SM_HIP(hipMemcpyAsync(m->d_p, p, n * m->dim * sizeof(double), hipMemcpyHostToDevice,
+                        m->stream));
+  m->have_tri = false;
+  int rc = host_rebuild2d(m);
+  if (rc) return rc;
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_triangulate(sm_mesher *m) {
+  SM_CHECK(m && m->N > 0, SM_ERR_STATE, "mesher_triangulate: no points");
+  SM_HIP(hipSetDevice(m->device));
+  int rc = triangulate(m);
+  if (rc) return rc;
+  SM_HIP(hipStreamSynchronize(m->stream));
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_run(sm_mesher *m, int iters, double *maxdp_out) {
+  SM_CHECK(m && m->N > 0, SM_ERR_STATE, "mesher_run: no points");
+  SM_CHECK(iters >= 0, SM_ERR_ARG, "mesher_run: negative iteration count");
+  SM_HIP(hipSetDevice(m->device));
+  const double limit = 1000.0 * m->prm.h0;
+  for (int it = 0; it < iters; ++it) {
+    int rc;
+    if ((rc = triangulate(m))) return rc;
+    if (it > 0) {
+      // the previous iteration's max movement arrived with the flips' sync
+      double mf;
+      std::memcpy(&mf, m->h_maxF, sizeof(double));
+      m->last_maxdp = m->prm.delta_t * mf;
+      if (!(m->last_maxdp < limit))
+        return sm_set_error(SM_ERR_NUMERIC,
+                            "max movement indicates there's a convergence problem (%g)",
+                            m->last_maxdp);
+    }
+    if ((rc = build_edges(m))) return rc;
+    if (m->host_sizing && (rc = host_sizing_pass(m))) return rc;
+    if ((rc = forces_and_update(m))) return rc;
+    SM_HIP(hipMemcpyAsync(m->h_maxF, m->d_maxF, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                          m->stream));
+    m->stats[0] += 1;
+    if (it == iters - 1) {
+      SM_HIP(hipStreamSynchronize(m->stream));
+      double mf;
+      std::memcpy(&mf, m->h_maxF, sizeof(double));
+      m->last_maxdp = m->prm.delta_t * mf;
+      if (!(m->last_maxdp < limit))
+        return sm_set_error(SM_ERR_NUMERIC,
+                            "max movement indicates there's a convergence problem (%g)",
+                            m->last_maxdp);
+    }
+  }
+  if (maxdp_out) *maxdp_out = m->last_maxdp;
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_sizes(sm_mesher *m, int64_t *npoints, int64_t *ncells,
+                               int64_t *nedges) {
+  SM_CHECK(m, SM_ERR_ARG, "mesher_sizes: null handle");
+  SM_HIP(hipSetDevice(m->device));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  if (npoints) *npoints = m->N;
+  if (nedges) *nedges = m->E;
+  if (ncells) {
+    std::vector<uint8_t> keep((size_t)std::max<int64_t>(m->T, 1));
+    SM_HIP(hipMemcpy(keep.data(), m->d_keep, m->T, hipMemcpyDeviceToHost));
+    int64_t c = 0;
+    for (int64_t t = 0; t < m->T; ++t) c += keep[t] != 0;
+    *ncells = c;
+  }
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_get_points(sm_mesher *m, double *p, int64_t cap) {
+  SM_CHECK(m && p && cap >= m->N, SM_ERR_CAPACITY, "mesher_get_points: buffer too small");
+  SM_HIP(hipSetDevice(m->device));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  SM_HIP(hipMemcpy(p, m->d_p, m->N * m->dim * sizeof(double), hipMemcpyDeviceToHost));
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_get_cells(sm_mesher *m, int32_t *cells, int64_t cap) {
+  SM_CHECK(m && cells, SM_ERR_ARG, "mesher_get_cells: null argument");
+  SM_HIP(hipSetDevice(m->device));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  const int k = m->dim + 1;
+  std::vector<uint8_t> keep((size_t)std::max<int64_t>(m->T, 1));
+  std::vector<int32_t> tri((size_t)std::max<int64_t>(m->T, 1) * k);
+  SM_HIP(hipMemcpy(keep.data(), m->d_keep, m->T, hipMemcpyDeviceToHost));
+  SM_HIP(hipMemcpy(tri.data(), m->d_tri, m->T * k * sizeof(int32_t), hipMemcpyDeviceToHost));
+  int64_t c = 0;
+  for (int64_t t = 0; t < m->T; ++t) {
+    if (!keep[t]) continue;
+    SM_CHECK(c < cap, SM_ERR_CAPACITY, "mesher_get_cells: buffer too small");
+    for (int j = 0; j < k; ++j) cells[c * k + j] = tri[t * k + j];
+    ++c;
+  }
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_get_edges(sm_mesher *m, int32_t *edges, int64_t cap) {
+  SM_CHECK(m && edges && cap >= m->E, SM_ERR_CAPACITY, "mesher_get_edges: buffer too small");
+  SM_HIP(hipSetDevice(m->device));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  SM_HIP(hipMemcpy(edges, m->d_edges, m->E * 2 * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats) {
+  SM_CHECK(m && stats, SM_ERR_ARG, "mesher_stats: null argument");
+  for (int i = 0; i < nstats && i < 8; ++i) stats[i] = m->stats[i];
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_device_points(sm_mesher *m, double **p) {
+  SM_CHECK(m && p, SM_ERR_ARG, "mesher_device_points: null argument");
+  *p = m->d_p;
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_stream(sm_mesher *m, void **stream) {
+  SM_CHECK(m && stream, SM_ERR_ARG, "mesher_stream: null argument");
+  *stream = (void *)m->stream;
+  return SM_OK;
+}

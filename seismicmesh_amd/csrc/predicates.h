@@ -1,0 +1,502 @@
+// Filtered exact geometric predicates for host and device (gfx950).
+//
+// Method: Shewchuk, "Adaptive Precision Floating-Point Arithmetic and Fast
+// Robust Geometric Predicates" (Discrete Comput. Geom. 18, 1997).  Each
+// predicate first evaluates its determinant in double precision and accepts
+// the sign when it exceeds the paper's forward error bound (stage A); else it
+// re-evaluates the determinant exactly with floating-point expansions
+// (sums of non-overlapping doubles).  Products use FMA for the exact product
+// tail.  This file must be compiled without floating-point contraction
+// (-ffp-contract=off) so that the double-precision stage matches the error
+// analysis.
+//
+// The exact stage keeps expansions in fixed-size arrays.  A predicate whose
+// exact evaluation would not fit returns SM_PRED_UNRESOLVED; callers treat
+// that as "cannot decide here" and fall back to the host rebuild.
+#pragma once
+
+#include <math.h>
+
+#if defined(__HIPCC__)
+#define SM_HD __host__ __device__
+#else
+#define SM_HD
+#endif
+
+#define SM_PRED_UNRESOLVED 0x7fffffff
+
+namespace smpred {
+
+// 2^-53: half an ulp of 1.0.
+constexpr double kEps = 1.1102230246251565e-16;
+constexpr double kCcwBoundA = (3.0 + 16.0 * kEps) * kEps;
+constexpr double kIccBoundA = (10.0 + 96.0 * kEps) * kEps;
+constexpr double kO3dBoundA = (7.0 + 56.0 * kEps) * kEps;
+constexpr double kIspBoundA = (16.0 + 224.0 * kEps) * kEps;
+
+// ---------------------------------------------------------------- EFTs
+SM_HD inline void two_sum(double a, double b, double &x, double &y) {
+  x = a + b;
+  double bv = x - a;
+  double av = x - bv;
+  y = (a - av) + (b - bv);
+}
+SM_HD inline void two_diff(double a, double b, double &x, double &y) {
+  x = a - b;
+  double bv = a - x;
+  double av = x + bv;
+  y = (a - av) + (bv - b);
+}
+SM_HD inline void two_prod(double a, double b, double &x, double &y) {
+  x = a * b;
+  y = fma(a, b, -x);
+}
+
+// An expansion: components in increasing magnitude, non-overlapping,
+// zero-free.  N is the capacity.
+template <int N> struct Exp {
+  double c[N];
+  int n = 0;
+  bool ovf = false;
+};
+
+// h = e + b  (Grow-Expansion with zero elimination)
+template <int N, int M>
+SM_HD inline void grow(const Exp<M> &e, double b, Exp<N> &h) {
+  h.n = 0;
+  h.ovf = e.ovf;
+  double q = b;
+  for (int i = 0; i < e.n; ++i) {
+    double s, t;
+    two_sum(q, e.c[i], s, t);
+    q = s;
+    if (t != 0.0) {
+      if (h.n >= N) { h.ovf = true; return; }
+      h.c[h.n++] = t;
+    }
+  }
+  if (q != 0.0 || h.n == 0) {
+    if (h.n >= N) { h.ovf = true; return; }
+    h.c[h.n++] = q;
+  }
+}
+
+// h = e + f  (repeated Grow-Expansion)
+template <int N, int M, int K>
+SM_HD inline void add(const Exp<M> &e, const Exp<K> &f, Exp<N> &h) {
+  Exp<N> tmp;
+  h.n = 0;
+  h.ovf = e.ovf || f.ovf;
+  for (int i = 0; i < e.n && i < N; ++i) h.c[h.n++] = e.c[i];
+  if (e.n > N) { h.ovf = true; return; }
+  for (int j = 0; j < f.n; ++j) {
+    grow(h, f.c[j], tmp);
+    bool o = h.ovf || tmp.ovf;
+    h = tmp;
+    h.ovf = o;
+    if (o) return;
+  }
+}
+
+template <int N, int M>
+SM_HD inline void negate(const Exp<M> &e, Exp<N> &h) {
+  h.n = e.n;
+  h.ovf = e.ovf;
+  for (int i = 0; i < e.n; ++i) h.c[i] = -e.c[i];
+}
+
+// h = e * b  (Scale-Expansion with zero elimination)
+template <int N, int M>
+SM_HD inline void scale(const Exp<M> &e, double b, Exp<N> &h) {
+  h.n = 0;
+  h.ovf = e.ovf;
+  if (e.n == 0 || b == 0.0) return;
+  double q, hh;
+  two_prod(e.c[0], b, q, hh);
+  if (hh != 0.0) h.c[h.n++] = hh;
+  for (int i = 1; i < e.n; ++i) {
+    double p1, p0, s, t;
+    two_prod(e.c[i], b, p1, p0);
+    two_sum(q, p0, s, t);
+    if (t != 0.0) {
+      if (h.n >= N) { h.ovf = true; return; }
+      h.c[h.n++] = t;
+    }
+    double s2, t2;
+    two_sum(p1, s, s2, t2);  // p1 dominates s: fast path is exact too
+    q = s2;
+    if (t2 != 0.0) {
+      if (h.n >= N) { h.ovf = true; return; }
+      h.c[h.n++] = t2;
+    }
+  }
+  if (q != 0.0 || h.n == 0) {
+    if (h.n >= N) { h.ovf = true; return; }
+    h.c[h.n++] = q;
+  }
+}
+
+// h = e * f
+template <int N, int M, int K>
+SM_HD inline void mul(const Exp<M> &e, const Exp<K> &f, Exp<N> &h) {
+  h.n = 0;
+  h.ovf = e.ovf || f.ovf;
+  if (h.ovf) return;
+  Exp<N> part, acc;
+  for (int j = 0; j < f.n; ++j) {
+    scale(e, f.c[j], part);
+    if (part.ovf) { h.ovf = true; return; }
+    add(h, part, acc);
+    if (acc.ovf) { h.ovf = true; return; }
+    h = acc;
+  }
+}
+
+template <int N> SM_HD inline int sign_of(const Exp<N> &e) {
+  if (e.ovf) return SM_PRED_UNRESOLVED;
+  if (e.n == 0) return 0;
+  double top = e.c[e.n - 1];
+  return (top > 0.0) - (top < 0.0);
+}
+
+template <int N> SM_HD inline void from_diff(double a, double b, Exp<N> &h) {
+  double x, y;
+  two_diff(a, b, x, y);
+  h.n = 0;
+  h.ovf = false;
+  if (y != 0.0) h.c[h.n++] = y;
+  if (x != 0.0) h.c[h.n++] = x;
+}
+
+// ---------------------------------------------------------------- orient2d
+// Sign of det[[ax-cx, ay-cy], [bx-cx, by-cy]]; > 0 when a, b, c turn CCW.
+SM_HD inline int orient2d_exact(const double *a, const double *b, const double *c) {
+  Exp<2> acx, acy, bcx, bcy;
+  from_diff(a[0], c[0], acx);
+  from_diff(a[1], c[1], acy);
+  from_diff(b[0], c[0], bcx);
+  from_diff(b[1], c[1], bcy);
+  Exp<8> l, r, nr;
+  mul(acx, bcy, l);
+  mul(acy, bcx, r);
+  negate(r, nr);
+  Exp<16> d;
+  add(l, nr, d);
+  return sign_of(d);
+}
+
+SM_HD inline int orient2d(const double *a, const double *b, const double *c) {
+  double detleft = (a[0] - c[0]) * (b[1] - c[1]);
+  double detright = (a[1] - c[1]) * (b[0] - c[0]);
+  double det = detleft - detright;
+  double detsum;
+  if (detleft > 0.0) {
+    if (detright <= 0.0) return (det > 0.0) - (det < 0.0);
+    detsum = detleft + detright;
+  } else if (detleft < 0.0) {
+    if (detright >= 0.0) return (det > 0.0) - (det < 0.0);
+    detsum = -detleft - detright;
+  } else {
+    // detleft is an exact zero: det = -detright carries the exact sign
+    return (det > 0.0) - (det < 0.0);
+  }
+  double err = kCcwBoundA * detsum;
+  if (det > err) return 1;
+  if (-det > err) return -1;
+  return orient2d_exact(a, b, c);
+}
+
+// ---------------------------------------------------------------- incircle
+// > 0 when d lies strictly inside the circle through the CCW triangle abc.
+template <int N = 96>
+SM_HD inline int incircle_exact(const double *a, const double *b, const double *c,
+                                const double *d) {
+  Exp<2> adx, ady, bdx, bdy, cdx, cdy;
+  from_diff(a[0], d[0], adx);
+  from_diff(a[1], d[1], ady);
+  from_diff(b[0], d[0], bdx);
+  from_diff(b[1], d[1], bdy);
+  from_diff(c[0], d[0], cdx);
+  from_diff(c[1], d[1], cdy);
+  Exp<8> t1, t2, nt;
+  Exp<16> bc, ca, ab, al, bl, cl;
+  mul(bdx, cdy, t1); mul(cdx, bdy, t2); negate(t2, nt); add(t1, nt, bc);
+  mul(cdx, ady, t1); mul(adx, cdy, t2); negate(t2, nt); add(t1, nt, ca);
+  mul(adx, bdy, t1); mul(bdx, ady, t2); negate(t2, nt); add(t1, nt, ab);
+  mul(adx, adx, t1); mul(ady, ady, t2); add(t1, t2, al);
+  mul(bdx, bdx, t1); mul(bdy, bdy, t2); add(t1, t2, bl);
+  mul(cdx, cdx, t1); mul(cdy, cdy, t2); add(t1, t2, cl);
+  Exp<N> p1, p2, p3, s12, det;
+  mul(al, bc, p1);
+  mul(bl, ca, p2);
+  mul(cl, ab, p3);
+  add(p1, p2, s12);
+  add(s12, p3, det);
+  return sign_of(det);
+}
+
+SM_HD inline int incircle(const double *a, const double *b, const double *c,
+                          const double *d) {
+  double adx = a[0] - d[0], bdx = b[0] - d[0], cdx = c[0] - d[0];
+  double ady = a[1] - d[1], bdy = b[1] - d[1], cdy = c[1] - d[1];
+  double bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+  double alift = adx * adx + ady * ady;
+  double cdxady = cdx * ady, adxcdy = adx * cdy;
+  double blift = bdx * bdx + bdy * bdy;
+  double adxbdy = adx * bdy, bdxady = bdx * ady;
+  double clift = cdx * cdx + cdy * cdy;
+  double det = alift * (bdxcdy - cdxbdy) + blift * (cdxady - adxcdy) +
+               clift * (adxbdy - bdxady);
+  double perm = (fabs(bdxcdy) + fabs(cdxbdy)) * alift +
+                (fabs(cdxady) + fabs(adxcdy)) * blift +
+                (fabs(adxbdy) + fabs(bdxady)) * clift;
+  double err = kIccBoundA * perm;
+  if (det > err) return 1;
+  if (-det > err) return -1;
+  return incircle_exact(a, b, c, d);
+}
+
+// Double-precision stage only: returns SM_PRED_UNRESOLVED when the sign is
+// not certain.  Device kernels use these and hand uncertain cases to a
+// separate exact pass.
+SM_HD inline int orient2d_filter(const double *a, const double *b, const double *c) {
+  double detleft = (a[0] - c[0]) * (b[1] - c[1]);
+  double detright = (a[1] - c[1]) * (b[0] - c[0]);
+  double det = detleft - detright;
+  double detsum;
+  if (detleft > 0.0) {
+    if (detright <= 0.0) return (det > 0.0) - (det < 0.0);
+    detsum = detleft + detright;
+  } else if (detleft < 0.0) {
+    if (detright >= 0.0) return (det > 0.0) - (det < 0.0);
+    detsum = -detleft - detright;
+  } else {
+    return (det > 0.0) - (det < 0.0);
+  }
+  double err = kCcwBoundA * detsum;
+  if (det > err) return 1;
+  if (-det > err) return -1;
+  return SM_PRED_UNRESOLVED;
+}
+
+SM_HD inline int incircle_filter(const double *a, const double *b, const double *c,
+                                 const double *d) {
+  double adx = a[0] - d[0], bdx = b[0] - d[0], cdx = c[0] - d[0];
+  double ady = a[1] - d[1], bdy = b[1] - d[1], cdy = c[1] - d[1];
+  double bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+  double alift = adx * adx + ady * ady;
+  double cdxady = cdx * ady, adxcdy = adx * cdy;
+  double blift = bdx * bdx + bdy * bdy;
+  double adxbdy = adx * bdy, bdxady = bdx * ady;
+  double clift = cdx * cdx + cdy * cdy;
+  double det = alift * (bdxcdy - cdxbdy) + blift * (cdxady - adxcdy) +
+               clift * (adxbdy - bdxady);
+  double perm = (fabs(bdxcdy) + fabs(cdxbdy)) * alift +
+                (fabs(cdxady) + fabs(adxcdy)) * blift +
+                (fabs(adxbdy) + fabs(bdxady)) * clift;
+  double err = kIccBoundA * perm;
+  if (det > err) return 1;
+  if (-det > err) return -1;
+  return SM_PRED_UNRESOLVED;
+}
+
+// ---------------------------------------------------------------- orient3d
+// > 0 when d lies below the plane of a, b, c (a, b, c CCW seen from above),
+// i.e. the sign of det[a-d; b-d; c-d].
+template <int N = 96>
+SM_HD inline int orient3d_exact(const double *a, const double *b, const double *c,
+                                const double *d) {
+  Exp<2> ad[3], bd[3], cd[3];
+  for (int k = 0; k < 3; ++k) {
+    from_diff(a[k], d[k], ad[k]);
+    from_diff(b[k], d[k], bd[k]);
+    from_diff(c[k], d[k], cd[k]);
+  }
+  // det = ad . (bd x cd)
+  Exp<8> t1, t2, nt;
+  Exp<16> m0, m1, m2;
+  mul(bd[1], cd[2], t1); mul(bd[2], cd[1], t2); negate(t2, nt); add(t1, nt, m0);
+  mul(bd[2], cd[0], t1); mul(bd[0], cd[2], t2); negate(t2, nt); add(t1, nt, m1);
+  mul(bd[0], cd[1], t1); mul(bd[1], cd[0], t2); negate(t2, nt); add(t1, nt, m2);
+  Exp<64> p0, p1, p2;
+  mul(ad[0], m0, p0);
+  mul(ad[1], m1, p1);
+  mul(ad[2], m2, p2);
+  Exp<N> s, det;
+  add(p0, p1, s);
+  add(s, p2, det);
+  return sign_of(det);
+}
+
+SM_HD inline double orient3d_det(const double *a, const double *b, const double *c,
+                                 const double *d, double &perm) {
+  double adx = a[0] - d[0], bdx = b[0] - d[0], cdx = c[0] - d[0];
+  double ady = a[1] - d[1], bdy = b[1] - d[1], cdy = c[1] - d[1];
+  double adz = a[2] - d[2], bdz = b[2] - d[2], cdz = c[2] - d[2];
+  double bdxcdy = bdx * cdy, cdxbdy = cdx * bdy;
+  double cdxady = cdx * ady, adxcdy = adx * cdy;
+  double adxbdy = adx * bdy, bdxady = bdx * ady;
+  double det = adz * (bdxcdy - cdxbdy) + bdz * (cdxady - adxcdy) + cdz * (adxbdy - bdxady);
+  perm = (fabs(bdxcdy) + fabs(cdxbdy)) * fabs(adz) +
+         (fabs(cdxady) + fabs(adxcdy)) * fabs(bdz) +
+         (fabs(adxbdy) + fabs(bdxady)) * fabs(cdz);
+  return det;
+}
+
+SM_HD inline int orient3d(const double *a, const double *b, const double *c,
+                          const double *d) {
+  double perm;
+  double det = orient3d_det(a, b, c, d, perm);
+  double err = kO3dBoundA * perm;
+  if (det > err) return 1;
+  if (-det > err) return -1;
+  return orient3d_exact(a, b, c, d);
+}
+
+// ---------------------------------------------------------------- insphere
+// > 0 when e lies strictly inside the sphere through a, b, c, d where
+// orient3d(a, b, c, d) > 0.
+template <int N = 160>
+SM_HD inline int insphere_exact(const double *a, const double *b, const double *c,
+                                const double *d, const double *e) {
+  const double *P[4] = {a, b, c, d};
+  Exp<2> v[4][3];
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 3; ++k) from_diff(P[i][k], e[k], v[i][k]);
+  Exp<24> lift[4];
+  for (int i = 0; i < 4; ++i) {
+    Exp<8> sx, sy, sz;
+    Exp<16> sxy;
+    mul(v[i][0], v[i][0], sx);
+    mul(v[i][1], v[i][1], sy);
+    mul(v[i][2], v[i][2], sz);
+    add(sx, sy, sxy);
+    add(sxy, sz, lift[i]);
+  }
+  // 2x2 minors of the xy columns: m[i][j] = x_i*y_j - x_j*y_i
+  Exp<16> m[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = i + 1; j < 4; ++j) {
+      Exp<8> t1, t2, nt;
+      mul(v[i][0], v[j][1], t1);
+      mul(v[j][0], v[i][1], t2);
+      negate(t2, nt);
+      add(t1, nt, m[i][j]);
+    }
+  // 3x3 minors with z: c3(i,j,k) = z_i*m[j][k] - z_j*m[i][k] + z_k*m[i][j]
+  auto c3 = [&](int i, int j, int k, Exp<N> &out) {
+    Exp<64> a1, a2, a3, na2;
+    Exp<N> s;
+    mul(v[i][2], m[j][k], a1);
+    mul(v[j][2], m[i][k], a2);
+    mul(v[k][2], m[i][j], a3);
+    negate(a2, na2);
+    add(a1, na2, s);
+    add(s, a3, out);
+  };
+  Exp<N> c012, c013, c023, c123;
+  c3(0, 1, 2, c012);
+  c3(0, 1, 3, c013);
+  c3(0, 2, 3, c023);
+  c3(1, 2, 3, c123);
+  // det4 = -l0*c123 + l1*c023 - l2*c013 + l3*c012
+  Exp<N> q0, q1, q2, q3, nq, acc, acc2;
+  mul(lift[0], c123, q0);
+  mul(lift[1], c023, q1);
+  mul(lift[2], c013, q2);
+  mul(lift[3], c012, q3);
+  negate(q0, nq);
+  add(nq, q1, acc);
+  negate(q2, nq);
+  add(acc, nq, acc2);
+  add(acc2, q3, acc);
+  // sign convention: positive when e is inside for a positively oriented tet
+  Exp<N> res;
+  negate(acc, res);
+  return sign_of(res);
+}
+
+// Double-precision stage: the 4x4 lifted determinant expanded along the
+// lift column, with a forward error bound of 24 eps times its permanent (the
+// evaluation tree is at most 10 roundings deep).
+SM_HD inline int insphere_filter(const double *a, const double *b, const double *c,
+                                 const double *d, const double *e) {
+  const double *P[4] = {a, b, c, d};
+  double x[4], y[4], z[4], l[4];
+  for (int i = 0; i < 4; ++i) {
+    x[i] = P[i][0] - e[0];
+    y[i] = P[i][1] - e[1];
+    z[i] = P[i][2] - e[2];
+    l[i] = x[i] * x[i] + y[i] * y[i] + z[i] * z[i];
+  }
+  double m[4][4], mp[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = i + 1; j < 4; ++j) {
+      double u = x[i] * y[j], v = x[j] * y[i];
+      m[i][j] = u - v;
+      mp[i][j] = fabs(u) + fabs(v);
+    }
+  auto c3 = [&](int i, int j, int k, double &perm) {
+    perm = fabs(z[i]) * mp[j][k] + fabs(z[j]) * mp[i][k] + fabs(z[k]) * mp[i][j];
+    return z[i] * m[j][k] - z[j] * m[i][k] + z[k] * m[i][j];
+  };
+  double p0, p1, p2, p3;
+  double c123 = c3(1, 2, 3, p0), c023 = c3(0, 2, 3, p1);
+  double c013 = c3(0, 1, 3, p2), c012 = c3(0, 1, 2, p3);
+  double det = (-l[0] * c123 + l[1] * c023) + (-l[2] * c013 + l[3] * c012);
+  double perm = (l[0] * p0 + l[1] * p1) + (l[2] * p2 + l[3] * p3);
+  double err = 24.0 * kEps * perm;
+  // det > 0 means e lies inside the sphere of a tet with orient3d < 0 in
+  // this file's orientation convention; report "inside" as positive for
+  // orient3d(a,b,c,d) > 0.
+  if (-det > err) return 1;
+  if (det > err) return -1;
+  return SM_PRED_UNRESOLVED;
+}
+
+SM_HD inline int insphere(const double *a, const double *b, const double *c,
+                          const double *d, const double *e) {
+  int s = insphere_filter(a, b, c, d, e);
+  if (s != SM_PRED_UNRESOLVED) return s;
+  return insphere_exact(a, b, c, d, e);
+}
+
+SM_HD inline int orient3d_filter(const double *a, const double *b, const double *c,
+                                 const double *d) {
+  double perm;
+  double det = orient3d_det(a, b, c, d, perm);
+  double err = kO3dBoundA * perm;
+  if (det > err) return 1;
+  if (-det > err) return -1;
+  return SM_PRED_UNRESOLVED;
+}
+
+// Host entry points: same filters, exact stage with room for any input a
+// double-precision coordinate set can produce in practice.
+inline int orient2d_h(const double *a, const double *b, const double *c) {
+  return orient2d(a, b, c);
+}
+inline int incircle_h(const double *a, const double *b, const double *c, const double *d) {
+  int s = incircle_filter(a, b, c, d);
+  if (s != SM_PRED_UNRESOLVED) return s;
+  s = incircle_exact<96>(a, b, c, d);
+  if (s != SM_PRED_UNRESOLVED) return s;
+  return incircle_exact<2048>(a, b, c, d);
+}
+inline int orient3d_h(const double *a, const double *b, const double *c, const double *d) {
+  int s = orient3d_filter(a, b, c, d);
+  if (s != SM_PRED_UNRESOLVED) return s;
+  s = orient3d_exact<96>(a, b, c, d);
+  if (s != SM_PRED_UNRESOLVED) return s;
+  return orient3d_exact<2048>(a, b, c, d);
+}
+inline int insphere_h(const double *a, const double *b, const double *c, const double *d,
+                      const double *e) {
+  int s = insphere_filter(a, b, c, d, e);
+  if (s != SM_PRED_UNRESOLVED) return s;
+  s = insphere_exact<160>(a, b, c, d, e);
+  if (s != SM_PRED_UNRESOLVED) return s;
+  return insphere_exact<4096>(a, b, c, d, e);
+}
+
+}  // namespace smpred

@@ -1,0 +1,364 @@
+"""generate_mesh / sliver_removal with the reference's signatures.
+
+Mirrors SeismicMesh/generation/mesh_generator.py (serial path): option
+parsing and defaults (:344-357, :624-652), domain/sizing unpacking
+(:562-621), fixed points and box corners (:425-430), the staggered initial
+lattice with seeded rejection (:808-852, generation/utils.py:15-25), the
+force loop (:460-527) and termination (:485-494, :655-677, :741-759).
+
+The force loop runs on the GPU through libsmgpu (one handle per call,
+csrc/mesher.hip): device Delaunay repair, centroid filter, sorted unique
+edges, bar forces with a deterministic gather, update and Newton
+projection.  Set-up (lattice, RNG) and termination (one-shot mesh clean-up)
+run on the host, as in the reference.
+
+Extension: ``precision`` (64 default, or 32) selects FP64 forces that
+reproduce the reference's float64 arithmetic, or FP32 forces.
+"""
+import ctypes
+import math
+import time
+import warnings
+
+import numpy as np
+
+from . import _lib
+from . import geometry as geo
+from . import mesh_utils as mu
+from .sizing import GridSizing, SizeFunction
+
+__all__ = ["generate_mesh", "sliver_removal", "Mesher2D"]
+
+_KNOWN = {
+    "verbose", "max_iter", "seed", "perform_checks", "pfix", "axis", "points", "domain",
+    "edge_length", "bbox", "min_dh_angle_bound", "max_dh_angle_bound", "delta_t", "h0",
+    "geps_mult", "subdomains", "gamma", "preserve", "mesh_improvement", "r0m_is_h0",
+    "precision",
+}
+
+
+def _parse_kwargs(kwargs):
+    for key in kwargs:
+        if key not in _KNOWN:
+            raise ValueError("Option %s with parameter %s not recognized " % (key, kwargs[key]))
+
+
+def _check_bbox(bbox):
+    if bbox is not None:
+        for b in bbox:
+            if isinstance(b, int):
+                raise ValueError("bbox must contain all floats")
+
+
+_DOMAINS = (geo.Ball, geo.Rectangle, geo.Disk, geo.Cube, geo.Cylinder, geo.Torus, geo.Prism,
+            geo.Union, geo.Intersection, geo.Difference, geo.Repeat)
+
+
+def _unpack_domain(domain, opts):
+    if isinstance(domain, _DOMAINS):
+        bbox, corners = domain.bbox, domain.corners
+    elif callable(domain):
+        raise NotImplementedError(
+            "domains given as plain Python functions cannot run on the GPU; build the "
+            "domain from seismicmesh_amd geometry objects (Disk, Rectangle, Union, ...)")
+    else:
+        raise ValueError("`domain` must be a function or a :class:`geometry` object")
+    _check_bbox(bbox)
+    return domain, bbox, corners
+
+
+class _Sizing:
+    """How the loop obtains h at edge midpoints."""
+
+    def __init__(self, kind, value=None, grid=None, func=None):
+        self.kind = kind  # "const" | "grid" | "host"
+        self.value = value
+        self.grid = grid
+        self.func = func
+
+    def __call__(self, x):
+        if self.kind == "const":
+            return np.array([self.value] * len(x))
+        if self.kind == "grid":
+            return self.grid(x)
+        return np.asarray(self.func(x), dtype=float)
+
+
+def _unpack_sizing(edge_length):
+    bbox = None
+    hmin = None
+    if isinstance(edge_length, SizeFunction):
+        bbox = edge_length.bbox
+        hmin = edge_length.hmin
+        lsf = True
+        if isinstance(edge_length.cell_size, GridSizing):
+            fh = _Sizing("grid", grid=edge_length.cell_size)
+        else:
+            fh = _Sizing("host", func=edge_length.eval)
+    elif callable(edge_length):
+        fh = _Sizing("host", func=edge_length)
+        lsf = False
+    elif np.isscalar(edge_length):
+        fh = _Sizing("const", value=edge_length)
+        hmin = edge_length
+        lsf = True
+    else:
+        raise ValueError(
+            "`edge_length` must either be a function, a `edge_length` object, or a scalar")
+    _check_bbox(bbox)
+    return fh, bbox, hmin, lsf
+
+
+def _minmax(b0, b1):
+    return tuple(max(a, b) if i % 2 else min(a, b) for i, (a, b) in enumerate(zip(b0, b1)))
+
+
+def create_staggered_grid(h0, dim, bbox):
+    """generation/utils.py:15-25: lattice of spacing h0, odd rows shifted."""
+    points = np.mgrid[tuple(slice(mn, mx + h0, h0) for mn, mx in bbox)].astype(float)
+    odd_rows = [i for i in range(points[0].shape[0]) if i % 2 != 0]
+    odd_cols = [i for i in range(points[1].shape[0]) if i % 2 != 0]
+    points[1][odd_rows] += h0 / 2
+    if dim == 3:
+        points[2][odd_cols] += h0 / 2
+    return points.reshape(dim, -1).T
+
+
+class Mesher2D:
+    """Owns one libsmgpu force-loop handle (2D)."""
+
+    def __init__(self, levels, sizing, h0, delta_t, geps, deps, nfix, precision=64, device=0):
+        _lib.require_gpu()
+        self._tapes = [geo.tape_array(l.tape()) for l in levels]
+        ptrs = (ctypes.c_void_p * len(levels))(*[ctypes.addressof(t) for t in self._tapes])
+        lens = (ctypes.c_int * len(levels))(*[len(t) for t in self._tapes])
+        prm = _lib.MesherParams()
+        prm.dim = 2
+        prm.precision = precision
+        prm.nfix = nfix
+        prm.nlevels = len(levels)
+        prm.h0 = h0
+        prm.delta_t = delta_t
+        prm.geps = geps
+        prm.deps = deps
+        prm.L0mult = 1 + 0.4 / 2 ** (2 - 1)
+        self._sizing = sizing
+        g = _lib.Grid()
+        g.dim = 2
+        cb = _lib.HOST_SIZING_FN()
+        if sizing.kind == "const":
+            g.hconst = float(sizing.value)
+        elif sizing.kind == "grid":
+            g = sizing.grid.grid_struct()
+        else:
+            def _cb(ctx, mid, n, h):
+                try:
+                    x = np.ctypeslib.as_array(mid, shape=(n, 2)).copy()
+                    out = np.ctypeslib.as_array(h, shape=(n,))
+                    out[:] = np.asarray(sizing.func(x), dtype=float).reshape(-1)
+                    return 0
+                except Exception:  # reported by the library as a failed callback
+                    return 1
+            cb = _lib.HOST_SIZING_FN(_cb)
+        self._cb = cb
+        self._grid = g
+        h = ctypes.c_void_p()
+        _lib.check(_lib.lib.sm_mesher_create(ctypes.byref(prm), ptrs, lens, ctypes.byref(g), cb,
+                                             None, device, ctypes.byref(h)), "sm_mesher_create")
+        self.h = h
+
+    def close(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and _lib is not None and _lib.lib is not None:
+            _lib.lib.sm_mesher_destroy(h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_points(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        self.n = len(p)
+        _lib.check(_lib.lib.sm_mesher_set_points(self.h, p.ctypes.data, len(p)),
+                   "sm_mesher_set_points")
+
+    def run(self, iters):
+        maxdp = ctypes.c_double(0.0)
+        _lib.check(_lib.lib.sm_mesher_run(self.h, int(iters), ctypes.byref(maxdp)), "sm_mesher_run")
+        return maxdp.value
+
+    def triangulate(self):
+        _lib.check(_lib.lib.sm_mesher_triangulate(self.h), "sm_mesher_triangulate")
+
+    def sizes(self):
+        n, c, e = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(_lib.lib.sm_mesher_sizes(self.h, ctypes.byref(n), ctypes.byref(c),
+                                            ctypes.byref(e)), "sm_mesher_sizes")
+        return n.value, c.value, e.value
+
+    def points(self):
+        n, _, _ = self.sizes()
+        p = np.empty((n, 2))
+        _lib.check(_lib.lib.sm_mesher_get_points(self.h, p.ctypes.data, n), "sm_mesher_get_points")
+        return p
+
+    def cells(self):
+        _, c, _ = self.sizes()
+        t = np.empty((c, 3), dtype=np.int32)
+        _lib.check(_lib.lib.sm_mesher_get_cells(self.h, t.ctypes.data, c), "sm_mesher_get_cells")
+        return t
+
+    def edges(self):
+        _, _, e = self.sizes()
+        out = np.empty((e, 2), dtype=np.int32)
+        _lib.check(_lib.lib.sm_mesher_get_edges(self.h, out.ctypes.data, e), "sm_mesher_get_edges")
+        return out
+
+    def stats(self):
+        s = (ctypes.c_int64 * 8)()
+        _lib.check(_lib.lib.sm_mesher_stats(self.h, s, 8), "sm_mesher_stats")
+        keys = ("iterations", "flip_rounds", "flips", "host_rebuilds", "exact_predicates")
+        return {k: int(s[i]) for i, k in enumerate(keys)}
+
+
+def _fd(domain):
+    return domain.eval
+
+
+def _remove_triangles_outside(p, t, domain, geps):
+    # mesh_generator.py:734-738
+    dim = p.shape[1]
+    pmid = p[t].sum(1) / (dim + 1)
+    return t[domain.eval(pmid) < -geps]
+
+
+def _improve_level_set_newton(p, t, domain, deps):
+    # mesh_generator.py:741-759 (five damped Newton steps on boundary vertices)
+    dim = p.shape[1]
+    bid = mu.get_boundary_vertices(t, dim)
+    alpha = 1
+    for it in range(5):
+        d = domain.eval(p[bid])
+        grads = []
+        for i in range(dim):
+            q = p[bid].copy()
+            q[:, i] = q[:, i] + deps
+            grads.append((domain.eval(q) - d) / deps)
+        g2 = sum(g ** 2 for g in grads)
+        g2 = np.where(g2 < deps, deps, g2)
+        p[bid] -= alpha * (d * np.vstack(grads) / g2).T
+        alpha /= it + 1
+    return p
+
+
+def _termination(p, t, opts, verbose=1):
+    # mesh_generator.py:655-677 (serial)
+    dim = p.shape[1]
+    if dim == 2:
+        p, t, _ = mu.fix_mesh(p, t, dim=dim, delete_unused=True)
+        p, t = mu.delete_boundary_entities(p, t, dim=2, min_qual=0.15, verbose=verbose)
+        if opts["subdomains"] is None and opts["mesh_improvement"]:
+            p, t = mu.laplacian2_fixed_point(p, t)
+    p, t, _ = mu.fix_mesh(p, t, dim=dim, delete_unused=True)
+    return p, t
+
+
+def generate_mesh(domain, edge_length, comm=None, **kwargs):
+    """Generate a 2D mesh: see mesh_generator.py:291-529 for the semantics."""
+    opts = {
+        "verbose": 1, "max_iter": 50, "seed": 0, "perform_checks": False, "pfix": None,
+        "axis": 1, "points": None, "delta_t": 0.30, "geps_mult": 0.1, "subdomains": None,
+        "mesh_improvement": True, "r0m_is_h0": False, "precision": 64,
+    }
+    opts.update(kwargs)
+    _parse_kwargs(kwargs)
+    if opts["verbose"] < 0:
+        raise ValueError("Unknown verbosity level")
+    v1 = opts["verbose"] >= 1
+    v2 = opts["verbose"] > 1
+
+    dom, bbox0, corners = _unpack_domain(domain, opts)
+    fh, bbox1, hmin, lsf = _unpack_sizing(edge_length)
+    bbox = bbox0 if bbox1 is None else _minmax(bbox0, bbox1)
+    if not isinstance(bbox, tuple):
+        raise ValueError("`bbox` must be a tuple")
+    dim = int(len(bbox) / 2)
+    if dim != 2:
+        raise NotImplementedError("the device force loop meshes 2D domains")
+    if bbox1 is not None and bbox0 != bbox1:
+        dom = geo.Rectangle(bbox)
+    bbox = np.array(bbox).reshape(-1, 2)
+    h0 = hmin if hmin is not None else opts["h0"]
+    if h0 < 0:
+        raise ValueError("`h0` must be > 0")
+    delta_t = opts["delta_t"]
+    geps = opts["geps_mult"] * h0
+    deps = np.sqrt(np.finfo(np.double).eps) * h0
+
+    if opts["pfix"] is not None:
+        pfix = np.array(opts["pfix"], dtype="d")
+    else:
+        pfix = np.empty((0, dim))
+    if corners is not None:
+        corners = corners[dom.eval(corners) > -1000 * deps]
+        pfix = np.append(pfix, corners, axis=0)
+    nfix = len(pfix)
+    if v1:
+        print(f"Constraining {nfix} fixed points...", flush=True)
+
+    if opts["points"] is None:
+        p = create_staggered_grid(h0, dim, bbox)
+        p = p[dom.eval(p) < geps]
+        r0 = fh(p)
+        if opts["r0m_is_h0"]:
+            r0m = 1.1 * h0 if 1.1 * h0 < r0.min() else h0
+        else:
+            r0m = r0.min()
+        np.random.seed(opts["seed"])
+        p = np.vstack((pfix, p[np.random.rand(p.shape[0]) < r0m ** dim / r0 ** dim]))
+    else:
+        p = np.array(opts["points"], dtype=float)
+
+    if opts["max_iter"] < 0:
+        raise ValueError("`max_iter` must be > 0")
+    max_iter = opts["max_iter"]
+    N = p.shape[0]
+    assert N > 0, "No vertices to mesh with!"
+    if v1:
+        print("Commencing mesh generation with %d vertices on rank %d." % (N, 0), flush=True)
+
+    levels = [dom]
+    if opts["subdomains"] is not None:
+        for sd in opts["subdomains"]:
+            levels.append(_unpack_domain(sd, opts)[0])
+
+    m = Mesher2D(levels, fh, h0, delta_t, geps, deps, nfix, precision=opts["precision"])
+    m.set_points(p)
+    updates = max(max_iter - 1, 0)
+    if v2:
+        for count in range(updates):
+            start = time.time()
+            maxdp = m.run(1)
+            n, c, _ = m.sizes()
+            print("Iteration #%d, max movement is %f, there are %d vertices and %d cells"
+                  % (count + 1, maxdp, n, c), flush=True)
+            print("     Elapsed wall-clock time %f : " % (time.time() - start), flush=True)
+    else:
+        m.run(updates)
+    m.triangulate()
+    p, t = m.points(), m.cells()
+    if v1:
+        print("Termination reached...maximum number of iterations reached.", flush=True)
+    p, t = _termination(p, t, opts, verbose=opts["verbose"])
+    p = _improve_level_set_newton(p, t, dom, deps)
+    t = _remove_triangles_outside(p, t, dom, h0 * 0.001)
+    return p, t
+
+
+def sliver_removal(points, domain, edge_length, comm=None, **kwargs):
+    """3D sliver removal (mesh_generator.py:59-288)."""
+    from .sliver import sliver_removal as _impl
+    return _impl(points, domain, edge_length, comm=comm, **kwargs)

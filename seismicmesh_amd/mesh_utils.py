@@ -1,0 +1,188 @@
+"""Mesh utilities of the reference's geometry module (host, one-shot).
+
+Mirrors SeismicMesh/geometry/utils.py: simp_vol (:175-201), fix_mesh
+(:204-249), simp_qual (:252-274), edge/facet/boundary helpers (:277-437),
+delete_boundary_entities (:440-469) and laplacian2_fixed_point (:494-547).
+These run once, after the force loop (termination), on the host; the
+Laplacian system is solved directly (the reference uses pyamg's
+Ruge-Stuben solver on the same matrix).
+"""
+import numpy as np
+
+__all__ = [
+    "simp_vol", "simp_qual", "fix_mesh", "unique_rows", "get_edges", "get_facets",
+    "get_boundary_edges", "get_boundary_facets", "get_boundary_vertices",
+    "vertex_to_entities", "get_boundary_entities", "delete_boundary_entities",
+    "laplacian2_fixed_point", "get_centroids",
+]
+
+
+def simp_vol(p, t):
+    dim = p.shape[1]
+    if dim == 2:
+        d01 = p[t[:, 1]] - p[t[:, 0]]
+        d02 = p[t[:, 2]] - p[t[:, 0]]
+        return (d01[:, 0] * d02[:, 1] - d01[:, 1] * d02[:, 0]) / 2
+    if dim == 3:
+        d01 = p[t[:, 1]] - p[t[:, 0]]
+        d02 = p[t[:, 2]] - p[t[:, 0]]
+        d03 = p[t[:, 3]] - p[t[:, 0]]
+        return np.einsum("ij,ij->i", np.cross(d01, d02), d03) / 6
+    raise NotImplementedError
+
+
+def simp_qual(p, t):
+    """2 * inradius / circumradius of triangles (1 = equilateral)."""
+    assert p.ndim == 2 and t.ndim == 2 and p.shape[1] + 1 == t.shape[1]
+
+    def length(a):
+        return np.sqrt((a ** 2).sum(1))
+
+    a = length(p[t[:, 1]] - p[t[:, 0]])
+    b = length(p[t[:, 2]] - p[t[:, 0]])
+    c = length(p[t[:, 2]] - p[t[:, 1]])
+    r = 0.5 * np.sqrt((b + c - a) * (c + a - b) * (a + b - c) / (a + b + c))
+    R = a * b * c / np.sqrt((a + b + c) * (b + c - a) * (c + a - b) * (a + b - c))
+    return 2 * r / R
+
+
+def unique_rows(A, return_index=False, return_inverse=False):
+    """Unique rows compared as raw bytes (the reference's row view)."""
+    A = np.require(A, requirements="C")
+    assert A.ndim == 2, "array must be 2-dim'l"
+    dt = np.dtype("S%d" % (A.dtype.itemsize * A.shape[1]))
+    B, I, J = np.unique(A.view(dt), return_index=True, return_inverse=True)
+    B = B.view(A.dtype).reshape((-1, A.shape[1]))
+    J = J.reshape(-1)
+    out = (B,)
+    if return_index:
+        out += (I,)
+    if return_inverse:
+        out += (J,)
+    return out if len(out) > 1 else out[0]
+
+
+def _unique_row_view(data):
+    b = np.ascontiguousarray(data).view(np.dtype((np.void, data.dtype.itemsize * data.shape[1])))
+    u, cnts = np.unique(b, return_counts=True)
+    return u.view(data.dtype).reshape(-1, data.shape[1]), cnts
+
+
+def fix_mesh(p, t, ptol=2e-13, dim=2, delete_unused=False, fix_orientation=True):
+    """Merge duplicate vertices/cells, drop unused vertices, orient cells."""
+    snap = (p.max(0) - p.min(0)).max() * ptol
+    _, ix, jx = unique_rows(np.round(p / snap) * snap, True, True)
+    p = p[ix]
+    t = jx[t]
+    t = np.sort(t, axis=1)
+    t, _ = _unique_row_view(t)
+    if delete_unused:
+        pix, _, jx = np.unique(t, return_index=True, return_inverse=True)
+        t = np.reshape(jx, t.shape)
+        p = p[pix, :]
+    if fix_orientation:
+        flip = simp_vol(p, t) < 0
+        t[flip, :2] = t[flip, 1::-1]
+    return p, t, jx
+
+
+def get_centroids(p, t):
+    dim = p.shape[1]
+    return p[t].sum(1) / (dim + 1)
+
+
+def get_edges(t, dim=2):
+    t = np.asarray(t)
+    if dim == 2:
+        return t[:, [[0, 1], [0, 2], [1, 2]]].reshape(-1, 2)
+    return t[:, [[0, 1], [1, 2], [2, 0], [0, 3], [1, 3], [2, 3]]].reshape(-1, 2)
+
+
+def get_facets(t):
+    ix = [[0, 1, 3], [1, 2, 3], [2, 0, 3], [1, 2, 0]]
+    return np.array(t[:, ix]).reshape((len(t) * 4, 3))
+
+
+def get_boundary_edges(t, dim=2):
+    e = np.sort(get_edges(t, dim=dim), axis=1)
+    unq, cnt = _unique_row_view(e)
+    return unq[cnt == (dim - 1)]
+
+
+def get_boundary_facets(t):
+    if t.shape[1] < 4:
+        raise ValueError("Only works for triangles")
+    f = np.sort(get_facets(t), axis=1)
+    unq, cnt = _unique_row_view(f)
+    return unq[cnt == 1]
+
+
+def get_boundary_vertices(t, dim=2):
+    if dim == 2:
+        b = get_boundary_edges(t)
+    elif dim == 3:
+        b = get_boundary_facets(t)
+    else:
+        raise ValueError("Dimension not supported.")
+    return np.unique(b.reshape(-1))
+
+
+def vertex_to_entities(p, t, dim=2):
+    ne = len(t)
+    ext = np.tile(np.arange(0, ne), (dim + 1, 1)).T.reshape((-1, 1), order="C")
+    ve = np.hstack((np.reshape(t, (-1, 1)), ext))
+    ve = ve[ve[:, 0].argsort(), :]
+    idx = np.insert(np.diff(ve[:, 0]), 0, 0)
+    ptr = np.argwhere(idx)
+    ptr = np.insert(ptr, 0, 0)
+    ptr = np.append(ptr, ne * (dim + 1))
+    return ve[:, 1], ptr
+
+
+def get_boundary_entities(p, t, dim=2):
+    bv = get_boundary_vertices(t, dim=dim)
+    vtoe, ptr = vertex_to_entities(p, t, dim=dim)
+    parts = [vtoe[ptr[v]:ptr[v + 1]] for v in bv]
+    bele = np.concatenate(parts) if parts else np.array([], dtype=int)
+    return np.unique(bele)
+
+
+def delete_boundary_entities(p, t, dim=2, min_qual=0.10, verbose=1):
+    qual = simp_qual(p, t)
+    bele = get_boundary_entities(p, t, dim=dim)
+    delete = qual[bele] < min_qual
+    if verbose:
+        print("Deleting " + str(np.sum(delete)) + " poor quality boundary entities...", flush=True)
+    delete = np.argwhere(delete == 1)
+    t = np.delete(t, bele[delete], axis=0)
+    p, t, _ = fix_mesh(p, t, delete_unused=True, dim=dim)
+    return p, t
+
+
+def laplacian2_fixed_point(p, t):
+    """Move interior vertices to the solution of the graph Laplace equation
+    with boundary vertices fixed (one direct sparse solve per coordinate)."""
+    import scipy.sparse as sp
+    from scipy.sparse.linalg import splu
+
+    if p.ndim != 2:
+        raise NotImplementedError("Laplacian smoothing only works in 2D for now")
+    n = len(p)
+    nds = t.T
+    local = np.array([[1, 2], [2, 0], [0, 1]]).T
+    idx = nds[local]
+    row = np.array([idx[0], idx[1], idx[0], idx[1]]).ravel()
+    col = np.array([idx[0], idx[1], idx[1], idx[0]]).ravel()
+    a = np.ones(idx.shape[1:], dtype=float)
+    val = np.array([+a, +a, -a, -a]).ravel()
+    m = sp.coo_matrix((val, (row, col)), shape=(n, n)).tocsr()
+    bnd = get_boundary_vertices(t)
+    for b in bnd:
+        m.data[m.indptr[b]:m.indptr[b + 1]] = 0.0
+    d = m.diagonal()
+    d[bnd] = 1.0
+    m.setdiag(d)
+    rhs = np.zeros((n, 2))
+    rhs[bnd] = p[bnd]
+    lu = splu(m.tocsc())
+    return np.column_stack([lu.solve(rhs[:, 0]), lu.solve(rhs[:, 1])]), t
