@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Throughput of the DistMesh force loop on one MI355X.
+
+Workload (default): the BP2004-shaped 2D velocity-model mesh -- a
+1911 x 5395 FP32 velocity grid over z in [-12 km, 0], x in [0, 67 km],
+sizing v / (2 Hz * 10) clamped to 75 m and gradient-limited at grade 0.15,
+on the Rectangle domain with its 4 corners fixed (seismicmesh_amd.workloads).
+
+A step is one force iteration over the whole point set: device Delaunay
+repair, centroid filter, sorted unique edges, bar forces, gather, update
+and Newton projection.  The timed region is `steps` iterations starting
+from the initial points (the reference's loop performs max_iter - 1 = 49 of
+them); initialisation (lattice, first triangulation) and termination are
+outside it, as in the metric's definition.
+
+Output: one JSON line with the metric, the roofline of the dominant kernel
+phase (HIP events on the loop's own stream) and a CPU baseline (the numpy
+oracle on a bounded sample of the same workload, 1 core).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def alg_bytes(phase, N, E, C, T, rounds=1):
+    """Algorithmic HBM bytes of one launch of a phase (2D, FP64 points,
+    FP32 per-edge arrays, int32 ids).  Each datum is counted once:
+    gathers of point coordinates count 16 B per referenced vertex."""
+    if phase == "edge_geom":
+        # edge ids 8 + both endpoints 32 + 4 sizing taps 16 + L, h out 8
+        return E * (8 + 32 + 16 + 8)
+    if phase == "edge_force":
+        # edge ids 8 + L, h 8 + both endpoints 32 + force vector out 8
+        return E * (8 + 8 + 32 + 8)
+    if phase == "vertex_update":
+        # offsets 16 + point in/out 32 per vertex; per edge its force vector
+        # read from both endpoints 16 and the second-endpoint list entry 4
+        return N * (16 + 32) + E * (16 + 4)
+    if phase == "unique_edges":
+        # cells read twice 24 + 3 pairs written/read 24 per cell, edges out 8
+        return C * (24 + 24) + E * 8 + N * 16
+    if phase == "centroid_filter":
+        # triangle ids 12 + 3 vertices 48 + keep flag 1
+        return T * (12 + 48 + 1)
+    if phase == "flip_round":
+        # orientation, mark, apply and fix passes over tri/adj: ~4 x 24 B
+        # per triangle + 3 vertices 48 B
+        return T * (4 * 24 + 48)
+    return 0
+
+
+def cpu_baseline(st, p0, iters):
+    """The oracle (numpy FP64 restatement + scipy Qhull) on `iters` force
+    iterations of the same workload, single thread."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:  # pragma: no cover
+        threadpool_limits = None
+    from oracle import distmesh as odm
+    from oracle import geometry as og
+    from oracle.sizing import build_interpolant
+
+    dom = st["dom"]
+    if hasattr(dom, "bbox0") and type(dom).__name__ == "Rectangle":
+        odom = og.Rectangle(dom.bbox0)
+    else:
+        odom = og.Disk([dom.xc, dom.yc], dom.r)
+    fh = st["fh"]
+    if fh.kind == "grid":
+        rgi = build_interpolant(fh.grid.values.astype(np.float64), fh.grid.bbox)
+        ofh = rgi
+    else:
+        val = fh.value
+        ofh = lambda x: np.array([val] * len(x))  # noqa: E731
+    s = dict(fd=odom.eval, fh=ofh, h0=st["h0"], dim=2, L0mult=1.2, geps=st["geps"],
+             deps=st["deps"], pfix=p0[:st["nfix"]], levels=[odom.eval], delta_t=st["delta_t"])
+    p = p0.copy()
+    ctx = threadpool_limits(limits=1) if threadpool_limits else None
+    if ctx:
+        ctx.__enter__()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        t_all = odm.delaunay(p)
+        p, _, _ = odm.force_step(p, t_all, s, list(range(st["nfix"])))
+    dt = time.perf_counter() - t0
+    if ctx:
+        ctx.__exit__(None, None, None)
+    return len(p0) * iters / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=49)
+    ap.add_argument("--warmup", type=int, default=49)
+    ap.add_argument("--workload", default="bp2004", choices=["bp2004", "disk"])
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
+    ap.add_argument("--cpu-iters", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(requested --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch  # noqa: F401  (device runtime)
+    from seismicmesh_amd import generation, workloads
+
+    if args.workload == "bp2004":
+        domain, ef, info = workloads.bp2004(scale=args.scale)
+        wl = {"workload": "bp2004_2d", "grid": list(info["grid"]), "bbox_m": list(info["bbox"]),
+              "hmin_m": 75.0, "grade": 0.15, "max_iter": 50}
+    else:
+        domain, ef, info = workloads.disk()
+        wl = {"workload": "unit_disk_h0.02", "max_iter": 50}
+    st = generation.prepare(domain, ef, verbose=0, precision=args.precision)
+    p0 = st["p"]
+    N = len(p0)
+    m = generation.make_mesher(st)
+
+    # warm-up: a full loop from the initial points (allocations, code objects)
+    m.set_points(p0)
+    if args.warmup > 0:
+        m.run(args.warmup)
+    # timed: `steps` iterations from the initial points
+    m.set_points(p0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    m.run(args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    stats = m.stats()
+    n, C, E = m.sizes()
+    value = N * args.steps / elapsed
+
+    # same region again with per-phase HIP events on the loop's stream
+    m.set_points(p0)
+    m.profile(True)
+    m.run(args.steps)
+    prof = m.profile_read()
+    m.profile(False)
+    T = 2 * N
+    best = max(prof, key=lambda k: prof[k][0])
+    ms_tot, launches = prof[best]
+    avg_ms = ms_tot / max(launches, 1)
+    traffic_alg = alg_bytes(best, N, E, C, T)
+    achieved = traffic_alg / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": best, "achieved": round(achieved, 2),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 3),
+                "alg_bytes_per_launch": traffic_alg,
+                "phases_ms": {k: round(v[0], 3) for k, v in prof.items()}}
+
+    cpu = None
+    if not args.no_cpu:
+        rate, secs = cpu_baseline(st, p0, args.cpu_iters)
+        cpu = {"value": round(rate, 1), "unit": "vertex-updates/s", "cores": 1, "kind": "port",
+               "sample": "%d force iterations of the same workload (N=%d) with the numpy "
+                         "oracle + scipy Qhull, %.1f s" % (args.cpu_iters, N, secs)}
+
+    out = {
+        "metric": "vertex-updates/s on 1 MI355X, fused+neural vs plain GPU at equal min quality",
+        "value": round(value, 1),
+        "unit": "vertex-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == 32 else "f64",
+        "data": "synthetic",
+        "config": dict(wl, N=N, edges=E, cells=C, path="device flips + fused loop (plain, no graph)",
+                       host_rebuilds=stats["host_rebuilds"], flip_rounds=stats["flip_rounds"]),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

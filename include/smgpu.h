@@ -183,6 +183,14 @@ int sm_mesher_get_edges(sm_mesher *m, int32_t *edges, int64_t cap);
 /* Statistics since creation: [0] iterations, [1] flip rounds, [2] flips,
  * [3] host rebuilds, [4] exact predicate fallbacks. */
 int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats);
+/* Per-phase kernel timing with HIP events recorded on the handle's stream.
+ * Phases: 0 flip rounds, 1 centroid filter, 2 unique edges + edge lists,
+ * 3 bar lengths and sizing, 4 L0 sums, 5 bar forces, 6 gather + update +
+ * projection.  sm_mesher_profile(m, 1) clears and enables;
+ * sm_mesher_profile_read fills ms[i] (total) and launches[i]. */
+#define SM_NPHASES 8
+int sm_mesher_profile(sm_mesher *m, int enable);
+int sm_mesher_profile_read(sm_mesher *m, double *ms, int64_t *launches, int nphases);
 /* Device pointers of the state, for zero-copy access from the host side. */
 int sm_mesher_device_points(sm_mesher *m, double **p);
 /* Stream used by the handle (hipStream_t as void*). */

@@ -99,6 +99,8 @@ _PROTOS = {
     "sm_mesher_get_cells": (_int, [_vp, _vp, _i64]),
     "sm_mesher_get_edges": (_int, [_vp, _vp, _i64]),
     "sm_mesher_stats": (_int, [_vp, _vp, _int]),
+    "sm_mesher_profile": (_int, [_vp, _int]),
+    "sm_mesher_profile_read": (_int, [_vp, _vp, _vp, _int]),
     "sm_mesher_device_points": (_int, [_vp, ctypes.POINTER(_vp)]),
     "sm_mesher_stream": (_int, [_vp, ctypes.POINTER(_vp)]),
     "sm_host_delaunay2d": (_i64, [_vp, _i64, _vp, _vp, _i64, _vp]),

@@ -272,6 +272,7 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
     int rc;
     if ((rc = fill<int32_t>(m->d_ctr, 8, 0, s))) return rc;
     if ((rc = fill<uint64_t>(m->d_claim, T, 0ull, s))) return rc;
+    const int pb = prof_begin(m);
     hipLaunchKernelGGL(k_orient, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_p, m->d_ori,
                        m->d_queue, m->d_ctr);
     SM_LAUNCH_CHECK();
@@ -290,6 +291,7 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
     hipLaunchKernelGGL(k_fix, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_partner,
                        m->d_mark, round);
     SM_LAUNCH_CHECK();
+    prof_end(m, 0, pb);
     SM_HIP(hipMemcpyAsync(m->h_ctr, m->d_ctr, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SM_HIP(hipStreamSynchronize(s));
     const int flips = m->h_ctr[0], inverted = m->h_ctr[1];

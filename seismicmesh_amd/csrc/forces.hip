@@ -417,6 +417,7 @@ static int forces_t(sm_mesher *m) {
   const bool exact = m->prm.precision == 64;
   const unsigned ge = blocks_for(E, B);
   GridView g = m->grid;
+  int pb = prof_begin(m);
   if (E > 0) {
     if (exact)
       hipLaunchKernelGGL((k_edge_geom<DIM, T, true>), dim3(ge), dim3(B), 0, s, E, m->d_edges,
@@ -428,6 +429,8 @@ static int forces_t(sm_mesher *m) {
                          m->d_hp, m->d_part);
     SM_LAUNCH_CHECK();
   }
+  prof_end(m, 3, pb);
+  pb = prof_begin(m);
   if (exact) {
     int64_t nch = (E + kNpChunk - 1) / kNpChunk;
     if (nch < 1) nch = 1;
@@ -453,19 +456,24 @@ static int forces_t(sm_mesher *m) {
                        m->d_sums, DIM);
     SM_LAUNCH_CHECK();
   }
+  prof_end(m, 4, pb);
+  pb = prof_begin(m);
   if (E > 0) {
     hipLaunchKernelGGL((k_edge_force<DIM, T>), dim3(ge), dim3(B), 0, s, E, m->d_edges, m->d_p,
                        (const T *)m->d_L, (const T *)m->d_h, m->d_sums, m->prm.L0mult,
                        (T *)m->d_F);
     SM_LAUNCH_CHECK();
   }
+  prof_end(m, 5, pb);
   int rc = fill<uint64_t>(m->d_maxF, 1, 0ull, s);
   if (rc) return rc;
+  pb = prof_begin(m);
   hipLaunchKernelGGL((k_vertex_update<DIM, T>), dim3(blocks_for(N, B)), dim3(B), 0, s, N,
                      m->d_p, eptr_of(m), m->d_sptr, m->d_slist, (const T *)m->d_F,
                      m->prm.nfix, m->prm.delta_t, m->d_tape_all, m->d_tape_off_dev,
                      m->d_tape_len_dev, m->prm.nlevels, m->prm.deps, m->prm.h0, m->d_maxF);
   SM_LAUNCH_CHECK();
+  prof_end(m, 6, pb);
   return SM_OK;
 }
 

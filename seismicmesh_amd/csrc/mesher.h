@@ -70,9 +70,24 @@ struct sm_mesher {
   int round = 0;
   bool have_tri = false;
   double last_maxdp = 0.0;
+
+  // optional per-phase timing with HIP events on the handle's stream
+  bool prof = false;
+  std::vector<hipEvent_t> ev_pool;
+  struct Rec {
+    int phase;
+    int a, b;  // indices into ev_pool
+  };
+  std::vector<Rec> recs;
+  double prof_ms[SM_NPHASES] = {};
+  int64_t prof_n[SM_NPHASES] = {};
 };
 
 namespace smk {
+// event-timed phases (no-ops unless profiling is enabled)
+int prof_begin(sm_mesher *m);
+void prof_end(sm_mesher *m, int phase, int begin_idx);
+int prof_drain(sm_mesher *m);
 // flip2d.hip
 int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild);
 int host_rebuild2d(sm_mesher *m);

@@ -131,10 +131,71 @@ int triangulate(sm_mesher *m) {
     int rc = host_rebuild2d(m);
     if (rc) return rc;
   }
-  return centroid_keep(m);
+  int b = prof_begin(m);
+  int rc = centroid_keep(m);
+  prof_end(m, 1, b);
+  return rc;
 }
 
 }  // namespace
+
+namespace smk {
+
+int prof_begin(sm_mesher *m) {
+  if (!m->prof) return -1;
+  int idx = (int)m->recs.size() * 2;
+  while ((int)m->ev_pool.size() < idx + 2) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return -1;
+    m->ev_pool.push_back(e);
+  }
+  (void)hipEventRecord(m->ev_pool[idx], m->stream);
+  return idx;
+}
+
+void prof_end(sm_mesher *m, int phase, int begin_idx) {
+  if (!m->prof || begin_idx < 0) return;
+  (void)hipEventRecord(m->ev_pool[begin_idx + 1], m->stream);
+  m->recs.push_back({phase, begin_idx, begin_idx + 1});
+}
+
+int prof_drain(sm_mesher *m) {
+  if (!m->prof || m->recs.empty()) return SM_OK;
+  SM_HIP(hipStreamSynchronize(m->stream));
+  for (const auto &r : m->recs) {
+    float ms = 0.f;
+    SM_HIP(hipEventElapsedTime(&ms, m->ev_pool[r.a], m->ev_pool[r.b]));
+    m->prof_ms[r.phase] += ms;
+    m->prof_n[r.phase] += 1;
+  }
+  m->recs.clear();
+  return SM_OK;
+}
+
+}  // namespace smk
+
+extern "C" int sm_mesher_profile(sm_mesher *m, int enable) {
+  SM_CHECK(m, SM_ERR_ARG, "mesher_profile: null handle");
+  int rc = prof_drain(m);
+  if (rc) return rc;
+  m->prof = enable != 0;
+  for (int i = 0; i < SM_NPHASES; ++i) {
+    m->prof_ms[i] = 0.0;
+    m->prof_n[i] = 0;
+  }
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_profile_read(sm_mesher *m, double *ms, int64_t *launches, int n) {
+  SM_CHECK(m && ms && launches, SM_ERR_ARG, "mesher_profile_read: null argument");
+  int rc = prof_drain(m);
+  if (rc) return rc;
+  for (int i = 0; i < n && i < SM_NPHASES; ++i) {
+    ms[i] = m->prof_ms[i];
+    launches[i] = m->prof_n[i];
+  }
+  return SM_OK;
+}
 
 extern "C" int sm_mesher_create(const sm_mesher_params *prm, const sm_node *const *tapes,
                                 const int *tape_lens, const sm_grid *sizing,
@@ -219,6 +280,7 @@ extern "C" int sm_mesher_destroy(sm_mesher *m) {
   if (m->h_ctr) (void)hipHostFree(m->h_ctr);
   if (m->h_sums) (void)hipHostFree(m->h_sums);
   if (m->h_maxF) (void)hipHostFree(m->h_maxF);
+  for (hipEvent_t e : m->ev_pool) (void)hipEventDestroy(e);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return SM_OK;
@@ -267,7 +329,9 @@ extern "C" int sm_mesher_run(sm_mesher *m, int iters, double *maxdp_out) {
                             "max movement indicates there's a convergence problem (%g)",
                             m->last_maxdp);
     }
+    int pb = prof_begin(m);
     if ((rc = build_edges(m))) return rc;
+    prof_end(m, 2, pb);
     if (m->host_sizing && (rc = host_sizing_pass(m))) return rc;
     if ((rc = forces_and_update(m))) return rc;
     SM_HIP(hipMemcpyAsync(m->h_maxF, m->d_maxF, sizeof(uint64_t), hipMemcpyDeviceToHost,
@@ -285,7 +349,7 @@ extern "C" int sm_mesher_run(sm_mesher *m, int iters, double *maxdp_out) {
     }
   }
   if (maxdp_out) *maxdp_out = m->last_maxdp;
-  return SM_OK;
+  return prof_drain(m);
 }
 
 extern "C" int sm_mesher_sizes(sm_mesher *m, int64_t *npoints, int64_t *ncells,

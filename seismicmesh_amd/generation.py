@@ -217,6 +217,19 @@ class Mesher2D:
         _lib.check(_lib.lib.sm_mesher_get_edges(self.h, out.ctypes.data, e), "sm_mesher_get_edges")
         return out
 
+    PHASES = ("flip_round", "centroid_filter", "unique_edges", "edge_geom", "l0_sums",
+              "edge_force", "vertex_update")
+
+    def profile(self, enable=True):
+        """Enable (and clear) per-phase HIP-event timing on the handle's stream."""
+        _lib.check(_lib.lib.sm_mesher_profile(self.h, int(enable)), "sm_mesher_profile")
+
+    def profile_read(self):
+        ms = (ctypes.c_double * 8)()
+        n = (ctypes.c_int64 * 8)()
+        _lib.check(_lib.lib.sm_mesher_profile_read(self.h, ms, n, 8), "sm_mesher_profile_read")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.PHASES)}
+
     def stats(self):
         s = (ctypes.c_int64 * 8)()
         _lib.check(_lib.lib.sm_mesher_stats(self.h, s, 8), "sm_mesher_stats")
@@ -266,8 +279,9 @@ def _termination(p, t, opts, verbose=1):
     return p, t
 
 
-def generate_mesh(domain, edge_length, comm=None, **kwargs):
-    """Generate a 2D mesh: see mesh_generator.py:291-529 for the semantics."""
+def prepare(domain, edge_length, **kwargs):
+    """The set-up half of generate_mesh (mesh_generator.py:343-458): option
+    checks, domain and sizing unpacking, fixed points, initial points."""
     opts = {
         "verbose": 1, "max_iter": 50, "seed": 0, "perform_checks": False, "pfix": None,
         "axis": 1, "points": None, "delta_t": 0.30, "geps_mult": 0.1, "subdomains": None,
@@ -335,9 +349,25 @@ def generate_mesh(domain, edge_length, comm=None, **kwargs):
         for sd in opts["subdomains"]:
             levels.append(_unpack_domain(sd, opts)[0])
 
-    m = Mesher2D(levels, fh, h0, delta_t, geps, deps, nfix, precision=opts["precision"])
-    m.set_points(p)
-    updates = max(max_iter - 1, 0)
+    return dict(opts=opts, dom=dom, fh=fh, h0=h0, delta_t=delta_t, geps=geps, deps=deps,
+                nfix=nfix, p=p, levels=levels, max_iter=max_iter, dim=dim)
+
+
+def make_mesher(st):
+    """A force-loop handle for a prepared problem (initial points not set)."""
+    return Mesher2D(st["levels"], st["fh"], st["h0"], st["delta_t"], st["geps"], st["deps"],
+                    st["nfix"], precision=st["opts"]["precision"])
+
+
+def generate_mesh(domain, edge_length, comm=None, **kwargs):
+    """Generate a 2D mesh: see mesh_generator.py:291-529 for the semantics."""
+    st = prepare(domain, edge_length, **kwargs)
+    opts, dom, h0, deps = st["opts"], st["dom"], st["h0"], st["deps"]
+    v1 = opts["verbose"] >= 1
+    v2 = opts["verbose"] > 1
+    m = make_mesher(st)
+    m.set_points(st["p"])
+    updates = max(st["max_iter"] - 1, 0)
     if v2:
         for count in range(updates):
             start = time.time()

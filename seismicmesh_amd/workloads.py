@@ -1,0 +1,59 @@
+"""Synthetic workloads (seeded) for benchmarks and tests.
+
+* ``disk``: unit disk, uniform edge length 0.02 (the reference's plain
+  DistMesh case; 7,879 initial vertices).
+* ``bp2004``: a BP2004-shaped 2D velocity model on a 1911 x 5395 FP32 grid
+  over z in [-12 km, 0], x in [0, 67 km] (6.25 m x 12.42 m cells): a
+  1480 m/s water layer over the top 1 km, then 1500 + 0.55 (depth - 1000)
+  m/s capped at 4000 m/s, plus 1-3 seeded elliptical salt bodies at
+  4500 m/s.  Sizing v / (2 Hz * 10), clamped to hmin = 75 m, gradient
+  limited at grade 0.15, on the Rectangle domain with its 4 corners fixed.
+  ``scale`` < 1 shrinks the grid (same physical box) for quick runs.
+"""
+import numpy as np
+
+from . import geometry as geo
+from .sizing import get_sizing_function_from_segy
+
+BP_BBOX = (-12000.0, 0.0, 0.0, 67000.0)
+BP_SHAPE = (1911, 5395)
+
+
+def bp2004_velocity(shape=BP_SHAPE, bbox=BP_BBOX, seed=2004):
+    nz, nx = shape
+    z = np.linspace(bbox[0], bbox[1], nz)
+    x = np.linspace(bbox[2], bbox[3], nx)
+    depth = -z
+    v = np.where(depth < 1000.0, 1480.0, np.minimum(1500.0 + 0.55 * (depth - 1000.0), 4000.0))
+    vp = np.repeat(v[:, None], nx, axis=1).astype(np.float32)
+    rng = np.random.default_rng(seed)
+    nbody = int(rng.integers(1, 4))
+    Z, X = np.meshgrid(depth, x, indexing="ij")
+    for _ in range(nbody):
+        cx = rng.uniform(8000.0, 59000.0)
+        cz = rng.uniform(2500.0, 9000.0)
+        ax = rng.uniform(3000.0, 9000.0)
+        az = rng.uniform(800.0, 2500.0)
+        inside = ((X - cx) / ax) ** 2 + ((Z - cz) / az) ** 2 <= 1.0
+        vp[inside] = 4500.0
+    return vp
+
+
+def bp2004(scale=1.0, seed=2004, quiet=True):
+    """Returns (domain, edge_length, info) for the BP2004-shaped workload."""
+    shape = (max(16, int(round(BP_SHAPE[0] * scale))), max(16, int(round(BP_SHAPE[1] * scale))))
+    vp = bp2004_velocity(shape, BP_BBOX, seed)
+    kw = dict(velocity_data=vp, hmin=75.0, hmax=1e4, wl=10, freq=2.0, grade=0.15)
+    if quiet:
+        import contextlib
+        import io
+        with contextlib.redirect_stdout(io.StringIO()):
+            ef = get_sizing_function_from_segy(None, BP_BBOX, **kw)
+    else:
+        ef = get_sizing_function_from_segy(None, BP_BBOX, **kw)
+    domain = geo.Rectangle(BP_BBOX)
+    return domain, ef, {"grid": shape, "bbox": BP_BBOX, "hmin": 75.0, "vp": vp}
+
+
+def disk(h=0.02):
+    return geo.Disk([0.0, 0.0], 1.0), h, {"bbox": (-1.0, 1.0, -1.0, 1.0)}

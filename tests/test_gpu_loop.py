@@ -1,0 +1,197 @@
+"""The device force loop against the oracle (GPU).
+
+FP64 iterations are compared bit for bit (the device reproduces numpy's
+order of operations); FP32 iterations to 1e-5 relative per update; whole
+meshes through the reference tests' known answers."""
+import contextlib
+import io
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    from seismicmesh_amd import _lib
+    _lib.require_gpu()
+
+
+def _oracle_state(domain, h, **kw):
+    from oracle import distmesh as odm
+    opts = dict(odm.GEN_DEFAULTS)
+    opts.update(kw)
+    opts["verbose"] = 0
+    return odm.setup(domain, h, opts)
+
+
+def _jittered_disk(n_side=60, seed=0):
+    rng = np.random.default_rng(seed)
+    g = np.stack(np.meshgrid(np.linspace(-1, 1, n_side), np.linspace(-1, 1, n_side)), -1).reshape(-1, 2)
+    g = g + rng.uniform(-0.3, 0.3, g.shape) * (2.0 / n_side)
+    return g[np.hypot(g[:, 0], g[:, 1]) < 0.98]
+
+
+def _mesher(domain, h, p, precision=64, nfix=0):
+    from seismicmesh_amd import generation
+    st = generation.prepare(domain, h, verbose=0, precision=precision, points=p)
+    st["nfix"] = nfix
+    m = generation.make_mesher(st)
+    m.set_points(p)
+    return m, st
+
+
+def test_fp64_iterations_bit_exact():
+    from oracle import distmesh as odm
+    from oracle import geometry as og
+    from seismicmesh_amd import geometry as geo
+    p0 = _jittered_disk()
+    m, _ = _mesher(geo.Disk([0, 0], 1.0), 0.035, p0)
+    s = _oracle_state(og.Disk([0, 0], 1.0), 0.035, points=p0)
+    p = p0.copy()
+    for it in range(20):
+        m.run(1)
+        p, t, _ = odm.force_step(p, odm.delaunay(p), s, [])
+        assert np.array_equal(m.points(), p), "iteration %d" % it
+    st = m.stats()
+    # most iterations are repaired by device flips; an exact host rebuild
+    # is the fallback when flips cannot untangle a large early move
+    assert st["flips"] > 0 and st["host_rebuilds"] < 20
+
+
+def test_device_triangulation_is_delaunay():
+    from oracle import distmesh as odm
+    from oracle import geometry as og
+    from seismicmesh_amd import geometry as geo
+    p0 = _jittered_disk(50, seed=3)
+    m, _ = _mesher(geo.Disk([0, 0], 1.0), 0.04, p0)
+    od = og.Disk([0, 0], 1.0)
+    for _ in range(8):
+        m.run(1)
+        m.triangulate()
+        p = m.points()
+        got = {tuple(sorted(c)) for c in m.cells().tolist()}
+        t = odm.remove_triangles_outside(p, odm.delaunay(p), od.eval, 0.1 * 0.04)
+        want = {tuple(sorted(c)) for c in t.tolist()}
+        assert got == want
+        # the unique edge list is the reference's sorted edge list of the kept cells
+        assert np.array_equal(m.edges(), odm.get_edges(t))
+
+
+def test_fp32_update_within_tolerance():
+    from oracle import distmesh as odm
+    from oracle import geometry as og
+    from oracle.sizing import build_interpolant
+    from seismicmesh_amd import generation, workloads
+    domain, ef, info = workloads.bp2004(scale=0.1)
+    st = generation.prepare(domain, ef, verbose=0, precision=32)
+    p0 = st["p"]
+    m = generation.make_mesher(st)
+    m.set_points(p0)
+    m.run(1)
+    p1 = m.points()
+    grid = ef.cell_size
+    s = dict(fd=og.Rectangle(info["bbox"]).eval,
+             fh=build_interpolant(grid.values.astype(np.float64), grid.bbox),
+             h0=st["h0"], dim=2, L0mult=1.2, geps=st["geps"], deps=st["deps"],
+             pfix=p0[:st["nfix"]], levels=[og.Rectangle(info["bbox"]).eval], delta_t=0.3)
+    q1, _, _ = odm.force_step(p0, odm.delaunay(p0), s, list(range(st["nfix"])))
+    dp_gpu, dp_ref = p1 - p0, q1 - p0
+    scale = np.abs(dp_ref).max()
+    assert scale > 0
+    assert np.abs(dp_gpu - dp_ref).max() <= 1e-5 * scale
+
+
+def _counts(fn, *a, **k):
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        out = fn(*a, **k)
+    return out, len(buf.getvalue())
+
+
+def test_verbose_byte_counts():
+    # reference tests/test_verbose.py:13-22
+    import seismicmesh_amd as sm
+    square = sm.Rectangle((0.0, 1.0, 0.0, 1.0))
+    for verbosity, size in zip([0, 1, 2], [0, 192, 6014]):
+        _, n = _counts(sm.generate_mesh, domain=square, edge_length=0.1, verbose=verbosity)
+        assert n == size
+
+
+def test_end_to_end_disk_vs_oracle():
+    import seismicmesh_amd as sm
+    from oracle import distmesh as odm
+    from oracle import geometry as og
+    p, t = sm.generate_mesh(sm.Disk([0, 0], 1.0), 0.05, verbose=0)
+    q, s = odm.generate_mesh(og.Disk([0, 0], 1.0), 0.05, verbose=0)
+    assert abs(len(p) - len(q)) <= 0.01 * len(q)
+    qa, qb = sm.mesh_utils.simp_qual(p, t), og.simp_qual(q, s)
+    assert abs(qa.min() - qb.min()) <= 0.01
+    assert abs(qa.mean() - qb.mean()) <= 0.01
+
+
+def test_sdf_disk_known_answer():
+    # reference tests/test_2dmesher_SDF.py:11-29 (callable sizing: host path)
+    import seismicmesh_amd as sm
+    disk = sm.Disk([0.0, 0.0], 1)
+    p, t = sm.generate_mesh(bbox=(-1.0, 1.0, -1.0, 1.0), domain=disk, h0=0.2,
+                            edge_length=lambda x: 0.2 - disk.eval(x) * 0.15, max_iter=100,
+                            verbose=0)
+    assert np.allclose([len(p), len(t)], [63, 93], atol=10)
+    assert np.allclose(np.sum(sm.mesh_utils.simp_vol(p, t)), 3.14, atol=0.2)
+
+
+def test_min_quality_known_answer():
+    # reference tests/test_2d_min_qual.py:7-31
+    import seismicmesh_amd as sm
+    for h in np.logspace(-1.0, -2.0, num=3):
+        rect = sm.Rectangle((0.0, 1.0, 0.0, 1.0))
+        disk0 = sm.Disk([0.0, 0.0], 0.25)
+        quarter = sm.Intersection([sm.Difference([rect, disk0]), sm.Disk([0.0, 0.0], 1.0)])
+        p, t = sm.generate_mesh(domain=quarter, edge_length=lambda x: h + 0.1 * np.abs(disk0.eval(x)),
+                                h0=h, verbose=0)
+        assert np.amin(sm.mesh_utils.simp_qual(p, t)) > 0.60
+
+
+def test_vs_water_known_answer():
+    # reference tests/test_2dmesher_vs_water.py:14-53 (gridded sizing on device)
+    import seismicmesh_amd as sm
+    bbox = (-10000.0, 0.0, 0.0, 10000.0)
+    vs = np.zeros((200, 200))
+    vs[0:150, :] = 1000
+    with contextlib.redirect_stdout(io.StringIO()):
+        ef = sm.get_sizing_function_from_segy(None, bbox=bbox, grade=0.0, grad=0.0, wl=5,
+                                              freq=2.0, hmin=10, hmax=10e6, velocity_data=vs,
+                                              nz=200, nx=200)
+    assert ef.eval((-5000, 5000)) == 100
+    assert ef.eval((-1, 5000)) == 150
+    ef.hmin = None
+    p, t = sm.generate_mesh(sm.Rectangle(bbox), ef, h0=125, verbose=0)
+    assert np.allclose([len(p), len(t)], [5616, 10955], atol=100)
+
+
+def test_immersion_known_answer():
+    # reference tests/test_immersion.py:10-32 (subdomain level sets)
+    import seismicmesh_amd as sm
+    for radius in (0.25, 0.35):
+        box0 = sm.Rectangle((0.0, 1.0, 0.0, 1.0))
+        disk0 = sm.Disk([0.5, 0.5], radius)
+        p, t = sm.generate_mesh(domain=box0, edge_length=lambda x: 0.05 * np.abs(disk0.eval(x)) + 0.05,
+                                h0=0.05, subdomains=[disk0], verbose=0)
+        sd = disk0.eval(p[t].sum(1) / 3)
+        vol = np.sum(sm.mesh_utils.simp_vol(p, t[sd < 0]))
+        assert np.isclose(vol, np.pi * radius ** 2, rtol=1e-2)
+
+
+def test_errors_match_reference():
+    import seismicmesh_amd as sm
+    sq = sm.Rectangle((0.0, 1.0, 0.0, 1.0))
+    with pytest.raises(ValueError):
+        sm.generate_mesh(sq, 0.1, not_an_option=1)
+    with pytest.raises(ValueError):
+        sm.generate_mesh(sq, 0.1, max_iter=-1, verbose=0)
+    with pytest.raises(ValueError):
+        sm.generate_mesh(sm.Rectangle((0, 1, 0, 1)), 0.1, verbose=0)  # int bbox
+    with pytest.raises(AssertionError):
+        sm.generate_mesh(sq, 0.1, delta_t=1e6, verbose=0)  # divergence assert
