@@ -99,7 +99,9 @@ typedef struct {
   int32_t dim;
   double prm[8];
   double xf[34];
-  double pad[2];
+  double pad[2]; /* pad[0] of the first node: 1.0 when the program needs
+                    more than 8 stack values or nests SM_OP_PUSH_MOD (it
+                    then runs in the slower scratch-memory interpreter) */
 } sm_node; /* 368 bytes */
 
 /* d[i] = sdf(x[i]) in FP64; x is (n, dim) row-major.  `tape` is a device
@@ -171,6 +173,10 @@ int sm_mesher_set_points(sm_mesher *m, const double *p, int64_t n);
  * filter, unique edges, bar forces, update, projection).  `maxdp_out`
  * (host, may be NULL) receives the last iteration's max movement. */
 int sm_mesher_run(sm_mesher *m, int iters, double *maxdp_out);
+/* FP32 handles run each iteration as one captured HIP graph (default on);
+ * disable to launch the kernels one by one with host decisions between
+ * stages. */
+int sm_mesher_set_graph(sm_mesher *m, int enable);
 /* Makes the triangulation of the current points Delaunay and filters cells
  * (the first half of an iteration), without moving points. */
 int sm_mesher_triangulate(sm_mesher *m);
@@ -181,7 +187,8 @@ int sm_mesher_get_points(sm_mesher *m, double *p, int64_t cap);
 int sm_mesher_get_cells(sm_mesher *m, int32_t *cells, int64_t cap);
 int sm_mesher_get_edges(sm_mesher *m, int32_t *edges, int64_t cap);
 /* Statistics since creation: [0] iterations, [1] flip rounds, [2] flips,
- * [3] host rebuilds, [4] exact predicate fallbacks. */
+ * [3] host rebuilds, [4] exact predicate evaluations, [5] speculative
+ * (graph) iterations redone because their flips had not converged. */
 int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats);
 /* Per-phase kernel timing with HIP events recorded on the handle's stream.
  * Phases: 0 flip rounds, 1 centroid filter, 2 unique edges + edge lists,

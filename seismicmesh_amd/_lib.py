@@ -16,6 +16,14 @@ if not os.path.exists(LIB_PATH):
         "(hipcc, gfx950) before importing seismicmesh_amd"
     )
 
+# One HIP runtime per process: torch ships its own libamdhip64 (same
+# soname).  Loading torch first makes libsmgpu bind to that copy instead of
+# loading a second runtime that would then keep torch from seeing the GPU.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover - torch is only the buffer provider
+    torch = None
+
 lib = ctypes.CDLL(LIB_PATH)
 
 SM_OK = 0
@@ -99,6 +107,7 @@ _PROTOS = {
     "sm_mesher_get_cells": (_int, [_vp, _vp, _i64]),
     "sm_mesher_get_edges": (_int, [_vp, _vp, _i64]),
     "sm_mesher_stats": (_int, [_vp, _vp, _int]),
+    "sm_mesher_set_graph": (_int, [_vp, _int]),
     "sm_mesher_profile": (_int, [_vp, _int]),
     "sm_mesher_profile_read": (_int, [_vp, _vp, _vp, _int]),
     "sm_mesher_device_points": (_int, [_vp, ctypes.POINTER(_vp)]),

@@ -84,8 +84,30 @@ __global__ void k_scatter(PairSrc src, const int32_t *rowptr, int32_t *cur, int3
   }
 }
 
-constexpr int kRegSort = 32;    // buckets up to this size: per-thread sort
+constexpr int kRegSort = 16;    // buckets up to this size: per-thread sort
 constexpr int kLdsSort = 4096;  // up to this size: one workgroup, LDS bitonic
+
+// Sorts 16 register values with a bitonic network: every index is a
+// compile-time constant after unrolling, so the values never leave VGPRs.
+__device__ inline void sort16(int32_t (&v)[16]) {
+#pragma unroll
+  for (int size = 2; size <= 16; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const int j = k ^ stride;
+        if (j > k) {
+          const bool up = (k & size) == 0;
+          int32_t a = v[k], b = v[j];
+          int32_t lo = a < b ? a : b, hi = a < b ? b : a;
+          v[k] = up ? lo : hi;
+          v[j] = up ? hi : lo;
+        }
+      }
+    }
+  }
+}
 
 // Per-bucket sort + unique count for short buckets (one thread each); long
 // buckets are flagged for the workgroup pass.
@@ -101,21 +123,17 @@ __global__ void k_row_sort(int64_t nrows, const int32_t *rowptr, int32_t *col, i
     ucnt[r] = 0;  // filled by the long-bucket pass
     return;
   }
-  int32_t v[kRegSort];
-  for (int k = 0; k < len; ++k) v[k] = col[s + k];
-  for (int k = 1; k < len; ++k) {
-    int32_t x = v[k];
-    int m = k - 1;
-    while (m >= 0 && v[m] > x) {
-      v[m + 1] = v[m];
-      --m;
-    }
-    v[m + 1] = x;
-  }
+  int32_t v[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) v[k] = k < len ? col[s + k] : INT32_MAX;
+  sort16(v);
   int32_t u = 0;
-  for (int k = 0; k < len; ++k) {
-    col[s + k] = v[k];
-    u += (k == 0 || v[k] != v[k - 1]);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    if (k < len) {
+      col[s + k] = v[k];
+      u += (k == 0 || v[k] != v[k - 1]);
+    }
   }
   ucnt[r] = u;
 }

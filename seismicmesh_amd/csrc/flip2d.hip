@@ -37,8 +37,28 @@ constexpr int kGhostOri = 2;
 
 __device__ inline const double *PT(const double *p, int v) { return p + 2 * (int64_t)v; }
 
+// A round's counter block (8 ints): [0] flips, [1] inverted triangles,
+// [2] orientation queue, [3] in-circle queue, [5] unresolved exact
+// predicates, [6] round id (unique over the handle's life), [7] skip.
+// A round is skipped when the previous one flipped nothing, was skipped or
+// hit an unresolved predicate, so a batch of rounds can be enqueued (or
+// replayed from a graph) without the host looking at the counters.
+__global__ void k_round_begin(const int32_t *prev, int32_t *ctr, int32_t *round_counter) {
+  int skip = prev != nullptr && (prev[7] || prev[0] == 0 || prev[5] > 0);
+  for (int i = 0; i < 6; ++i) ctr[i] = 0;
+  ctr[7] = skip;
+  if (!skip) ctr[6] = ++(*round_counter);
+}
+
+__global__ void k_clear_claims(int64_t T, uint64_t *claim, const int32_t *ctr) {
+  if (ctr[7]) return;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < T) claim[t] = 0ull;
+}
+
 __global__ void k_orient(int64_t T, const int32_t *tri, const double *p, int32_t *ori,
                          int32_t *queue, int32_t *ctr) {
+  if (ctr[7]) return;
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
   int a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
@@ -59,6 +79,7 @@ __global__ void k_orient(int64_t T, const int32_t *tri, const double *p, int32_t
 
 __global__ void k_orient_exact(const int32_t *tri, const double *p, int32_t *ori,
                                const int32_t *queue, int32_t *ctr) {
+  if (ctr[7]) return;
   const int n = ctr[2];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     int t = queue[i];
@@ -127,7 +148,9 @@ __device__ inline int valid_new(const double *p, int a, int b, int c) {
 
 __global__ void k_mark(int64_t T, const int32_t *tri, const int32_t *adj, const int32_t *ori,
                        const double *p, uint64_t *claim, uint64_t *cand, int32_t *queue,
-                       int32_t *ctr, int round) {
+                       int32_t *ctr) {
+  if (ctr[7]) return;
+  const int round = ctr[6];
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
   for (int k = 0; k < 3; ++k) {
@@ -172,8 +195,10 @@ __global__ void k_mark(int64_t T, const int32_t *tri, const int32_t *adj, const 
 }
 
 __global__ void k_mark_exact(const int32_t *tri, const int32_t *adj, const double *p,
-                             uint64_t *claim, uint64_t *cand, const int32_t *queue, int32_t *ctr,
-                             int round) {
+                             uint64_t *claim, uint64_t *cand, const int32_t *queue,
+                             int32_t *ctr) {
+  if (ctr[7]) return;
+  const int round = ctr[6];
   const int n = ctr[3];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     int id = queue[i];
@@ -206,8 +231,9 @@ __global__ void k_mark_exact(const int32_t *tri, const int32_t *adj, const doubl
 }
 
 __global__ void k_apply(int64_t T, int32_t *tri, int32_t *adj, const uint64_t *claim,
-                        const uint64_t *cand, int32_t *partner, int32_t *mark, int32_t *ctr,
-                        int round) {
+                        const uint64_t *cand, int32_t *partner, int32_t *mark, int32_t *ctr) {
+  if (ctr[7]) return;
+  const int round = ctr[6];
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
   for (int k = 0; k < 3; ++k) {
@@ -242,7 +268,9 @@ __global__ void k_apply(int64_t T, int32_t *tri, int32_t *adj, const uint64_t *c
 }
 
 __global__ void k_fix(int64_t T, const int32_t *tri, int32_t *adj, const int32_t *partner,
-                      const int32_t *mark, int round) {
+                      const int32_t *mark, const int32_t *ctr) {
+  if (ctr[7]) return;
+  const int round = ctr[6];
   int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= T) return;
   for (int k = 0; k < 3; ++k) {
@@ -259,57 +287,216 @@ __global__ void k_fix(int64_t T, const int32_t *tri, int32_t *adj, const int32_t
   }
 }
 
+// Summary of a batch of rounds into status[0..7]: [0] rounds executed,
+// [1] flips, [2] converged (the last executed round flipped nothing),
+// [3] inverted triangles seen by that round, [4] unresolved predicates,
+// [5] exact-predicate evaluations.
+__global__ void k_flip_status(const int32_t *ctr, int nrounds, int32_t *status) {
+  int last = -1, flips = 0, unres = 0, exact = 0;
+  for (int r = 0; r < nrounds; ++r) {
+    const int32_t *c = ctr + 8 * r;
+    if (c[7]) break;
+    last = r;
+    flips += c[0];
+    unres += c[5];
+    exact += c[2] + c[3];
+  }
+  status[0] = last + 1;
+  status[1] = flips;
+  status[2] = last >= 0 && ctr[8 * last] == 0;
+  status[3] = last >= 0 ? ctr[8 * last + 1] : 0;
+  status[4] = unres;
+  status[5] = exact;
+}
+
 }  // namespace
 
-int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
+__global__ void k_lerp(int64_t n, const double *a, const double *b, double s, double *out) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = a[i] + s * (b[i] - a[i]);
+}
+
+// Copies the triangulation and the points it is valid for into the
+// snapshot buffers, but only when the flip status says the repair
+// converged to a valid triangulation (so a speculative graph iteration
+// never overwrites the last good state with a tangled one).
+__global__ void k_snapshot_if_valid(const int32_t *status, int64_t T, const int32_t *tri,
+                                    const int32_t *adj, int32_t *tri_s, int32_t *adj_s,
+                                    int64_t n2, const double *p, double *p_s) {
+  if (!(status[2] && status[3] == 0 && status[4] == 0)) return;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = i; k < 3 * T; k += stride) {
+    tri_s[k] = tri[k];
+    adj_s[k] = adj[k];
+  }
+  for (int64_t k = i; k < n2; k += stride) p_s[k] = p[k];
+}
+
+int enqueue_snapshot(sm_mesher *m, const int32_t *status) {
+  const int64_t work = std::max<int64_t>(3 * m->T, m->N * 2);
+  unsigned g = blocks_for(work, 256);
+  if (g > 2048) g = 2048;
+  hipLaunchKernelGGL(k_snapshot_if_valid, dim3(g), dim3(256), 0, m->stream, status, m->T,
+                     m->d_tri, m->d_adj, m->d_tri_s, m->d_adj_s, m->N * 2, m->d_p, m->d_pvalid);
+  SM_LAUNCH_CHECK();
+  return SM_OK;
+}
+
+namespace {
+
+// Runs flip batches on `pts` until the rounds converge; returns 1 when the
+// triangulation is then valid (Delaunay) for `pts`, 0 when it is not and
+// -1 when an exact predicate could not be settled.
+int repair_on(sm_mesher *m, const double *pts, int max_rounds) {
+  hipStream_t s = m->stream;
+  int32_t *dstat = m->d_ctr + kCtrStatus;
+  for (int done = 0; done < max_rounds;) {
+    int rc = enqueue_flip_rounds(m, 4, dstat, pts);
+    if (rc) return rc > 0 ? -rc : rc - 100;
+    if (hipMemcpyAsync(m->h_ctr, dstat, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return -2;
+    account_flips(m, m->h_ctr);
+    done += m->h_ctr[0];
+    if (m->h_ctr[4] > 0) return -1;
+    if (m->h_ctr[2]) return m->h_ctr[3] == 0 ? 1 : 0;
+  }
+  return 0;
+}
+
+}  // namespace
+
+// Kinetic repair: restore the last valid triangulation (valid for the
+// points in d_pvalid) and move the points towards their current positions
+// in sub-steps, repairing by flips after each; a failed sub-step is retried
+// from the last good state with half the step.  Returns true on success.
+bool kinetic_repair(sm_mesher *m) {
+  hipStream_t s = m->stream;
+  const int64_t n2 = m->N * 2;
+  double s0 = 0.0, step = 0.5;
+  auto restore = [&]() {
+    (void)hipMemcpyAsync(m->d_tri, m->d_tri_s, m->T * 3 * sizeof(int32_t),
+                         hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(m->d_adj, m->d_adj_s, m->T * 3 * sizeof(int32_t),
+                         hipMemcpyDeviceToDevice, s);
+  };
+  auto save = [&]() {
+    (void)hipMemcpyAsync(m->d_tri_s, m->d_tri, m->T * 3 * sizeof(int32_t),
+                         hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(m->d_adj_s, m->d_adj, m->T * 3 * sizeof(int32_t),
+                         hipMemcpyDeviceToDevice, s);
+  };
+  restore();
+  int stages = 0;
+  while (s0 < 1.0) {
+    double s1 = std::min(1.0, s0 + step);
+    const double *pts = m->d_p;
+    if (s1 < 1.0) {
+      hipLaunchKernelGGL(k_lerp, dim3(blocks_for(n2, 256)), dim3(256), 0, s, n2, m->d_pvalid,
+                         m->d_p, s1, m->d_pstage);
+      pts = m->d_pstage;
+    }
+    int ok = repair_on(m, pts, 64);
+    ++stages;
+    if (ok == 1) {
+      s0 = s1;
+      if (s0 < 1.0) save();
+      step = std::min(2.0 * step, 1.0);
+    } else {
+      if (ok < 0 || step < 1.0 / 512 || stages > 200) return false;
+      restore();
+      step *= 0.5;
+    }
+  }
+  m->stats[6] += 1;
+  return true;
+}
+
+int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double *pts) {
   hipStream_t s = m->stream;
   const int64_t T = m->T;
   const int B = 256;
   const unsigned g = blocks_for(T, B);
-  *need_rebuild = false;
-  for (int r = 0; r < max_rounds; ++r) {
-    const int round = ++m->round;
-    int rc;
-    if ((rc = fill<int32_t>(m->d_ctr, 8, 0, s))) return rc;
-    if ((rc = fill<uint64_t>(m->d_claim, T, 0ull, s))) return rc;
-    const int pb = prof_begin(m);
-    hipLaunchKernelGGL(k_orient, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_p, m->d_ori,
-                       m->d_queue, m->d_ctr);
-    SM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_orient_exact, dim3(32), dim3(64), 0, s, m->d_tri, m->d_p, m->d_ori,
-                       m->d_queue, m->d_ctr);
-    SM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, m->d_p,
-                       m->d_claim, m->d_cand, m->d_queue + T, m->d_ctr, round);
-    SM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_mark_exact, dim3(32), dim3(64), 0, s, m->d_tri, m->d_adj, m->d_p,
-                       m->d_claim, m->d_cand, m->d_queue + T, m->d_ctr, round);
-    SM_LAUNCH_CHECK();
+  int32_t *base = m->d_ctr + kCtrRounds;
+  for (int r = 0; r < nrounds; ++r) {
+    int32_t *c = base + 8 * r;
+    const int32_t *prev = r == 0 ? nullptr : c - 8;
+    hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(1), 0, s, prev, c, m->d_ctr + kCtrRoundId);
+    hipLaunchKernelGGL(k_clear_claims, dim3(g), dim3(B), 0, s, T, m->d_claim, (const int32_t *)c);
+    hipLaunchKernelGGL(k_orient, dim3(g), dim3(B), 0, s, T, m->d_tri, pts, m->d_ori,
+                       m->d_queue, c);
+    hipLaunchKernelGGL(k_orient_exact, dim3(1), dim3(64), 0, s, m->d_tri, pts, m->d_ori,
+                       m->d_queue, c);
+    hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, pts,
+                       m->d_claim, m->d_cand, m->d_queue + T, c);
+    hipLaunchKernelGGL(k_mark_exact, dim3(1), dim3(64), 0, s, m->d_tri, m->d_adj, pts,
+                       m->d_claim, m->d_cand, m->d_queue + T, c);
     hipLaunchKernelGGL(k_apply, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_claim,
-                       m->d_cand, m->d_partner, m->d_mark, m->d_ctr, round);
-    SM_LAUNCH_CHECK();
+                       m->d_cand, m->d_partner, m->d_mark, c);
     hipLaunchKernelGGL(k_fix, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_partner,
-                       m->d_mark, round);
+                       m->d_mark, (const int32_t *)c);
     SM_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_flip_status, dim3(1), dim3(1), 0, s, (const int32_t *)base, nrounds,
+                     status);
+  SM_LAUNCH_CHECK();
+  return SM_OK;
+}
+
+void account_flips(sm_mesher *m, const int32_t *st) {
+  m->stats[1] += st[0];
+  m->stats[2] += st[1];
+  m->stats[4] += st[5];
+}
+
+int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
+  hipStream_t s = m->stream;
+  *need_rebuild = false;
+  int32_t *dstat = m->d_ctr + kCtrStatus;
+  bool ok = false, exact_failed = false;
+  for (int done = 0; done < max_rounds;) {
+    const int batch = 4;
+    const int pb = prof_begin(m);
+    int rc = enqueue_flip_rounds(m, batch, dstat, m->d_p);
+    if (rc) return rc;
     prof_end(m, 0, pb);
-    SM_HIP(hipMemcpyAsync(m->h_ctr, m->d_ctr, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SM_HIP(hipMemcpyAsync(m->h_ctr, dstat, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
     SM_HIP(hipStreamSynchronize(s));
-    const int flips = m->h_ctr[0], inverted = m->h_ctr[1];
-    m->stats[1] += 1;
-    m->stats[2] += flips;
-    m->stats[4] += m->h_ctr[2] + m->h_ctr[3];
-    if (m->h_ctr[5] > 0) {
-      *need_rebuild = true;
-      return SM_OK;
+    account_flips(m, m->h_ctr);
+    done += m->h_ctr[0];
+    if (m->h_ctr[4] > 0) {  // an exact predicate did not fit: rebuild exactly
+      exact_failed = true;
+      break;
     }
-    if (flips == 0) {
-      // no flip this round: every edge was locally Delaunay (or blocked);
-      // inverted triangles left mean the flips could not untangle
-      *need_rebuild = inverted > 0;
-      return SM_OK;
+    if (m->h_ctr[2]) {
+      // the last round flipped nothing: every edge is locally Delaunay
+      // unless inverted triangles are left that flips could not untangle
+      ok = m->h_ctr[3] == 0;
+      break;
     }
   }
-  *need_rebuild = true;  // did not converge within the round budget
+  if (!ok && !exact_failed && m->have_snap) ok = kinetic_repair(m);
+  if (!ok) {
+    *need_rebuild = true;
+    return SM_OK;
+  }
+  return snapshot_valid(m);
+}
+
+// Records the current triangulation and points as the last valid state.
+int snapshot_valid(sm_mesher *m) {
+  hipStream_t s = m->stream;
+  int32_t *dstat = m->d_ctr + kCtrStatus;
+  int32_t *valid = m->h_ctr + kCtrScratchHost;  // pinned staging
+  const int32_t v[8] = {1, 0, 1, 0, 0, 0, 0, 0};
+  for (int i = 0; i < 8; ++i) valid[i] = v[i];
+  SM_HIP(hipMemcpyAsync(dstat, valid, 8 * sizeof(int32_t), hipMemcpyHostToDevice, s));
+  int rc = enqueue_snapshot(m, dstat);
+  if (rc) return rc;
+  SM_HIP(hipStreamSynchronize(s));
+  m->have_snap = true;
   return SM_OK;
 }
 
@@ -333,7 +520,7 @@ int host_rebuild2d(sm_mesher *m) {
   SM_HIP(hipStreamSynchronize(m->stream));
   m->stats[3] += 1;
   m->have_tri = true;
-  return SM_OK;
+  return snapshot_valid(m);
 }
 
 }  // namespace smk

@@ -31,7 +31,7 @@ namespace smk {
 
 namespace {
 
-template <int DIM>
+template <int DIM, int MODE>
 __global__ void k_keep(int64_t T, const int32_t *tri, const double *p, const sm_node *tape,
                        int ntape, double geps, uint8_t *keep) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -52,16 +52,18 @@ __global__ void k_keep(int64_t T, const int32_t *tri, const double *p, const sm_
     for (int k = 1; k <= DIM; ++k) s = s + p[(int64_t)c[k] * DIM + d];
     x[d] = s / (double)(DIM + 1);
   }
-  keep[t] = sdf_eval<DIM>(tape, ntape, x) < -geps;
+  keep[t] = sdf_eval<DIM, MODE>(tape, ntape, x) < -geps;
 }
 
 // second-endpoint lists: count, fill, order
-__global__ void k_count_second(int64_t E, const int32_t *edges, int32_t *cnt) {
+__global__ void k_count_second(const int32_t *Ep, const int32_t *edges, int32_t *cnt) {
+  const int64_t E = *Ep;
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e < E) atomicAdd(&cnt[edges[2 * e + 1]], 1);
 }
-__global__ void k_fill_second(int64_t E, const int32_t *edges, const int32_t *ptr, int32_t *cur,
-                              int32_t *list) {
+__global__ void k_fill_second(const int32_t *Ep, const int32_t *edges, const int32_t *ptr,
+                              int32_t *cur, int32_t *list) {
+  const int64_t E = *Ep;
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   int b = edges[2 * e + 1];
@@ -89,9 +91,10 @@ template <> __device__ inline float zero_len<float>() { return 2.220446049250313
 // Per-edge bar length and sizing; per-block partial sums of L^d and h^d
 // (FP64) for the fast path, or the per-edge powers for the exact path.
 template <int DIM, typename T, bool EXACT>
-__global__ void k_edge_geom(int64_t E, const int32_t *edges, const double *p, GridView grid,
-                            const double *h_ext, T *L_out, T *h_out, double *Lp, double *hp,
-                            double *part) {
+__global__ void k_edge_geom(const int32_t *Ep, const int32_t *edges, const double *p,
+                            GridView grid, const double *h_ext, T *L_out, T *h_out, double *Lp,
+                            double *hp, double *part) {
+  const int64_t E = *Ep;
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double sl = 0.0, sh = 0.0;
   if (e < E) {
@@ -272,8 +275,10 @@ __global__ void k_np_total(const double *chunk_sums, int64_t nchunks, double *ou
 }
 
 template <int DIM, typename T>
-__global__ void k_edge_force(int64_t E, const int32_t *edges, const double *p, const T *L,
-                             const T *h, const double *scale_ptr, double L0mult, T *F) {
+__global__ void k_edge_force(const int32_t *Ep, const int32_t *edges, const double *p,
+                             const T *L, const T *h, const double *scale_ptr, double L0mult,
+                             T *F) {
+  const int64_t E = *Ep;
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   const T scale = (T)scale_ptr[2];
@@ -290,10 +295,10 @@ __global__ void k_edge_force(int64_t E, const int32_t *edges, const double *p, c
   }
 }
 
-template <int DIM>
+template <int DIM, int MODE>
 __device__ inline void project_level(double *x, const sm_node *tape, int ntape, double deps,
                                      double hmin, int level) {
-  double d = sdf_eval<DIM>(tape, ntape, x);
+  double d = sdf_eval<DIM, MODE>(tape, ntape, x);
   bool in = level == 0 ? (d > 0.0) : (d > 0.0 && d < hmin / 1.5);
   if (!in) return;
   double g[DIM];
@@ -302,7 +307,7 @@ __device__ inline void project_level(double *x, const sm_node *tape, int ntape, 
     double y[3];
 #pragma unroll
     for (int k = 0; k < DIM; ++k) y[k] = x[k] + (k == i ? deps : 0.0);
-    g[i] = (sdf_eval<DIM>(tape, ntape, y) - d) / deps;
+    g[i] = (sdf_eval<DIM, MODE>(tape, ntape, y) - d) / deps;
   }
   double g2 = g[0] * g[0];
 #pragma unroll
@@ -312,7 +317,7 @@ __device__ inline void project_level(double *x, const sm_node *tape, int ntape, 
   for (int i = 0; i < DIM; ++i) x[i] = x[i] - (d * g[i]) / g2;
 }
 
-template <int DIM, typename T>
+template <int DIM, typename T, int MODE>
 __global__ void k_vertex_update(int64_t N, double *p, const int32_t *first_ptr,
                                 const int32_t *second_ptr, const int32_t *second, const T *F,
                                 int nfix, double dt, const sm_node *tapes, const int *tape_off,
@@ -345,22 +350,30 @@ __global__ void k_vertex_update(int64_t N, double *p, const int32_t *first_ptr,
 #pragma unroll
   for (int d = 0; d < DIM; ++d) x[d] = p[v * DIM + d] + dt * (double)f[d];
   for (int l = 0; l < nlevels; ++l)
-    project_level<DIM>(x, tapes + tape_off[l], tape_len[l], deps, hmin, l);
+    project_level<DIM, MODE>(x, tapes + tape_off[l], tape_len[l], deps, hmin, l);
 #pragma unroll
   for (int d = 0; d < DIM; ++d) p[v * DIM + d] = x[d];
 }
 
 }  // namespace
 
-int centroid_keep(sm_mesher *m) {
+template <int DIM, int MODE>
+static void launch_keep(sm_mesher *m) {
   const int B = 256;
+  hipLaunchKernelGGL((k_keep<DIM, MODE>), dim3(blocks_for(m->T, B)), dim3(B), 0, m->stream,
+                     m->T, m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0], m->prm.geps,
+                     m->d_keep);
+}
+
+int centroid_keep(sm_mesher *m) {
   if (m->T == 0) return SM_OK;
-  if (m->dim == 2)
-    hipLaunchKernelGGL(k_keep<2>, dim3(blocks_for(m->T, B)), dim3(B), 0, m->stream, m->T,
-                       m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0], m->prm.geps, m->d_keep);
-  else
-    hipLaunchKernelGGL(k_keep<3>, dim3(blocks_for(m->T, B)), dim3(B), 0, m->stream, m->T,
-                       m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0], m->prm.geps, m->d_keep);
+  if (m->dim == 2) {
+    if (m->deep_sdf) launch_keep<2, kSdfDeep>(m);
+    else launch_keep<2, kSdfReg>(m);
+  } else {
+    if (m->deep_sdf) launch_keep<3, kSdfDeep>(m);
+    else launch_keep<3, kSdfReg>(m);
+  }
   SM_LAUNCH_CHECK();
   return SM_OK;
 }
@@ -377,32 +390,38 @@ static int32_t *eptr_of(sm_mesher *m) {
   return reinterpret_cast<int32_t *>(w);
 }
 
-int build_edges(sm_mesher *m) {
+// Launch bound for per-edge kernels: a triangulation of N points has at
+// most 3N - 6 edges, so kernels sized for 3N read the actual count on the
+// device and no host round trip is needed between the edge build and the
+// forces.
+static int64_t edge_launch_bound(const sm_mesher *m) {
+  return m->dim == 2 ? 3 * m->N + 8 : m->capE;
+}
+
+int build_edges(sm_mesher *m, bool sync_edges) {
   hipStream_t s = m->stream;
   const int dim = m->dim;
   int64_t np = m->capT * (dim == 2 ? 3 : 6);
-  int rc = unique_edges_run(nullptr, m->d_tri, m->d_keep, dim, m->T, np, m->N, m->d_edges,
-                            m->d_ctr + 4, nullptr, m->d_ews, m->ews_bytes, s);
+  int32_t *Ep = m->d_ctr + kCtrE;
+  int rc = unique_edges_run(nullptr, m->d_tri, m->d_keep, dim, m->T, np, m->N, m->d_edges, Ep,
+                            nullptr, m->d_ews, m->ews_bytes, s);
   if (rc) return rc;
-  // the edge count is needed on the host to size the next launches
-  SM_HIP(hipMemcpyAsync(m->h_ctr + 4, m->d_ctr + 4, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  SM_HIP(hipStreamSynchronize(s));
-  m->E = m->h_ctr[4];
+  if (sync_edges) {
+    SM_HIP(hipMemcpyAsync(m->h_ctr + kCtrE, Ep, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    SM_HIP(hipStreamSynchronize(s));
+    m->E = m->h_ctr[kCtrE];
+  }
   const int B = 256;
-  const int64_t N = m->N, E = m->E;
+  const int64_t N = m->N, Eb = edge_launch_bound(m);
   if ((rc = fill<int32_t>(m->d_sptr, N + 1, 0, s))) return rc;
   if ((rc = fill<int32_t>(m->d_scur, N + 1, 0, s))) return rc;
-  if (E > 0) {
-    hipLaunchKernelGGL(k_count_second, dim3(blocks_for(E, B)), dim3(B), 0, s, E, m->d_edges,
-                       m->d_sptr);
-    SM_LAUNCH_CHECK();
-  }
+  hipLaunchKernelGGL(k_count_second, dim3(blocks_for(Eb, B)), dim3(B), 0, s, (const int32_t *)Ep,
+                     m->d_edges, m->d_sptr);
+  SM_LAUNCH_CHECK();
   if ((rc = scan_exclusive(m->d_sptr, m->d_sptr, N + 1, nullptr, m->d_scan_ws, s))) return rc;
-  if (E > 0) {
-    hipLaunchKernelGGL(k_fill_second, dim3(blocks_for(E, B)), dim3(B), 0, s, E, m->d_edges,
-                       m->d_sptr, m->d_scur, m->d_slist);
-    SM_LAUNCH_CHECK();
-  }
+  hipLaunchKernelGGL(k_fill_second, dim3(blocks_for(Eb, B)), dim3(B), 0, s, (const int32_t *)Ep,
+                     m->d_edges, m->d_sptr, m->d_scur, m->d_slist);
+  SM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sort_second, dim3(blocks_for(N, B)), dim3(B), 0, s, N, m->d_sptr,
                      m->d_slist);
   SM_LAUNCH_CHECK();
@@ -415,20 +434,19 @@ static int forces_t(sm_mesher *m) {
   const int B = 256;
   const int64_t E = m->E, N = m->N;
   const bool exact = m->prm.precision == 64;
-  const unsigned ge = blocks_for(E, B);
+  const unsigned ge = blocks_for(edge_launch_bound(m), B);
+  const int32_t *Ep = m->d_ctr + kCtrE;
   GridView g = m->grid;
   int pb = prof_begin(m);
-  if (E > 0) {
-    if (exact)
-      hipLaunchKernelGGL((k_edge_geom<DIM, T, true>), dim3(ge), dim3(B), 0, s, E, m->d_edges,
-                         m->d_p, g, m->d_hext, (T *)m->d_L, (T *)m->d_h, m->d_Lp,
-                         m->d_hp, m->d_part);
-    else
-      hipLaunchKernelGGL((k_edge_geom<DIM, T, false>), dim3(ge), dim3(B), 0, s, E, m->d_edges,
-                         m->d_p, g, m->d_hext, (T *)m->d_L, (T *)m->d_h, m->d_Lp,
-                         m->d_hp, m->d_part);
-    SM_LAUNCH_CHECK();
-  }
+  if (exact)
+    hipLaunchKernelGGL((k_edge_geom<DIM, T, true>), dim3(ge), dim3(B), 0, s, Ep, m->d_edges,
+                       m->d_p, g, m->d_hext, (T *)m->d_L, (T *)m->d_h, m->d_Lp, m->d_hp,
+                       m->d_part);
+  else
+    hipLaunchKernelGGL((k_edge_geom<DIM, T, false>), dim3(ge), dim3(B), 0, s, Ep, m->d_edges,
+                       m->d_p, g, m->d_hext, (T *)m->d_L, (T *)m->d_h, m->d_Lp, m->d_hp,
+                       m->d_part);
+  SM_LAUNCH_CHECK();
   prof_end(m, 3, pb);
   pb = prof_begin(m);
   if (exact) {
@@ -451,27 +469,31 @@ static int forces_t(sm_mesher *m) {
     SM_HIP(hipStreamSynchronize(s));
     m->h_sums[2] = std::pow(m->h_sums[0] / m->h_sums[1], 1.0 / DIM);
     SM_HIP(hipMemcpyAsync(m->d_sums + 2, m->h_sums + 2, sizeof(double), hipMemcpyHostToDevice, s));
-  } else if (E > 0) {
+  } else {
     hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, (int64_t)ge, m->d_part,
                        m->d_sums, DIM);
     SM_LAUNCH_CHECK();
   }
   prof_end(m, 4, pb);
   pb = prof_begin(m);
-  if (E > 0) {
-    hipLaunchKernelGGL((k_edge_force<DIM, T>), dim3(ge), dim3(B), 0, s, E, m->d_edges, m->d_p,
-                       (const T *)m->d_L, (const T *)m->d_h, m->d_sums, m->prm.L0mult,
-                       (T *)m->d_F);
-    SM_LAUNCH_CHECK();
-  }
+  hipLaunchKernelGGL((k_edge_force<DIM, T>), dim3(ge), dim3(B), 0, s, Ep, m->d_edges, m->d_p,
+                     (const T *)m->d_L, (const T *)m->d_h, m->d_sums, m->prm.L0mult,
+                     (T *)m->d_F);
+  SM_LAUNCH_CHECK();
   prof_end(m, 5, pb);
   int rc = fill<uint64_t>(m->d_maxF, 1, 0ull, s);
   if (rc) return rc;
   pb = prof_begin(m);
-  hipLaunchKernelGGL((k_vertex_update<DIM, T>), dim3(blocks_for(N, B)), dim3(B), 0, s, N,
-                     m->d_p, eptr_of(m), m->d_sptr, m->d_slist, (const T *)m->d_F,
-                     m->prm.nfix, m->prm.delta_t, m->d_tape_all, m->d_tape_off_dev,
-                     m->d_tape_len_dev, m->prm.nlevels, m->prm.deps, m->prm.h0, m->d_maxF);
+  if (m->deep_sdf)
+    hipLaunchKernelGGL((k_vertex_update<DIM, T, kSdfDeep>), dim3(blocks_for(N, B)), dim3(B), 0,
+                       s, N, m->d_p, eptr_of(m), m->d_sptr, m->d_slist, (const T *)m->d_F,
+                       m->prm.nfix, m->prm.delta_t, m->d_tape_all, m->d_tape_off_dev,
+                       m->d_tape_len_dev, m->prm.nlevels, m->prm.deps, m->prm.h0, m->d_maxF);
+  else
+    hipLaunchKernelGGL((k_vertex_update<DIM, T, kSdfReg>), dim3(blocks_for(N, B)), dim3(B), 0,
+                       s, N, m->d_p, eptr_of(m), m->d_sptr, m->d_slist, (const T *)m->d_F,
+                       m->prm.nfix, m->prm.delta_t, m->d_tape_all, m->d_tape_off_dev,
+                       m->d_tape_len_dev, m->prm.nlevels, m->prm.deps, m->prm.h0, m->d_maxF);
   SM_LAUNCH_CHECK();
   prof_end(m, 6, pb);
   return SM_OK;

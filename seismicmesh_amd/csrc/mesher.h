@@ -18,6 +18,7 @@ struct sm_mesher {
   sm_node *d_tape_all = nullptr;  // concatenated, with offsets below
   std::vector<int> tape_off;
   int *d_tape_off_dev = nullptr, *d_tape_len_dev = nullptr;
+  bool deep_sdf = false;  // some program needs the scratch-stack interpreter
 
   // host sizing callback (for sizing given as a Python callable): called
   // with the edge midpoints, fills h; the device path then uses d_hext
@@ -58,11 +59,23 @@ struct sm_mesher {
   double *d_sums = nullptr;                             // [sumL, sumH, scale]
   uint64_t *d_maxF = nullptr;
 
-  // counters and their pinned host mirror
-  // [0] flips, [1] inverted, [2] ori-queue, [3] inc-queue, [4] edge count,
-  // [5] unresolved left, [6] kept cells
+  // device counters (layout in smk::kCtr*) and pinned host mirrors
   int32_t *d_ctr = nullptr;
   int32_t *h_ctr = nullptr;
+  int32_t *h_status = nullptr;  // written by the captured iteration
+
+  // the FP32 iteration captured as a graph (one launch + one sync each)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int64_t graph_T = -1;
+  bool use_graph = true;
+  double *d_psnap = nullptr;  // points at the start of the iteration
+
+  // last valid state: a Delaunay triangulation and the points it is valid
+  // for (the start of kinetic repair), plus a staging array for sub-steps
+  int32_t *d_tri_s = nullptr, *d_adj_s = nullptr;
+  double *d_pvalid = nullptr, *d_pstage = nullptr;
+  bool have_snap = false;
   double *h_sums = nullptr;
   uint64_t *h_maxF = nullptr;
 
@@ -84,6 +97,22 @@ struct sm_mesher {
 };
 
 namespace smk {
+// d_ctr layout (int32): [4] unique edge count, [16] last round id,
+// [24..31] status of the last batch of flip rounds, [32..) one 8-int block
+// per flip round of a batch
+constexpr int kCtrE = 4;
+constexpr int kCtrRoundId = 16;
+constexpr int kCtrStatus = 24;
+constexpr int kCtrRounds = 32;
+constexpr int kMaxBatchRounds = 64;
+constexpr int kCtrTotal = kCtrRounds + 8 * kMaxBatchRounds;
+constexpr int kCtrScratchHost = 64;  // pinned host staging inside h_ctr
+
+int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double *pts);
+void account_flips(sm_mesher *m, const int32_t *status);
+int enqueue_snapshot(sm_mesher *m, const int32_t *status);
+int snapshot_valid(sm_mesher *m);
+bool kinetic_repair(sm_mesher *m);
 // event-timed phases (no-ops unless profiling is enabled)
 int prof_begin(sm_mesher *m);
 void prof_end(sm_mesher *m, int phase, int begin_idx);
@@ -91,8 +120,10 @@ int prof_drain(sm_mesher *m);
 // flip2d.hip
 int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild);
 int host_rebuild2d(sm_mesher *m);
-// forces.hip
+// forces.hip: `sync_edges` reads the edge count back to the host (needed
+// by the exact-order sums); otherwise every launch is sized for the
+// largest possible edge count and reads the count on the device
 int centroid_keep(sm_mesher *m);
-int build_edges(sm_mesher *m);
+int build_edges(sm_mesher *m, bool sync_edges);
 int forces_and_update(sm_mesher *m);
 }  // namespace smk

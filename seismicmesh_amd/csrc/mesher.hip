@@ -52,6 +52,18 @@ void free_point_state(sm_mesher *m) {
   dfree(m->d_part);
   dfree(m->d_mid);
   dfree(m->d_hext);
+  dfree(m->d_psnap);
+  dfree(m->d_tri_s);
+  dfree(m->d_adj_s);
+  dfree(m->d_pvalid);
+  dfree(m->d_pstage);
+  m->have_snap = false;
+  // a captured iteration refers to these buffers
+  if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->gexec = nullptr;
+  m->graph = nullptr;
+  m->graph_T = -1;
 }
 
 int alloc_point_state(sm_mesher *m, int64_t N) {
@@ -91,6 +103,11 @@ int alloc_point_state(sm_mesher *m, int64_t N) {
     if ((rc = dalloc(&m->d_mid, m->capE * dim))) return rc;
     if ((rc = dalloc(&m->d_hext, m->capE))) return rc;
   }
+  if ((rc = dalloc(&m->d_psnap, N * dim))) return rc;
+  if ((rc = dalloc(&m->d_pvalid, N * dim))) return rc;
+  if ((rc = dalloc(&m->d_pstage, N * dim))) return rc;
+  if ((rc = dalloc(&m->d_tri_s, T * (dim + 1)))) return rc;
+  if ((rc = dalloc(&m->d_adj_s, T * (dim + 1)))) return rc;
   if ((rc = fill<int32_t>(m->d_mark, T, 0, m->stream))) return rc;
   return SM_OK;
 }
@@ -222,6 +239,7 @@ extern "C" int sm_mesher_create(const sm_mesher_params *prm, const sm_node *cons
     m->tape_off.push_back(total);
     m->tape_len.push_back(tape_lens[l]);
     total += tape_lens[l];
+    if (tapes[l][0].pad[0] != 0.0) m->deep_sdf = true;
   }
   SM_HIP(hipMalloc(&m->d_tape_all, total * sizeof(sm_node)));
   for (int l = 0; l < prm->nlevels; ++l) {
@@ -254,10 +272,15 @@ extern "C" int sm_mesher_create(const sm_mesher_params *prm, const sm_node *cons
     SM_HIP(hipMemcpy(m->d_values, sizing->values, nv * sizeof(float), hipMemcpyHostToDevice));
     m->grid.values = m->d_values;
   }
-  SM_HIP(hipMalloc(&m->d_ctr, 16 * sizeof(int32_t)));
+  SM_HIP(hipMalloc(&m->d_ctr, kCtrTotal * sizeof(int32_t)));
+  {
+    int rc0 = fill<int32_t>(m->d_ctr, kCtrTotal, 0, m->stream);
+    if (rc0) return rc0;
+  }
+  SM_HIP(hipHostMalloc((void **)&m->h_status, 16 * sizeof(int32_t), hipHostMallocDefault));
   SM_HIP(hipMalloc(&m->d_sums, 4 * sizeof(double)));
   SM_HIP(hipMalloc(&m->d_maxF, sizeof(uint64_t)));
-  SM_HIP(hipHostMalloc((void **)&m->h_ctr, 16 * sizeof(int32_t), hipHostMallocDefault));
+  SM_HIP(hipHostMalloc((void **)&m->h_ctr, kCtrTotal * sizeof(int32_t), hipHostMallocDefault));
   SM_HIP(hipHostMalloc((void **)&m->h_sums, 4 * sizeof(double), hipHostMallocDefault));
   SM_HIP(hipHostMalloc((void **)&m->h_maxF, sizeof(uint64_t), hipHostMallocDefault));
   *out = m;
@@ -278,6 +301,7 @@ extern "C" int sm_mesher_destroy(sm_mesher *m) {
   dfree(m->d_sums);
   dfree(m->d_maxF);
   if (m->h_ctr) (void)hipHostFree(m->h_ctr);
+  if (m->h_status) (void)hipHostFree(m->h_status);
   if (m->h_sums) (void)hipHostFree(m->h_sums);
   if (m->h_maxF) (void)hipHostFree(m->h_maxF);
   for (hipEvent_t e : m->ev_pool) (void)hipEventDestroy(e);
@@ -307,49 +331,161 @@ extern "C" int sm_mesher_triangulate(sm_mesher *m) {
   SM_HIP(hipSetDevice(m->device));
   int rc = triangulate(m);
   if (rc) return rc;
+  // the edge list then describes the current triangulation too
+  if ((rc = build_edges(m, true))) return rc;
   SM_HIP(hipStreamSynchronize(m->stream));
   return SM_OK;
 }
+
+namespace {
+
+// mesh_generator.py:513-521: max movement and the divergence assertion
+int check_movement(sm_mesher *m) {
+  double mf;
+  std::memcpy(&mf, m->h_maxF, sizeof(double));
+  m->last_maxdp = m->prm.delta_t * mf;
+  if (!(m->last_maxdp < 1000.0 * m->prm.h0))
+    return sm_set_error(SM_ERR_NUMERIC,
+                        "max movement indicates there's a convergence problem (%g)",
+                        m->last_maxdp);
+  return SM_OK;
+}
+
+// The second half of an iteration on a valid triangulation.
+int edges_forces_update(sm_mesher *m) {
+  int rc;
+  const bool sync = m->prm.precision == 64 || m->host_sizing != nullptr;
+  int pb = prof_begin(m);
+  if ((rc = build_edges(m, sync))) return rc;
+  prof_end(m, 2, pb);
+  if (m->host_sizing && (rc = host_sizing_pass(m))) return rc;
+  if ((rc = forces_and_update(m))) return rc;
+  SM_HIP(hipMemcpyAsync(m->h_maxF, m->d_maxF, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                        m->stream));
+  return SM_OK;
+}
+
+// One iteration with host decisions between the stages.
+int iterate_plain(sm_mesher *m) {
+  int rc;
+  if ((rc = triangulate(m))) return rc;
+  if ((rc = edges_forces_update(m))) return rc;
+  SM_HIP(hipStreamSynchronize(m->stream));
+  m->stats[0] += 1;
+  return check_movement(m);
+}
+
+constexpr int kGraphRounds = 3;
+
+// Captures one whole FP32 iteration: snapshot of the points, a batch of flip
+// rounds (later rounds skip themselves once a round flips nothing),
+// centroid filter, edges, forces, update and projection, then copies of the
+// flip status and the max movement to pinned host memory.
+int capture_iteration(sm_mesher *m) {
+  if (m->gexec) {
+    (void)hipGraphExecDestroy(m->gexec);
+    m->gexec = nullptr;
+  }
+  if (m->graph) {
+    (void)hipGraphDestroy(m->graph);
+    m->graph = nullptr;
+  }
+  hipStream_t s = m->stream;
+  const bool prof = m->prof;
+  m->prof = false;
+  SM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = SM_OK;
+  do {
+    if (hipMemcpyAsync(m->d_psnap, m->d_p, m->N * m->dim * sizeof(double),
+                       hipMemcpyDeviceToDevice, s) != hipSuccess) {
+      rc = sm_set_error(SM_ERR_HIP, "capture: snapshot copy failed");
+      break;
+    }
+    if ((rc = enqueue_flip_rounds(m, kGraphRounds, m->d_ctr + kCtrStatus, m->d_p))) break;
+    if ((rc = enqueue_snapshot(m, m->d_ctr + kCtrStatus))) break;
+    if ((rc = centroid_keep(m))) break;
+    if ((rc = build_edges(m, false))) break;
+    if ((rc = forces_and_update(m))) break;
+    if (hipMemcpyAsync(m->h_status, m->d_ctr + kCtrStatus, 8 * sizeof(int32_t),
+                       hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(m->h_maxF, m->d_maxF, sizeof(uint64_t), hipMemcpyDeviceToHost, s) !=
+            hipSuccess) {
+      rc = sm_set_error(SM_ERR_HIP, "capture: status copy failed");
+      break;
+    }
+  } while (false);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(s, &g);
+  m->prof = prof;
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return sm_set_error(SM_ERR_HIP, "capture failed: %s", hipGetErrorString(e));
+  m->graph = g;
+  SM_HIP(hipGraphInstantiate(&m->gexec, g, nullptr, nullptr, 0));
+  m->graph_T = m->T;
+  return SM_OK;
+}
+
+// One iteration as a single graph launch and one synchronisation.  The
+// iteration runs speculatively on whatever triangulation the captured flip
+// rounds reach; when they did not reach a valid Delaunay triangulation the
+// points are restored from the snapshot and the iteration is redone with
+// host decisions (more flip rounds, or an exact rebuild).
+int iterate_graph(sm_mesher *m) {
+  int rc;
+  if (!m->gexec || m->graph_T != m->T) {
+    if ((rc = capture_iteration(m))) return rc;
+  }
+  SM_HIP(hipGraphLaunch(m->gexec, m->stream));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  const int32_t *st = m->h_status;
+  account_flips(m, st);
+  const bool valid = st[2] && st[3] == 0 && st[4] == 0;
+  if (valid) {
+    m->stats[0] += 1;
+    return check_movement(m);
+  }
+  m->stats[5] += 1;  // speculative iteration rolled back
+  SM_HIP(hipMemcpyAsync(m->d_p, m->d_psnap, m->N * m->dim * sizeof(double),
+                        hipMemcpyDeviceToDevice, m->stream));
+  // more flip rounds or a kinetic repair from the last valid state; an
+  // exact predicate that could not be settled goes straight to the host
+  bool rebuild = st[4] > 0;
+  if (!rebuild) {
+    if ((rc = flip_repair2d(m, 64, &rebuild))) return rc;
+  }
+  if (rebuild && (rc = host_rebuild2d(m))) return rc;
+  if ((rc = centroid_keep(m))) return rc;
+  if ((rc = edges_forces_update(m))) return rc;
+  SM_HIP(hipStreamSynchronize(m->stream));
+  m->stats[0] += 1;
+  return check_movement(m);
+}
+
+}  // namespace
 
 extern "C" int sm_mesher_run(sm_mesher *m, int iters, double *maxdp_out) {
   SM_CHECK(m && m->N > 0, SM_ERR_STATE, "mesher_run: no points");
   SM_CHECK(iters >= 0, SM_ERR_ARG, "mesher_run: negative iteration count");
   SM_HIP(hipSetDevice(m->device));
-  const double limit = 1000.0 * m->prm.h0;
+  // the graph path needs a device-only iteration: FP32 forces, sizing on
+  // the device, and no per-kernel event timing
+  const bool graph = m->prm.precision == 32 && !m->host_sizing && !m->prof && m->have_tri &&
+                     m->use_graph;
   for (int it = 0; it < iters; ++it) {
-    int rc;
-    if ((rc = triangulate(m))) return rc;
-    if (it > 0) {
-      // the previous iteration's max movement arrived with the flips' sync
-      double mf;
-      std::memcpy(&mf, m->h_maxF, sizeof(double));
-      m->last_maxdp = m->prm.delta_t * mf;
-      if (!(m->last_maxdp < limit))
-        return sm_set_error(SM_ERR_NUMERIC,
-                            "max movement indicates there's a convergence problem (%g)",
-                            m->last_maxdp);
-    }
-    int pb = prof_begin(m);
-    if ((rc = build_edges(m))) return rc;
-    prof_end(m, 2, pb);
-    if (m->host_sizing && (rc = host_sizing_pass(m))) return rc;
-    if ((rc = forces_and_update(m))) return rc;
-    SM_HIP(hipMemcpyAsync(m->h_maxF, m->d_maxF, sizeof(uint64_t), hipMemcpyDeviceToHost,
-                          m->stream));
-    m->stats[0] += 1;
-    if (it == iters - 1) {
-      SM_HIP(hipStreamSynchronize(m->stream));
-      double mf;
-      std::memcpy(&mf, m->h_maxF, sizeof(double));
-      m->last_maxdp = m->prm.delta_t * mf;
-      if (!(m->last_maxdp < limit))
-        return sm_set_error(SM_ERR_NUMERIC,
-                            "max movement indicates there's a convergence problem (%g)",
-                            m->last_maxdp);
-    }
+    int rc = graph ? iterate_graph(m) : iterate_plain(m);
+    if (rc) return rc;
   }
   if (maxdp_out) *maxdp_out = m->last_maxdp;
   return prof_drain(m);
+}
+
+extern "C" int sm_mesher_set_graph(sm_mesher *m, int enable) {
+  SM_CHECK(m, SM_ERR_ARG, "mesher_set_graph: null handle");
+  m->use_graph = enable != 0;
+  return SM_OK;
 }
 
 extern "C" int sm_mesher_sizes(sm_mesher *m, int64_t *npoints, int64_t *ncells,
@@ -357,6 +493,12 @@ extern "C" int sm_mesher_sizes(sm_mesher *m, int64_t *npoints, int64_t *ncells,
   SM_CHECK(m, SM_ERR_ARG, "mesher_sizes: null handle");
   SM_HIP(hipSetDevice(m->device));
   SM_HIP(hipStreamSynchronize(m->stream));
+  if (m->d_ctr) {
+    // the edge count lives on the device (the graph path never reads it back)
+    int32_t e = 0;
+    SM_HIP(hipMemcpy(&e, m->d_ctr + kCtrE, sizeof(int32_t), hipMemcpyDeviceToHost));
+    m->E = e;
+  }
   if (npoints) *npoints = m->N;
   if (nedges) *nedges = m->E;
   if (ncells) {

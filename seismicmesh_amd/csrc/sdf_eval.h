@@ -140,9 +140,102 @@ __device__ inline double smooth_difference(double d1, double d2, double k) {
   return np_max(-d1, d2) + (h * h * 0.25) / k;
 }
 
+// Register-resident value stack.  Every lane of a wave walks the same
+// program, so `sp` is wave-uniform and the switches below are uniform
+// branches; no value ever goes to scratch memory.
+constexpr int kRegStack = 8;
+struct RegStack {
+  double s[kRegStack];
+  int sp = 0;
+  __device__ inline double get(int i) const {
+    switch (i) {
+      case 0: return s[0];
+      case 1: return s[1];
+      case 2: return s[2];
+      case 3: return s[3];
+      case 4: return s[4];
+      case 5: return s[5];
+      case 6: return s[6];
+      default: return s[7];
+    }
+  }
+  __device__ inline void set(int i, double v) {
+    switch (i) {
+      case 0: s[0] = v; break;
+      case 1: s[1] = v; break;
+      case 2: s[2] = v; break;
+      case 3: s[3] = v; break;
+      case 4: s[4] = v; break;
+      case 5: s[5] = v; break;
+      case 6: s[6] = v; break;
+      default: s[7] = v; break;
+    }
+  }
+};
+
+// Interpreter for programs of stack depth <= 8 and at most one pushed
+// evaluation point (every program the Python lowering emits for flat or
+// moderately nested CSG); sdf_eval_deep handles the rest.
 template <int DIM>
-__device__ inline double sdf_eval(const sm_node *__restrict__ tape, int ntape,
-                                  const double *x0) {
+__device__ inline double sdf_eval_reg(const sm_node *__restrict__ tape, int ntape,
+                                      const double *x0) {
+  RegStack st;
+  double x[3], q[3];
+  bool moved = false;
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) x[k] = x0[k];
+  for (int i = 0; i < ntape; ++i) {
+    const sm_node &nd = tape[i];
+    const int op = nd.op;
+    if (op < SM_OP_UNION) {
+      double v = sdf_primitive<DIM>(nd, moved ? q : x);
+      if (st.sp < kRegStack) st.set(st.sp++, v);
+    } else if (op <= SM_OP_DIFFERENCE) {
+      const int n = nd.nargs;
+      if (n < 1 || n > st.sp) continue;
+      const double k = nd.prm[0];
+      const int b = st.sp - n;
+      double r;
+      if (k == 0.0) {
+        r = st.get(b);
+        for (int j = 1; j < n; ++j) {
+          double a = st.get(b + j);
+          r = op == SM_OP_UNION ? np_min(r, a) : op == SM_OP_INTERSECT ? np_max(r, a)
+                                                                         : np_max(r, -a);
+        }
+      } else if (op == SM_OP_DIFFERENCE) {
+        r = st.get(b + n - 1);
+        for (int j = n - 2; j >= 0; --j) r = smooth_difference(r, st.get(b + j), k);
+      } else {
+        r = st.get(b);
+        for (int j = 1; j < n; ++j)
+          r = op == SM_OP_UNION ? smooth_union(r, st.get(b + j), k)
+                                : smooth_intersection(r, st.get(b + j), k);
+      }
+      st.sp = b;
+      st.set(st.sp++, r);
+    } else if (op == SM_OP_SDIFF_STEP) {
+      if (st.sp < 2) continue;
+      double bv = st.get(--st.sp);
+      double av = st.get(st.sp - 1);
+      st.set(st.sp - 1, smooth_difference(av, bv, nd.prm[0]));
+    } else if (op == SM_OP_PUSH_MOD) {
+#pragma unroll
+      for (int c = 0; c < DIM; ++c) {
+        double p = nd.prm[c];
+        q[c] = np_mod(x[c] + 0.5 * p, p) - 0.5 * p;
+      }
+      moved = true;
+    } else if (op == SM_OP_POP_POINT) {
+      moved = false;
+    }
+  }
+  return st.sp > 0 ? st.get(st.sp - 1) : 0.0;
+}
+
+template <int DIM>
+__device__ inline double sdf_eval_deep(const sm_node *__restrict__ tape, int ntape,
+                                       const double *x0) {
   double st[kSdfStack];
   double pts[kSdfPoints][3];
   int sp = 0, pp = 0;
@@ -209,6 +302,22 @@ __device__ inline double sdf_eval(const sm_node *__restrict__ tape, int ntape,
     }
   }
   return sp > 0 ? st[sp - 1] : 0.0;
+}
+
+// Programs are tagged by the host: nd[0].pad[0] = 1 marks a program that
+// needs the deep (scratch-stack) interpreter.  DEEP selects the interpreter
+// at compile time (hot kernels are instantiated both ways so the common
+// variant has no scratch memory at all); DYNAMIC reads the tag.  A lone
+// primitive is evaluated directly.
+enum SdfMode { kSdfReg = 0, kSdfDeep = 1, kSdfDynamic = 2 };
+
+template <int DIM, int MODE = kSdfDynamic>
+__device__ inline double sdf_eval(const sm_node *__restrict__ tape, int ntape,
+                                  const double *x0) {
+  if (ntape == 1) return sdf_primitive<DIM>(tape[0], x0);
+  if (MODE == kSdfDeep) return sdf_eval_deep<DIM>(tape, ntape, x0);
+  if (MODE == kSdfDynamic && tape[0].pad[0] != 0.0) return sdf_eval_deep<DIM>(tape, ntape, x0);
+  return sdf_eval_reg<DIM>(tape, ntape, x0);
 }
 
 }  // namespace smk

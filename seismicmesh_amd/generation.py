@@ -220,6 +220,10 @@ class Mesher2D:
     PHASES = ("flip_round", "centroid_filter", "unique_edges", "edge_geom", "l0_sums",
               "edge_force", "vertex_update")
 
+    def set_graph(self, enable=True):
+        """Run FP32 iterations as one captured graph launch each (default)."""
+        _lib.check(_lib.lib.sm_mesher_set_graph(self.h, int(enable)), "sm_mesher_set_graph")
+
     def profile(self, enable=True):
         """Enable (and clear) per-phase HIP-event timing on the handle's stream."""
         _lib.check(_lib.lib.sm_mesher_profile(self.h, int(enable)), "sm_mesher_profile")
@@ -233,7 +237,8 @@ class Mesher2D:
     def stats(self):
         s = (ctypes.c_int64 * 8)()
         _lib.check(_lib.lib.sm_mesher_stats(self.h, s, 8), "sm_mesher_stats")
-        keys = ("iterations", "flip_rounds", "flips", "host_rebuilds", "exact_predicates")
+        keys = ("iterations", "flip_rounds", "flips", "host_rebuilds", "exact_predicates",
+                "rollbacks", "kinetic_repairs")
         return {k: int(s[i]) for i, k in enumerate(keys)}
 
 

@@ -319,10 +319,13 @@ class Repeat(_Evaluable):
 
 # --------------------------------------------------------------- evaluation
 def tape_array(tape):
-    validate_tape(tape)
+    depth, points = validate_tape(tape)
     arr = (Node * len(tape))()
     for i, nd in enumerate(tape):
         arr[i] = nd
+    # programs deeper than the register interpreter's 8 values (or with
+    # nested Repeat points) are flagged for the scratch-stack interpreter
+    arr[0].pad[0] = 1.0 if (depth > 8 or points > 1) else 0.0
     return arr
 
 
@@ -352,6 +355,7 @@ def validate_tape(tape):
     """Check an SDF program's stack discipline before it reaches the GPU:
     at most 24 live values and 3 pushed points, exactly one value left."""
     sp, pp = 0, 0
+    max_sp, max_pp = 0, 0
     for nd in tape:
         op = nd.op
         if op < _lib.OP_UNION:
@@ -372,6 +376,7 @@ def validate_tape(tape):
             raise ValueError("unknown SDF opcode %d" % op)
         if sp > 24 or pp > 3 or pp < 0:
             raise ValueError("SDF program too deep for the device evaluator")
+        max_sp, max_pp = max(max_sp, sp), max(max_pp, pp)
     if sp != 1 or pp != 0:
         raise ValueError("malformed SDF program")
-    return tape
+    return max_sp, max_pp
