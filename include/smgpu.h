@@ -211,6 +211,13 @@ int sm_mesher_stream(sm_mesher *m, void **stream);
 int64_t sm_host_delaunay2d(const double *p, int64_t n, int32_t *tri, int32_t *adj,
                            int64_t cap, uint8_t *inserted);
 
+/* Delaunay tetrahedralisation of p (n, 3), exact predicates (replaces
+ * delaunay_class3.cpp:116-129, :276-344).  Writes the finite tetrahedra,
+ * positively oriented; returns their count or an error code.  inserted[i] is
+ * 0 for exact duplicates. */
+int64_t sm_host_delaunay3d(const double *p, int64_t n, int32_t *tet, int64_t cap,
+                           uint8_t *inserted);
+
 /* ------------------------------------------------------- 3D quality kernels */
 /* angles: (6*ncells) dihedral angles in radians, FP64 (fast_geometry.cpp
  * edge table {{2,3},{1,3},{1,2},{0,3},{0,2},{0,1}}). */

@@ -113,6 +113,7 @@ _PROTOS = {
     "sm_mesher_device_points": (_int, [_vp, ctypes.POINTER(_vp)]),
     "sm_mesher_stream": (_int, [_vp, ctypes.POINTER(_vp)]),
     "sm_host_delaunay2d": (_i64, [_vp, _i64, _vp, _vp, _i64, _vp]),
+    "sm_host_delaunay3d": (_i64, [_vp, _i64, _vp, _i64, _vp]),
     "sm_dihedral_angles": (_int, [_vp, _vp, _i64, _vp, _vp]),
     "sm_circumsphere_grad": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "sm_limgrad": (_int, [_vp, _int, _int, _int, _dbl, _dbl, _vp]),

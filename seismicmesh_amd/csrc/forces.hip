@@ -395,7 +395,9 @@ static int32_t *eptr_of(sm_mesher *m) {
 // device and no host round trip is needed between the edge build and the
 // forces.
 static int64_t edge_launch_bound(const sm_mesher *m) {
-  return m->dim == 2 ? 3 * m->N + 8 : m->capE;
+  // 3D: no planar bound; the 3D loop reads the edge count back (it
+  // synchronises for the host tetrahedralisation anyway)
+  return m->dim == 2 ? 3 * m->N + 8 : std::max<int64_t>(m->E, 1);
 }
 
 int build_edges(sm_mesher *m, bool sync_edges) {

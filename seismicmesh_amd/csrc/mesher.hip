@@ -70,7 +70,10 @@ int alloc_point_state(sm_mesher *m, int64_t N) {
   free_point_state(m);
   const int dim = m->dim;
   m->capN = N;
-  m->capT = 2 * N + 16;
+  // 2D: real + ghost triangles of the triangulated sphere, 2N - 2;
+  // 3D: tetrahedra of a Delaunay tetrahedralisation, ~6.5N for well-spaced
+  // points, bounded here generously
+  m->capT = dim == 2 ? 2 * N + 16 : 10 * N + 64;
   const int64_t T = m->capT;
   const int64_t npairs = T * (dim == 2 ? 3 : 6);
   m->capE = npairs;
@@ -112,22 +115,24 @@ int alloc_point_state(sm_mesher *m, int64_t N) {
   return SM_OK;
 }
 
-__global__ void k_midpoints2(int64_t E, const int32_t *edges, const double *p, double *mid) {
+__global__ void k_midpoints(int64_t E, int dim, const int32_t *edges, const double *p,
+                            double *mid) {
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
-  int a = edges[2 * e], b = edges[2 * e + 1];
-  for (int d = 0; d < 2; ++d) mid[2 * e + d] = (p[2 * a + d] + p[2 * b + d]) / 2.0;
+  int64_t a = edges[2 * e], b = edges[2 * e + 1];
+  for (int d = 0; d < dim; ++d) mid[dim * e + d] = (p[dim * a + d] + p[dim * b + d]) / 2.0;
 }
 
 int host_sizing_pass(sm_mesher *m) {
   // midpoints -> host -> user function -> device h
   const int64_t E = m->E;
+  const int dim = m->dim;
   if (E == 0) return SM_OK;
-  hipLaunchKernelGGL(k_midpoints2, dim3(blocks_for(E, 256)), dim3(256), 0, m->stream, E,
+  hipLaunchKernelGGL(k_midpoints, dim3(blocks_for(E, 256)), dim3(256), 0, m->stream, E, dim,
                      m->d_edges, m->d_p, m->d_mid);
   SM_LAUNCH_CHECK();
-  std::vector<double> mid((size_t)E * 2), h((size_t)E);
-  SM_HIP(hipMemcpyAsync(mid.data(), m->d_mid, E * 2 * sizeof(double), hipMemcpyDeviceToHost,
+  std::vector<double> mid((size_t)E * dim), h((size_t)E);
+  SM_HIP(hipMemcpyAsync(mid.data(), m->d_mid, E * dim * sizeof(double), hipMemcpyDeviceToHost,
                         m->stream));
   SM_HIP(hipStreamSynchronize(m->stream));
   int rc = m->host_sizing(m->host_sizing_ctx, mid.data(), E, h.data());
@@ -138,7 +143,36 @@ int host_sizing_pass(sm_mesher *m) {
   return SM_OK;
 }
 
+// 3D: the tetrahedralisation is rebuilt on the host from the current points
+// every iteration (as the reference does with CGAL), then uploaded.
+int host_rebuild3d(sm_mesher *m) {
+  const int64_t N = m->N;
+  std::vector<double> hp((size_t)N * 3);
+  SM_HIP(hipMemcpyAsync(hp.data(), m->d_p, N * 3 * sizeof(double), hipMemcpyDeviceToHost,
+                        m->stream));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  std::vector<int32_t> tet((size_t)m->capT * 4);
+  std::vector<uint8_t> ins((size_t)N);
+  int64_t T = sm_host_delaunay3d(hp.data(), N, tet.data(), m->capT, ins.data());
+  if (T < 0) return sm_set_error((int)T, "host 3D Delaunay failed (%lld)", (long long)T);
+  m->T = T;
+  SM_HIP(hipMemcpyAsync(m->d_tri, tet.data(), T * 4 * sizeof(int32_t), hipMemcpyHostToDevice,
+                        m->stream));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  m->stats[3] += 1;
+  m->have_tri = true;
+  return SM_OK;
+}
+
 int triangulate(sm_mesher *m) {
+  if (m->dim == 3) {
+    int rc = host_rebuild3d(m);
+    if (rc) return rc;
+    int b = prof_begin(m);
+    rc = centroid_keep(m);
+    prof_end(m, 1, b);
+    return rc;
+  }
   bool rebuild = !m->have_tri;
   if (!rebuild) {
     int rc = flip_repair2d(m, 64, &rebuild);
@@ -219,7 +253,7 @@ extern "C" int sm_mesher_create(const sm_mesher_params *prm, const sm_node *cons
                                 sm_host_sizing_fn host_sizing, void *host_sizing_ctx,
                                 int device, sm_mesher **out) {
   SM_CHECK(prm && tapes && tape_lens && sizing && out, SM_ERR_ARG, "mesher_create: null argument");
-  SM_CHECK(prm->dim == 2, SM_ERR_ARG, "mesher_create: the device force loop is 2D (got dim=%d)",
+  SM_CHECK(prm->dim == 2 || prm->dim == 3, SM_ERR_ARG, "mesher_create: dim must be 2 or 3 (got %d)",
            prm->dim);
   SM_CHECK(prm->precision == 32 || prm->precision == 64, SM_ERR_ARG,
            "mesher_create: precision must be 32 or 64");
@@ -354,7 +388,7 @@ int check_movement(sm_mesher *m) {
 // The second half of an iteration on a valid triangulation.
 int edges_forces_update(sm_mesher *m) {
   int rc;
-  const bool sync = m->prm.precision == 64 || m->host_sizing != nullptr;
+  const bool sync = m->prm.precision == 64 || m->host_sizing != nullptr || m->dim == 3;
   int pb = prof_begin(m);
   if ((rc = build_edges(m, sync))) return rc;
   prof_end(m, 2, pb);
@@ -472,8 +506,8 @@ extern "C" int sm_mesher_run(sm_mesher *m, int iters, double *maxdp_out) {
   SM_HIP(hipSetDevice(m->device));
   // the graph path needs a device-only iteration: FP32 forces, sizing on
   // the device, and no per-kernel event timing
-  const bool graph = m->prm.precision == 32 && !m->host_sizing && !m->prof && m->have_tri &&
-                     m->use_graph;
+  const bool graph = m->dim == 2 && m->prm.precision == 32 && !m->host_sizing && !m->prof &&
+                     m->have_tri && m->use_graph;
   for (int it = 0; it < iters; ++it) {
     int rc = graph ? iterate_graph(m) : iterate_plain(m);
     if (rc) return rc;

@@ -301,8 +301,8 @@ SM_HD inline int incircle_filter(const double *a, const double *b, const double 
 }
 
 // ---------------------------------------------------------------- orient3d
-// > 0 when d lies below the plane of a, b, c (a, b, c CCW seen from above),
-// i.e. the sign of det[a-d; b-d; c-d].
+// Sign of the volume det[b-a; c-a; d-a] (> 0: a positively oriented
+// tetrahedron, the convention of simp_vol), computed as -det[a-d; b-d; c-d].
 template <int N = 96>
 SM_HD inline int orient3d_exact(const double *a, const double *b, const double *c,
                                 const double *d) {
@@ -325,7 +325,8 @@ SM_HD inline int orient3d_exact(const double *a, const double *b, const double *
   Exp<N> s, det;
   add(p0, p1, s);
   add(s, p2, det);
-  return sign_of(det);
+  int sg = sign_of(det);
+  return sg == SM_PRED_UNRESOLVED ? sg : -sg;
 }
 
 SM_HD inline double orient3d_det(const double *a, const double *b, const double *c,
@@ -348,8 +349,8 @@ SM_HD inline int orient3d(const double *a, const double *b, const double *c,
   double perm;
   double det = orient3d_det(a, b, c, d, perm);
   double err = kO3dBoundA * perm;
-  if (det > err) return 1;
-  if (-det > err) return -1;
+  if (det > err) return -1;
+  if (-det > err) return 1;
   return orient3d_exact(a, b, c, d);
 }
 
@@ -446,9 +447,8 @@ SM_HD inline int insphere_filter(const double *a, const double *b, const double 
   double det = (-l[0] * c123 + l[1] * c023) + (-l[2] * c013 + l[3] * c012);
   double perm = (l[0] * p0 + l[1] * p1) + (l[2] * p2 + l[3] * p3);
   double err = 24.0 * kEps * perm;
-  // det > 0 means e lies inside the sphere of a tet with orient3d < 0 in
-  // this file's orientation convention; report "inside" as positive for
-  // orient3d(a,b,c,d) > 0.
+  // this expansion of the lifted determinant is negative when e lies inside
+  // the sphere of a tetrahedron with orient3d(a,b,c,d) > 0
   if (-det > err) return 1;
   if (det > err) return -1;
   return SM_PRED_UNRESOLVED;
@@ -466,8 +466,8 @@ SM_HD inline int orient3d_filter(const double *a, const double *b, const double 
   double perm;
   double det = orient3d_det(a, b, c, d, perm);
   double err = kO3dBoundA * perm;
-  if (det > err) return 1;
-  if (-det > err) return -1;
+  if (det > err) return -1;
+  if (-det > err) return 1;
   return SM_PRED_UNRESOLVED;
 }
 

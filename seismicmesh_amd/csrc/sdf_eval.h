@@ -196,21 +196,26 @@ __device__ inline double sdf_eval_reg(const sm_node *__restrict__ tape, int ntap
       const double k = nd.prm[0];
       const int b = st.sp - n;
       double r;
-      if (k == 0.0) {
+      // one branch per operation (a chain of selects on the opcode here was
+      // miscompiled for gfx950 into the wrong reduction)
+      if (k == 0.0 && op == SM_OP_UNION) {
         r = st.get(b);
-        for (int j = 1; j < n; ++j) {
-          double a = st.get(b + j);
-          r = op == SM_OP_UNION ? np_min(r, a) : op == SM_OP_INTERSECT ? np_max(r, a)
-                                                                         : np_max(r, -a);
-        }
+        for (int j = 1; j < n; ++j) r = np_min(r, st.get(b + j));
+      } else if (k == 0.0 && op == SM_OP_INTERSECT) {
+        r = st.get(b);
+        for (int j = 1; j < n; ++j) r = np_max(r, st.get(b + j));
+      } else if (k == 0.0) {
+        r = st.get(b);
+        for (int j = 1; j < n; ++j) r = np_max(r, -st.get(b + j));
       } else if (op == SM_OP_DIFFERENCE) {
         r = st.get(b + n - 1);
         for (int j = n - 2; j >= 0; --j) r = smooth_difference(r, st.get(b + j), k);
+      } else if (op == SM_OP_UNION) {
+        r = st.get(b);
+        for (int j = 1; j < n; ++j) r = smooth_union(r, st.get(b + j), k);
       } else {
         r = st.get(b);
-        for (int j = 1; j < n; ++j)
-          r = op == SM_OP_UNION ? smooth_union(r, st.get(b + j), k)
-                                : smooth_intersection(r, st.get(b + j), k);
+        for (int j = 1; j < n; ++j) r = smooth_intersection(r, st.get(b + j), k);
       }
       st.sp = b;
       st.set(st.sp++, r);

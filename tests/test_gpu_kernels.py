@@ -49,6 +49,21 @@ SHAPES_3D = [
 ]
 
 
+def _eval_mode(tape, dim, x, deep, dev):
+    import torch
+    from seismicmesh_amd import _lib
+    from seismicmesh_amd import geometry as geo
+    arr = geo.tape_array(tape)
+    arr[0].pad[0] = 1.0 if deep else 0.0
+    tt = torch.frombuffer(bytearray(bytes(arr)), dtype=torch.uint8).to(dev)
+    xt = _t(x, dev)
+    d = torch.empty(len(x), dtype=torch.float64, device=dev)
+    _lib.check(_lib.lib.sm_sdf_eval(tt.data_ptr(), len(arr), dim, xt.data_ptr(), len(x),
+                                    d.data_ptr(), None), "sm_sdf_eval")
+    torch.cuda.synchronize()
+    return d.cpu().numpy()
+
+
 @pytest.mark.parametrize("name,make", SHAPES_2D + SHAPES_3D, ids=[s[0] for s in SHAPES_2D + SHAPES_3D])
 def test_sdf_bit_exact(dev, name, make):
     from oracle import geometry as og
@@ -56,7 +71,11 @@ def test_sdf_bit_exact(dev, name, make):
     a, b = make(geo), make(og)
     rng = np.random.default_rng(0)
     x = rng.uniform(-2, 2, (20000, a.dim))
-    assert np.array_equal(a.eval(x), b.eval(x)), name
+    ref = b.eval(x)
+    assert np.array_equal(a.eval(x), ref), name
+    # both interpreters: registers (default) and the scratch-stack one
+    for deep in (False, True):
+        assert np.array_equal(_eval_mode(a.tape(), a.dim, x, deep, dev), ref), (name, deep)
 
 
 def test_sdf_rotated_within_rounding(dev):
