@@ -8,7 +8,7 @@
  * Device entry points take device pointers (caller-owned) and a HIP stream
  * (passed as void*; NULL = the default stream) and only enqueue work: results
  * are ready when the stream is synchronised.  Host entry points (sm_host_*)
- * run synchronously on the calling thread.  A handle (sm_mesher2d) is not
+ * run synchronously on the calling thread.  A handle (sm_mesher) is not
  * thread-safe; use one per stream.
  *
  * Reference interfaces replaced (paths relative to the SeismicMesh 3.6.2
@@ -22,17 +22,21 @@
  *                                       (fast_geometry.cpp:104-214)
  *   sm_sizing_eval                    <- sizing/mesh_size_function.py:391-408
  *                                       (RegularGridInterpolator __call__)
- *   sm_centroid_filter                <- generation/mesh_generator.py:734-738
- *   sm_bar_forces                     <- generation/mesh_generator.py:691-712
- *                                       and generation/utils.py:48-68 (dense)
- *   sm_update_project                 <- generation/mesh_generator.py:499-506,
- *                                       :762-784 (_project_points_back_newton)
- *   sm_host_delaunay2d, sm_flip_repair2d
- *                                     <- generation/cpp/delaunay_class.cpp
+ *   sm_mesher_* (the handle)          <- generation/mesh_generator.py:460-527
+ *                                       (the force loop: re-triangulation,
+ *                                       _remove_triangles_outside :734-738,
+ *                                       _compute_forces :691-712 with
+ *                                       generation/utils.py:48-68 (dense),
+ *                                       update and
+ *                                       _project_points_back_newton :762-784)
+ *   sm_mesher_find_slivers,
+ *   sm_mesher_perturb_slivers         <- generation/mesh_generator.py:222-272,
+ *                                       :532-540 (sliver_removal loop body)
+ *   sm_host_delaunay2d                <- generation/cpp/delaunay_class.cpp
  *                                       :108-121, :232-298 (CGAL insert and
  *                                       finite cells/vertices)
- *   sm_mesher2d_*                     <- generation/mesh_generator.py:460-527
- *                                       (the force loop itself)
+ *   sm_host_delaunay3d                <- generation/cpp/delaunay_class3.cpp
+ *                                       :116-129, :276-344
  *   sm_dihedral_angles                <- fast_geometry.cpp:351-452
  *   sm_circumsphere_grad              <- fast_geometry.cpp:580-703
  *   sm_limgrad                        <- sizing/cpp/FastHJ.cpp:63-187
@@ -150,7 +154,7 @@ typedef struct sm_mesher sm_mesher;
 typedef int (*sm_host_sizing_fn)(void *ctx, const double *mid, int64_t n, double *h);
 
 typedef struct {
-  int32_t dim;            /* 2 (3D handled by the 3D entry points)           */
+  int32_t dim;            /* 2 (triangles) or 3 (tetrahedra)                 */
   int32_t precision;      /* 64: FP64 forces (bit-exact with the reference's
                              order of operations); 32: FP32 forces            */
   int32_t nfix;           /* the first nfix points are fixed (pfix)          */
@@ -167,10 +171,12 @@ int sm_mesher_create(const sm_mesher_params *prm, const sm_node *const *tapes,
                      sm_host_sizing_fn host_sizing, void *host_sizing_ctx, int device,
                      sm_mesher **out);
 int sm_mesher_destroy(sm_mesher *m);
-/* Uploads points (host, (n, dim) FP64) and builds the first triangulation. */
+/* Uploads points (host, (n, dim) FP64); 2D builds the first triangulation
+ * (3D re-tetrahedralises at every triangulation step). */
 int sm_mesher_set_points(sm_mesher *m, const double *p, int64_t n);
-/* Runs `iters` force iterations (each: triangulation repair, centroid
- * filter, unique edges, bar forces, update, projection).  `maxdp_out`
+/* Runs `iters` force iterations (each: triangulation repair in 2D or an
+ * exact host re-tetrahedralisation in 3D, centroid filter, unique edges,
+ * bar forces, update, projection).  `maxdp_out`
  * (host, may be NULL) receives the last iteration's max movement. */
 int sm_mesher_run(sm_mesher *m, int iters, double *maxdp_out);
 /* FP32 handles run each iteration as one captured HIP graph (default on);
@@ -187,8 +193,10 @@ int sm_mesher_get_points(sm_mesher *m, double *p, int64_t cap);
 int sm_mesher_get_cells(sm_mesher *m, int32_t *cells, int64_t cap);
 int sm_mesher_get_edges(sm_mesher *m, int32_t *edges, int64_t cap);
 /* Statistics since creation: [0] iterations, [1] flip rounds, [2] flips,
- * [3] host rebuilds, [4] exact predicate evaluations, [5] speculative
- * (graph) iterations redone because their flips had not converged. */
+ * [3] host rebuilds (every 3D triangulation is one), [4] exact predicate
+ * evaluations, [5] speculative (graph) iterations redone because their flips
+ * had not converged, [6] kinetic repairs (points moved in sub-steps from the
+ * last valid triangulation). */
 int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats);
 /* Per-phase kernel timing with HIP events recorded on the handle's stream.
  * Phases: 0 flip rounds, 1 centroid filter, 2 unique edges + edge lists,
@@ -202,6 +210,17 @@ int sm_mesher_profile_read(sm_mesher *m, double *ms, int64_t *launches, int npha
 int sm_mesher_device_points(sm_mesher *m, double **p);
 /* Stream used by the handle (hipStream_t as void*). */
 int sm_mesher_stream(sm_mesher *m, void **stream);
+
+/* ------------------------------------------------------- sliver removal (3D)
+ * On the handle's current kept tetrahedra (after sm_mesher_triangulate):
+ * selects the cells with a dihedral angle < min_dh or > max_dh (radians)
+ * and returns their number. */
+int sm_mesher_find_slivers(sm_mesher *m, double min_dh, double max_dh, int64_t *nslivers);
+/* Moves the first vertex of each selected cell by scale * g / |g|, g the
+ * circumsphere gradient (infinite components set to 1); where several
+ * selected cells share a first vertex the last one in cell order is
+ * applied.  The selection is consumed; triangulate again before the next. */
+int sm_mesher_perturb_slivers(sm_mesher *m, double scale);
 
 /* ------------------------------------------------------- host triangulation */
 /* Delaunay triangulation of the convex hull of p (n, 2), exact predicates.

@@ -112,6 +112,8 @@ _PROTOS = {
     "sm_mesher_profile_read": (_int, [_vp, _vp, _vp, _int]),
     "sm_mesher_device_points": (_int, [_vp, ctypes.POINTER(_vp)]),
     "sm_mesher_stream": (_int, [_vp, ctypes.POINTER(_vp)]),
+    "sm_mesher_find_slivers": (_int, [_vp, _dbl, _dbl, ctypes.POINTER(_i64)]),
+    "sm_mesher_perturb_slivers": (_int, [_vp, _dbl]),
     "sm_host_delaunay2d": (_i64, [_vp, _i64, _vp, _vp, _i64, _vp]),
     "sm_host_delaunay3d": (_i64, [_vp, _i64, _vp, _i64, _vp]),
     "sm_dihedral_angles": (_int, [_vp, _vp, _i64, _vp, _vp]),

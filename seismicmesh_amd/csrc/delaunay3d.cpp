@@ -17,7 +17,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
-#include <unordered_map>
 #include <vector>
 
 #include "predicates.h"
@@ -55,6 +54,14 @@ inline uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
     for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
   return h;
 }
+
+// kOther[a][b]: the two slots of a tetrahedron other than a and b (a != b)
+constexpr int kOther[4][4][2] = {
+    {{0, 0}, {2, 3}, {1, 3}, {1, 2}},
+    {{2, 3}, {0, 0}, {0, 3}, {0, 2}},
+    {{1, 3}, {0, 3}, {0, 0}, {0, 1}},
+    {{1, 2}, {0, 2}, {0, 1}, {0, 0}},
+};
 
 struct Builder3 {
   std::vector<double> P;  // points + 4 far vertices, (n + 4, 3)
@@ -109,15 +116,29 @@ struct Builder3 {
     return -2;
   }
 
+  // per-insertion scratch, reused across insertions
+  struct BF {
+    int t, i, out, back;  // cavity tet, its face, the outside tet, its face
+  };
+  std::vector<int> cav, stack, ids;
+  std::vector<BF> bnd;
+  std::vector<Tet> nt;
+  // open-addressing table of the new tetrahedra's faces through q, keyed by
+  // the face's edge opposite q; slots hold (key, tet, local face)
+  struct Slot {
+    uint64_t key;
+    int t, j;
+  };
+  std::vector<Slot> table;
+  std::vector<int> used;
+
   bool insert(int qi) {
     int t0 = locate(qi);
     if (t0 < 0) return false;
     const double *q = pt(qi);
-    struct BF {
-      int t, i, out;
-    };
-    std::vector<int> cav, stack{t0};
-    std::vector<BF> bnd;
+    cav.clear();
+    bnd.clear();
+    stack.assign(1, t0);
     if ((size_t)T.size() > mark.size()) mark.resize(T.size() * 2, 0);
     mark[t0] = 1;
     while (!stack.empty()) {
@@ -131,14 +152,20 @@ struct Builder3 {
           mark[nb] = 1;
           stack.push_back(nb);
         } else {
-          bnd.push_back({t, i, nb});
+          // the outside tet's face back to t, found before any relinking
+          int back = -1;
+          if (nb >= 0)
+            for (int j = 0; j < 4; ++j)
+              if (T[nb].n[j] == t) back = j;
+          bnd.push_back({t, i, nb, back});
         }
       }
     }
     // new tetrahedra: the cavity tetrahedron with the boundary face's
     // opposite vertex replaced by q (same orientation)
-    std::vector<Tet> nt(bnd.size());
-    for (size_t k = 0; k < bnd.size(); ++k) {
+    const size_t nb = bnd.size(), nc = cav.size();
+    nt.resize(nb);
+    for (size_t k = 0; k < nb; ++k) {
       Tet r = T[bnd[k].t];
       r.v[bnd[k].i] = qi;
       for (int j = 0; j < 4; ++j) r.n[j] = -1;
@@ -146,56 +173,47 @@ struct Builder3 {
       nt[k] = r;
     }
     for (int t : cav) mark[t] = 0;
-    std::vector<int> ids(bnd.size());
-    for (size_t k = 0; k < bnd.size(); ++k) ids[k] = k < cav.size() ? cav[k] : -1;
-    for (size_t k = cav.size(); k < bnd.size(); ++k) ids[k] = new_tet();
-    for (size_t k = bnd.size(); k < cav.size(); ++k) free_slots.push_back(cav[k]);
+    ids.resize(nb);
+    for (size_t k = 0; k < nb && k < nc; ++k) ids[k] = cav[k];
+    for (size_t k = nc; k < nb; ++k) ids[k] = new_tet();
+    for (size_t k = nb; k < nc; ++k) free_slots.push_back(cav[k]);
     if ((size_t)T.size() > mark.size()) mark.resize(T.size() * 2, 0);
-    for (size_t k = 0; k < bnd.size(); ++k) T[ids[k]] = nt[k];
+    for (size_t k = 0; k < nb; ++k) T[ids[k]] = nt[k];
     // outside neighbours point back to the new tetrahedra
-    for (size_t k = 0; k < bnd.size(); ++k) {
-      int o = bnd[k].out;
-      if (o < 0) continue;
-      const Tet &r = T[ids[k]];
-      int f[3], c = 0;
-      for (int j = 0; j < 4; ++j)
-        if (j != bnd[k].i) f[c++] = r.v[j];
-      Tet &ot = T[o];
-      for (int j = 0; j < 4; ++j) {
-        int cnt = 0;
-        for (int l = 0; l < 4; ++l)
-          if (l != j && (ot.v[l] == f[0] || ot.v[l] == f[1] || ot.v[l] == f[2])) ++cnt;
-        if (cnt == 3) {
-          ot.n[j] = ids[k];
-          break;
-        }
-      }
-    }
+    for (size_t k = 0; k < nb; ++k)
+      if (bnd[k].out >= 0) T[bnd[k].out].n[bnd[k].back] = ids[k];
     // link new tetrahedra through their faces containing q: the face of
     // new tet k opposite vertex j (j != slot of q) is {q} + the boundary
-    // face's edge opposite that vertex; key it by the edge
-    std::unordered_map<uint64_t, std::pair<int, int>> faces;
-    faces.reserve(bnd.size() * 3);
-    for (size_t k = 0; k < bnd.size(); ++k) {
+    // face's edge opposite that vertex; each such edge is shared by exactly
+    // two new tetrahedra
+    size_t cap = 64;
+    while (cap < 4 * 3 * nb) cap <<= 1;
+    if (table.size() < cap) table.assign(cap, Slot{~0ull, -1, -1});
+    const size_t mask = cap - 1;
+    used.clear();
+    for (size_t k = 0; k < nb; ++k) {
       const Tet &r = T[ids[k]];
-      int qs = bnd[k].i;
+      const int qs = bnd[k].i;
       for (int j = 0; j < 4; ++j) {
         if (j == qs) continue;
-        int e[2], c = 0;
-        for (int l = 0; l < 4; ++l)
-          if (l != j && l != qs) e[c++] = r.v[l];
-        uint64_t key = ((uint64_t)(uint32_t)std::min(e[0], e[1]) << 32) |
-                       (uint32_t)std::max(e[0], e[1]);
-        auto it = faces.find(key);
-        if (it == faces.end()) {
-          faces.emplace(key, std::make_pair((int)ids[k], j));
-        } else {
-          int ot = it->second.first, oj = it->second.second;
+        // the two slots other than j and qs
+        const int s0 = kOther[qs][j][0], s1 = kOther[qs][j][1];
+        const int e[2] = {r.v[s0], r.v[s1]};
+        const uint64_t key = ((uint64_t)(uint32_t)std::min(e[0], e[1]) << 32) |
+                             (uint32_t)std::max(e[0], e[1]);
+        size_t h = (size_t)((key * 0x9E3779B97F4A7C15ull) >> 32) & mask;
+        while (table[h].key != ~0ull && table[h].key != key) h = (h + 1) & mask;
+        if (table[h].key == key) {
+          const int ot = table[h].t, oj = table[h].j;
           T[ids[k]].n[j] = ot;
           T[ot].n[oj] = ids[k];
+        } else {
+          table[h] = Slot{key, ids[k], j};
+          used.push_back((int)h);
         }
       }
     }
+    for (int h : used) table[h].key = ~0ull;
     last = ids[0];
     return true;
   }

@@ -88,6 +88,16 @@ template <typename T> __device__ inline T zero_len();
 template <> __device__ inline double zero_len<double>() { return 2.220446049250313e-16; }
 template <> __device__ inline float zero_len<float>() { return 2.220446049250313e-16f; }
 
+// x^3 rounded once: x*x kept as a double-double (fma remainder), times x
+// with the product's remainder folded back before the final rounding.
+__device__ inline double cube_rn(double x) {
+  double p = x * x;
+  double pe = fma(x, x, -p);
+  double hi = p * x;
+  double lo = fma(p, x, -hi) + pe * x;
+  return hi + lo;
+}
+
 // Per-edge bar length and sizing; per-block partial sums of L^d and h^d
 // (FP64) for the fast path, or the per-edge powers for the exact path.
 template <int DIM, typename T, bool EXACT>
@@ -120,8 +130,8 @@ __global__ void k_edge_geom(const int32_t *Ep, const int32_t *edges, const doubl
         Lp[e] = (double)L * (double)L;
         hp[e] = (double)h * (double)h;
       } else {
-        Lp[e] = (double)L * (double)L * (double)L;
-        hp[e] = (double)h * (double)h * (double)h;
+        Lp[e] = cube_rn((double)L);
+        hp[e] = cube_rn((double)h);
       }
     } else {
       double Ld = L, hd = h;

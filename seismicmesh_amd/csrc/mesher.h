@@ -79,6 +79,12 @@ struct sm_mesher {
   double *h_sums = nullptr;
   uint64_t *h_maxF = nullptr;
 
+  // sliver removal: per-cell flags then ranks, per-vertex winning rank
+  int32_t *d_sl_rank = nullptr, *d_sl_win = nullptr, *d_sl_total = nullptr;
+  void *d_sl_ws = nullptr;
+  int64_t sl_capT = 0, sl_capN = 0;
+  int64_t nsl = -1;  // selected slivers, -1 before a selection
+
   int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int round = 0;
   bool have_tri = false;

@@ -1,6 +1,6 @@
 // The force-loop handle: device state and the per-iteration sequence of
 // SeismicMesh/generation/mesh_generator.py:460-527 (re-triangulate, filter
-// cells, forces, update, project), for 2D domains.
+// cells, forces, update, project), for 2D and 3D domains.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -57,6 +57,13 @@ void free_point_state(sm_mesher *m) {
   dfree(m->d_adj_s);
   dfree(m->d_pvalid);
   dfree(m->d_pstage);
+  dfree(m->d_sl_rank);
+  dfree(m->d_sl_win);
+  dfree(m->d_sl_total);
+  if (m->d_sl_ws) (void)hipFree(m->d_sl_ws);
+  m->d_sl_ws = nullptr;
+  m->sl_capT = m->sl_capN = 0;
+  m->nsl = -1;
   m->have_snap = false;
   // a captured iteration refers to these buffers
   if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
@@ -165,6 +172,7 @@ int host_rebuild3d(sm_mesher *m) {
 }
 
 int triangulate(sm_mesher *m) {
+  m->nsl = -1;  // a sliver selection refers to the previous cells
   if (m->dim == 3) {
     int rc = host_rebuild3d(m);
     if (rc) return rc;
@@ -355,7 +363,8 @@ extern "C" int sm_mesher_set_points(sm_mesher *m, const double *p, int64_t n) {
   SM_HIP(hipMemcpyAsync(m->d_p, p, n * m->dim * sizeof(double), hipMemcpyHostToDevice,
                         m->stream));
   m->have_tri = false;
-  int rc = host_rebuild2d(m);
+  // 3D rebuilds its tetrahedralisation at every triangulate()
+  int rc = m->dim == 2 ? host_rebuild2d(m) : SM_OK;
   if (rc) return rc;
   return SM_OK;
 }

@@ -27,7 +27,7 @@ from . import geometry as geo
 from . import mesh_utils as mu
 from .sizing import GridSizing, SizeFunction
 
-__all__ = ["generate_mesh", "sliver_removal", "Mesher2D"]
+__all__ = ["generate_mesh", "sliver_removal", "Mesher"]
 
 _KNOWN = {
     "verbose", "max_iter", "seed", "perform_checks", "pfix", "axis", "points", "domain",
@@ -124,16 +124,18 @@ def create_staggered_grid(h0, dim, bbox):
     return points.reshape(dim, -1).T
 
 
-class Mesher2D:
-    """Owns one libsmgpu force-loop handle (2D)."""
+class Mesher:
+    """Owns one libsmgpu force-loop handle (2D triangles or 3D tetrahedra)."""
 
-    def __init__(self, levels, sizing, h0, delta_t, geps, deps, nfix, precision=64, device=0):
+    def __init__(self, levels, sizing, h0, delta_t, geps, deps, nfix, precision=64, device=0,
+                 dim=2):
         _lib.require_gpu()
+        self.dim = dim
         self._tapes = [geo.tape_array(l.tape()) for l in levels]
         ptrs = (ctypes.c_void_p * len(levels))(*[ctypes.addressof(t) for t in self._tapes])
         lens = (ctypes.c_int * len(levels))(*[len(t) for t in self._tapes])
         prm = _lib.MesherParams()
-        prm.dim = 2
+        prm.dim = dim
         prm.precision = precision
         prm.nfix = nfix
         prm.nlevels = len(levels)
@@ -141,10 +143,10 @@ class Mesher2D:
         prm.delta_t = delta_t
         prm.geps = geps
         prm.deps = deps
-        prm.L0mult = 1 + 0.4 / 2 ** (2 - 1)
+        prm.L0mult = 1 + 0.4 / 2 ** (dim - 1)
         self._sizing = sizing
         g = _lib.Grid()
-        g.dim = 2
+        g.dim = dim
         cb = _lib.HOST_SIZING_FN()
         if sizing.kind == "const":
             g.hconst = float(sizing.value)
@@ -153,7 +155,7 @@ class Mesher2D:
         else:
             def _cb(ctx, mid, n, h):
                 try:
-                    x = np.ctypeslib.as_array(mid, shape=(n, 2)).copy()
+                    x = np.ctypeslib.as_array(mid, shape=(n, dim)).copy()
                     out = np.ctypeslib.as_array(h, shape=(n,))
                     out[:] = np.asarray(sizing.func(x), dtype=float).reshape(-1)
                     return 0
@@ -201,13 +203,13 @@ class Mesher2D:
 
     def points(self):
         n, _, _ = self.sizes()
-        p = np.empty((n, 2))
+        p = np.empty((n, self.dim))
         _lib.check(_lib.lib.sm_mesher_get_points(self.h, p.ctypes.data, n), "sm_mesher_get_points")
         return p
 
     def cells(self):
         _, c, _ = self.sizes()
-        t = np.empty((c, 3), dtype=np.int32)
+        t = np.empty((c, self.dim + 1), dtype=np.int32)
         _lib.check(_lib.lib.sm_mesher_get_cells(self.h, t.ctypes.data, c), "sm_mesher_get_cells")
         return t
 
@@ -305,10 +307,13 @@ def prepare(domain, edge_length, **kwargs):
     if not isinstance(bbox, tuple):
         raise ValueError("`bbox` must be a tuple")
     dim = int(len(bbox) / 2)
-    if dim != 2:
-        raise NotImplementedError("the device force loop meshes 2D domains")
+    if dim not in (2, 3):
+        raise ValueError("`bbox` must describe a 2D or 3D box")
+    # mesh_generator.py:388-390: corners are not constrained in 3D with graded sizing
+    if fh.kind != "const" and dim == 3:
+        corners = None
     if bbox1 is not None and bbox0 != bbox1:
-        dom = geo.Rectangle(bbox)
+        dom = geo.Rectangle(bbox) if dim == 2 else geo.Cube(bbox)
     bbox = np.array(bbox).reshape(-1, 2)
     h0 = hmin if hmin is not None else opts["h0"]
     if h0 < 0:
@@ -360,12 +365,15 @@ def prepare(domain, edge_length, **kwargs):
 
 def make_mesher(st):
     """A force-loop handle for a prepared problem (initial points not set)."""
-    return Mesher2D(st["levels"], st["fh"], st["h0"], st["delta_t"], st["geps"], st["deps"],
-                    st["nfix"], precision=st["opts"]["precision"])
+    return Mesher(st["levels"], st["fh"], st["h0"], st["delta_t"], st["geps"], st["deps"],
+                  st["nfix"], precision=st["opts"]["precision"], dim=st["dim"])
+
+
+Mesher2D = Mesher
 
 
 def generate_mesh(domain, edge_length, comm=None, **kwargs):
-    """Generate a 2D mesh: see mesh_generator.py:291-529 for the semantics."""
+    """Generate a 2D or 3D mesh: see mesh_generator.py:291-529 for the semantics."""
     st = prepare(domain, edge_length, **kwargs)
     opts, dom, h0, deps = st["opts"], st["dom"], st["h0"], st["deps"]
     v1 = opts["verbose"] >= 1

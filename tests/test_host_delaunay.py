@@ -1,4 +1,4 @@
-"""Host exact Delaunay (C ABI sm_host_delaunay2d) against the exact
+"""Host exact Delaunay (C ABI sm_host_delaunay2d/3d) against the exact
 brute-force oracle and Qhull (CPU)."""
 import numpy as np
 import pytest
@@ -70,3 +70,50 @@ def test_large_matches_qhull():
     real = tri[(tri >= 0).all(1)]
     want = {tuple(sorted(s)) for s in Delaunay(p).simplices.tolist()}
     assert {tuple(sorted(s)) for s in real.tolist()} == want
+
+
+# ----------------------------------------------------------------- 3D
+def _tets(p):
+    from seismicmesh_amd import _lib
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    n = len(p)
+    cap = 10 * n + 64
+    t = np.zeros((cap, 4), np.int32)
+    ins = np.zeros(n, np.uint8)
+    m = _lib.lib.sm_host_delaunay3d(p.ctypes.data, n, t.ctypes.data, cap, ins.ctypes.data)
+    assert m >= 0, m
+    return t[:m], ins
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_3d_empty_spheres_exact(seed):
+    # brute force against the exact (rational) predicates on small inputs
+    from oracle.predicates import insphere, orient3d
+    rng = np.random.default_rng(seed)
+    p = rng.random((int(rng.integers(5, 30)), 3))
+    t, ins = _tets(p)
+    assert ins.all()
+    for tt in t:
+        a, b, c, d = p[tt]
+        assert orient3d(a, b, c, d) > 0  # positively oriented
+        for k in range(len(p)):
+            if k not in tt:
+                assert insphere(a, b, c, d, p[k]) <= 0
+    # the union of the cells is the convex hull: volumes add up
+    from scipy.spatial import ConvexHull
+    vol = np.abs(np.einsum("ij,ij->i", np.cross(p[t[:, 1]] - p[t[:, 0]], p[t[:, 2]] - p[t[:, 0]]),
+                           p[t[:, 3]] - p[t[:, 0]])).sum() / 6
+    assert abs(vol - ConvexHull(p).volume) < 1e-12
+
+
+def test_3d_matches_qhull_and_skips_duplicates():
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(7)
+    p = rng.random((3000, 3))
+    t, _ = _tets(p)
+    want = {tuple(sorted(x)) for x in Delaunay(p).simplices.tolist()}
+    assert {tuple(sorted(x)) for x in t.tolist()} == want
+    q = np.vstack((p[:50], p[:10]))
+    t2, ins = _tets(q)
+    assert ins[:50].all() and not ins[50:].any()
+    assert t2.max() < 50
