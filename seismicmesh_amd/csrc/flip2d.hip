@@ -38,43 +38,88 @@ constexpr int kGhostOri = 2;
 __device__ inline const double *PT(const double *p, int v) { return p + 2 * (int64_t)v; }
 
 // A round's counter block (8 ints): [0] flips, [1] inverted triangles,
-// [2] orientation queue, [3] in-circle queue, [5] unresolved exact
-// predicates, [6] round id (unique over the handle's life), [7] skip.
-// A round is skipped when the previous one flipped nothing, was skipped or
-// hit an unresolved predicate, so a batch of rounds can be enqueued (or
-// replayed from a graph) without the host looking at the counters.
-__global__ void k_round_begin(const int32_t *prev, int32_t *ctr, int32_t *round_counter) {
-  int skip = prev != nullptr && (prev[7] || prev[0] == 0 || prev[5] > 0);
-  for (int i = 0; i < 6; ++i) ctr[i] = 0;
-  ctr[7] = skip;
-  if (!skip) ctr[6] = ++(*round_counter);
+// [2] orientation queue, [3] in-circle queue, [5] unresolved predicates,
+// [6] round id (unique over the handle's life), [7] skip.  A round is
+// skipped when the previous one flipped nothing, was skipped or hit an
+// unresolved predicate, so a batch of rounds can be enqueued (or replayed
+// from a graph) without the host looking at the counters.  Round 0 of a
+// batch is opened by k_batch_begin; round r > 0 takes its skip flag and id
+// from round r - 1's final counters (its first kernel computes and stores
+// them), and its counters are zeroed by round r - 1's last kernel.
+__global__ void k_batch_begin(int32_t *c0, const int32_t *round_counter) {
+  for (int i = 0; i < 6; ++i) c0[i] = 0;
+  c0[6] = *round_counter + 1;
+  c0[7] = 0;
 }
 
-__global__ void k_clear_claims(int64_t T, uint64_t *claim, const int32_t *ctr) {
-  if (ctr[7]) return;
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < T) claim[t] = 0ull;
+// Neighbour links of triangle s after the flips of round `round`: a link to
+// a flipped triangle that no longer holds the shared edge now belongs to its
+// partner.
+__device__ inline void fix_links(int64_t s, const int32_t *tri, int32_t *adj,
+                                 const int32_t *partner, const int32_t *mark, int round) {
+  for (int k = 0; k < 3; ++k) {
+    int nb = adj[3 * s + k];
+    if (mark[nb] != round) continue;
+    int u = tri[3 * s + (k + 1) % 3], v = tri[3 * s + (k + 2) % 3];
+    bool hu = false, hv = false;
+    for (int j = 0; j < 3; ++j) {
+      int w = tri[3 * (int64_t)nb + j];
+      hu |= (w == u);
+      hv |= (w == v);
+    }
+    if (!(hu && hv)) adj[3 * s + k] = partner[nb];
+  }
 }
 
-__global__ void k_orient(int64_t T, const int32_t *tri, const double *p, int32_t *ori,
-                         int32_t *queue, int32_t *ctr) {
-  if (ctr[7]) return;
+// Round r's first kernel: the link fix of round r - 1, then (unless round r
+// is skipped) the claim reset and the orientation of every triangle.
+// Uncertain orientations are queued for the exact pass, or, with
+// queue_exact == 0, counted as unresolved (the batch then reports failure).
+__global__ void k_fix_orient(int64_t T, const int32_t *tri, int32_t *adj,
+                             const int32_t *partner, const int32_t *mark, const double *p,
+                             int32_t *ori, uint64_t *claim, int32_t *queue,
+                             const int32_t *prev, int32_t *c, int queue_exact) {
+  int skip = c[7], id = c[6];
+  if (prev) {
+    skip = prev[7] || prev[0] == 0 || prev[5] > 0;
+    id = prev[6] + 1;
+  }
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (prev && t == 0) {
+    c[6] = id;
+    c[7] = skip;
+  }
   if (t >= T) return;
-  int a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
-  if (a < 0 || b < 0 || c < 0) {
+  if (prev && !prev[7]) fix_links(t, tri, adj, partner, mark, prev[6]);
+  if (skip) return;
+  claim[t] = 0ull;
+  int a = tri[3 * t], b = tri[3 * t + 1], cc = tri[3 * t + 2];
+  if (a < 0 || b < 0 || cc < 0) {
     ori[t] = kGhostOri;
     return;
   }
-  int o = smpred::orient2d_filter(PT(p, a), PT(p, b), PT(p, c));
+  int o = smpred::orient2d_filter(PT(p, a), PT(p, b), PT(p, cc));
+  if (o == SM_PRED_UNRESOLVED) o = smpred::orient2d_if_exact(PT(p, a), PT(p, b), PT(p, cc));
   if (o == SM_PRED_UNRESOLVED) {
-    int slot = atomicAdd(&ctr[2], 1);
-    queue[slot] = (int32_t)t;
+    if (queue_exact) {
+      int slot = atomicAdd(&c[2], 1);
+      queue[slot] = (int32_t)t;
+    } else {
+      atomicAdd(&c[5], 1);
+    }
     ori[t] = 0;  // provisional, settled by k_orient_exact
     return;
   }
   ori[t] = o;
-  if (o <= 0) atomicAdd(&ctr[1], 1);
+  if (o <= 0) atomicAdd(&c[1], 1);
+}
+
+// The link fix of a batch's last round.
+__global__ void k_fix_last(int64_t T, const int32_t *tri, int32_t *adj, const int32_t *partner,
+                           const int32_t *mark, const int32_t *c) {
+  if (c[7]) return;
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < T) fix_links(t, tri, adj, partner, mark, c[6]);
 }
 
 __global__ void k_orient_exact(const int32_t *tri, const double *p, int32_t *ori,
@@ -125,6 +170,7 @@ __device__ inline int in_circle_of(const double *p, int x, int a, int b, int y) 
   if (y < 0) return 0;  // INF is never inside a finite circle
   if (x >= 0 && a >= 0 && b >= 0) {
     int s = smpred::incircle_filter(PT(p, x), PT(p, a), PT(p, b), PT(p, y));
+    if (s == SM_PRED_UNRESOLVED) s = smpred::incircle_if_exact(PT(p, x), PT(p, a), PT(p, b), PT(p, y));
     if (s == SM_PRED_UNRESOLVED) return s;
     return s > 0;
   }
@@ -134,6 +180,7 @@ __device__ inline int in_circle_of(const double *p, int x, int a, int b, int y) 
   else if (a < 0) { u = b; v = x; }
   else { u = x; v = a; }
   int s = smpred::orient2d_filter(PT(p, u), PT(p, v), PT(p, y));
+  if (s == SM_PRED_UNRESOLVED) s = smpred::orient2d_if_exact(PT(p, u), PT(p, v), PT(p, y));
   if (s == SM_PRED_UNRESOLVED) return s;
   return s > 0;
 }
@@ -142,13 +189,14 @@ __device__ inline int in_circle_of(const double *p, int x, int a, int b, int y) 
 __device__ inline int valid_new(const double *p, int a, int b, int c) {
   if (a < 0 || b < 0 || c < 0) return 1;
   int s = smpred::orient2d_filter(PT(p, a), PT(p, b), PT(p, c));
+  if (s == SM_PRED_UNRESOLVED) s = smpred::orient2d_if_exact(PT(p, a), PT(p, b), PT(p, c));
   if (s == SM_PRED_UNRESOLVED) return 0;
   return s > 0;
 }
 
 __global__ void k_mark(int64_t T, const int32_t *tri, const int32_t *adj, const int32_t *ori,
                        const double *p, uint64_t *claim, uint64_t *cand, int32_t *queue,
-                       int32_t *ctr) {
+                       int32_t *ctr, int queue_exact) {
   if (ctr[7]) return;
   const int round = ctr[6];
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -179,8 +227,12 @@ __global__ void k_mark(int64_t T, const int32_t *tri, const int32_t *adj, const 
       } else {
         int s = in_circle_of(p, q.x, q.a, q.b, q.y);
         if (s == SM_PRED_UNRESOLVED) {
-          int slot = atomicAdd(&ctr[3], 1);
-          queue[slot] = (int32_t)(3 * t + k);
+          if (queue_exact) {
+            int slot = atomicAdd(&ctr[3], 1);
+            queue[slot] = (int32_t)(3 * t + k);
+          } else {
+            atomicAdd(&ctr[5], 1);
+          }
         } else if (s) {
           prio = (1ull << 32) | scramble((uint32_t)(3 * t + k), round);
         }
@@ -231,7 +283,9 @@ __global__ void k_mark_exact(const int32_t *tri, const int32_t *adj, const doubl
 }
 
 __global__ void k_apply(int64_t T, int32_t *tri, int32_t *adj, const uint64_t *claim,
-                        const uint64_t *cand, int32_t *partner, int32_t *mark, int32_t *ctr) {
+                        const uint64_t *cand, int32_t *partner, int32_t *mark, int32_t *ctr,
+                        int32_t *next) {
+  if (next && blockIdx.x == 0 && threadIdx.x < 6) next[threadIdx.x] = 0;
   if (ctr[7]) return;
   const int round = ctr[6];
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -267,31 +321,12 @@ __global__ void k_apply(int64_t T, int32_t *tri, int32_t *adj, const uint64_t *c
   }
 }
 
-__global__ void k_fix(int64_t T, const int32_t *tri, int32_t *adj, const int32_t *partner,
-                      const int32_t *mark, const int32_t *ctr) {
-  if (ctr[7]) return;
-  const int round = ctr[6];
-  int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= T) return;
-  for (int k = 0; k < 3; ++k) {
-    int nb = adj[3 * s + k];
-    if (mark[nb] != round) continue;
-    int u = tri[3 * s + (k + 1) % 3], v = tri[3 * s + (k + 2) % 3];
-    bool hu = false, hv = false;
-    for (int j = 0; j < 3; ++j) {
-      int w = tri[3 * (int64_t)nb + j];
-      hu |= (w == u);
-      hv |= (w == v);
-    }
-    if (!(hu && hv)) adj[3 * s + k] = partner[nb];
-  }
-}
-
 // Summary of a batch of rounds into status[0..7]: [0] rounds executed,
 // [1] flips, [2] converged (the last executed round flipped nothing),
 // [3] inverted triangles seen by that round, [4] unresolved predicates,
 // [5] exact-predicate evaluations.
-__global__ void k_flip_status(const int32_t *ctr, int nrounds, int32_t *status) {
+__global__ void k_flip_status(const int32_t *ctr, int nrounds, int32_t *status,
+                              int32_t *round_counter) {
   int last = -1, flips = 0, unres = 0, exact = 0;
   for (int r = 0; r < nrounds; ++r) {
     const int32_t *c = ctr + 8 * r;
@@ -307,6 +342,7 @@ __global__ void k_flip_status(const int32_t *ctr, int nrounds, int32_t *status) 
   status[3] = last >= 0 ? ctr[8 * last + 1] : 0;
   status[4] = unres;
   status[5] = exact;
+  if (last >= 0) *round_counter = ctr[8 * last + 6];
 }
 
 }  // namespace
@@ -352,7 +388,7 @@ int repair_on(sm_mesher *m, const double *pts, int max_rounds) {
   hipStream_t s = m->stream;
   int32_t *dstat = m->d_ctr + kCtrStatus;
   for (int done = 0; done < max_rounds;) {
-    int rc = enqueue_flip_rounds(m, 4, dstat, pts);
+    int rc = enqueue_flip_rounds(m, 4, dstat, pts, true);
     if (rc) return rc > 0 ? -rc : rc - 100;
     if (hipMemcpyAsync(m->h_ctr, dstat, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
@@ -414,33 +450,42 @@ bool kinetic_repair(sm_mesher *m) {
   return true;
 }
 
-int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double *pts) {
+int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double *pts,
+                        bool exact) {
+  SM_CHECK(nrounds >= 1 && nrounds <= kMaxBatchRounds, SM_ERR_ARG,
+           "flip rounds: batch of %d rounds", nrounds);
   hipStream_t s = m->stream;
   const int64_t T = m->T;
   const int B = 256;
   const unsigned g = blocks_for(T, B);
   int32_t *base = m->d_ctr + kCtrRounds;
+  int32_t *rc = m->d_ctr + kCtrRoundId;
+  const int qx = exact ? 1 : 0;
+  hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(1), 0, s, base, (const int32_t *)rc);
   for (int r = 0; r < nrounds; ++r) {
     int32_t *c = base + 8 * r;
     const int32_t *prev = r == 0 ? nullptr : c - 8;
-    hipLaunchKernelGGL(k_round_begin, dim3(1), dim3(1), 0, s, prev, c, m->d_ctr + kCtrRoundId);
-    hipLaunchKernelGGL(k_clear_claims, dim3(g), dim3(B), 0, s, T, m->d_claim, (const int32_t *)c);
-    hipLaunchKernelGGL(k_orient, dim3(g), dim3(B), 0, s, T, m->d_tri, pts, m->d_ori,
-                       m->d_queue, c);
-    hipLaunchKernelGGL(k_orient_exact, dim3(1), dim3(64), 0, s, m->d_tri, pts, m->d_ori,
-                       m->d_queue, c);
+    int32_t *next = r + 1 < nrounds ? c + 8 : nullptr;
+    hipLaunchKernelGGL(k_fix_orient, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj,
+                       m->d_partner, m->d_mark, pts, m->d_ori, m->d_claim, m->d_queue, prev, c,
+                       qx);
+    if (exact)
+      hipLaunchKernelGGL(k_orient_exact, dim3(8), dim3(64), 0, s, m->d_tri, pts, m->d_ori,
+                         m->d_queue, c);
     hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, pts,
-                       m->d_claim, m->d_cand, m->d_queue + T, c);
-    hipLaunchKernelGGL(k_mark_exact, dim3(1), dim3(64), 0, s, m->d_tri, m->d_adj, pts,
-                       m->d_claim, m->d_cand, m->d_queue + T, c);
+                       m->d_claim, m->d_cand, m->d_queue + T, c, qx);
+    if (exact)
+      hipLaunchKernelGGL(k_mark_exact, dim3(8), dim3(64), 0, s, m->d_tri, m->d_adj, pts,
+                         m->d_claim, m->d_cand, m->d_queue + T, c);
     hipLaunchKernelGGL(k_apply, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_claim,
-                       m->d_cand, m->d_partner, m->d_mark, c);
-    hipLaunchKernelGGL(k_fix, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_partner,
-                       m->d_mark, (const int32_t *)c);
+                       m->d_cand, m->d_partner, m->d_mark, c, next);
     SM_LAUNCH_CHECK();
   }
+  const int32_t *lastc = base + 8 * (nrounds - 1);
+  hipLaunchKernelGGL(k_fix_last, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_partner,
+                     m->d_mark, lastc);
   hipLaunchKernelGGL(k_flip_status, dim3(1), dim3(1), 0, s, (const int32_t *)base, nrounds,
-                     status);
+                     status, rc);
   SM_LAUNCH_CHECK();
   return SM_OK;
 }
@@ -459,7 +504,7 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
   for (int done = 0; done < max_rounds;) {
     const int batch = 4;
     const int pb = prof_begin(m);
-    int rc = enqueue_flip_rounds(m, batch, dstat, m->d_p);
+    int rc = enqueue_flip_rounds(m, batch, dstat, m->d_p, true);
     if (rc) return rc;
     prof_end(m, 0, pb);
     SM_HIP(hipMemcpyAsync(m->h_ctr, dstat, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s));

@@ -114,7 +114,10 @@ constexpr int kMaxBatchRounds = 64;
 constexpr int kCtrTotal = kCtrRounds + 8 * kMaxBatchRounds;
 constexpr int kCtrScratchHost = 64;  // pinned host staging inside h_ctr
 
-int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double *pts);
+// exact: settle uncertain predicates on the device; otherwise count them
+// as unresolved (the batch then reports failure)
+int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double *pts,
+                        bool exact);
 void account_flips(sm_mesher *m, const int32_t *status);
 int enqueue_snapshot(sm_mesher *m, const int32_t *status);
 int snapshot_valid(sm_mesher *m);

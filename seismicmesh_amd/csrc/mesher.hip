@@ -418,7 +418,7 @@ int iterate_plain(sm_mesher *m) {
   return check_movement(m);
 }
 
-constexpr int kGraphRounds = 3;
+constexpr int kGraphRounds = 5;
 
 // Captures one whole FP32 iteration: snapshot of the points, a batch of flip
 // rounds (later rounds skip themselves once a round flips nothing),
@@ -444,7 +444,8 @@ int capture_iteration(sm_mesher *m) {
       rc = sm_set_error(SM_ERR_HIP, "capture: snapshot copy failed");
       break;
     }
-    if ((rc = enqueue_flip_rounds(m, kGraphRounds, m->d_ctr + kCtrStatus, m->d_p))) break;
+    if ((rc = enqueue_flip_rounds(m, kGraphRounds, m->d_ctr + kCtrStatus, m->d_p, false)))
+      break;
     if ((rc = enqueue_snapshot(m, m->d_ctr + kCtrStatus))) break;
     if ((rc = centroid_keep(m))) break;
     if ((rc = build_edges(m, false))) break;
@@ -493,12 +494,11 @@ int iterate_graph(sm_mesher *m) {
   m->stats[5] += 1;  // speculative iteration rolled back
   SM_HIP(hipMemcpyAsync(m->d_p, m->d_psnap, m->N * m->dim * sizeof(double),
                         hipMemcpyDeviceToDevice, m->stream));
-  // more flip rounds or a kinetic repair from the last valid state; an
-  // exact predicate that could not be settled goes straight to the host
-  bool rebuild = st[4] > 0;
-  if (!rebuild) {
-    if ((rc = flip_repair2d(m, 64, &rebuild))) return rc;
-  }
+  // more flip rounds (settling uncertain predicates exactly, which the
+  // captured rounds leave to this path) or a kinetic repair from the last
+  // valid state; the host rebuilds when those fail
+  bool rebuild = false;
+  if ((rc = flip_repair2d(m, 64, &rebuild))) return rc;
   if (rebuild && (rc = host_rebuild2d(m))) return rc;
   if ((rc = centroid_keep(m))) return rc;
   if ((rc = edges_forces_update(m))) return rc;

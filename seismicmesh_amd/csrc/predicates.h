@@ -81,20 +81,39 @@ SM_HD inline void grow(const Exp<M> &e, double b, Exp<N> &h) {
   }
 }
 
-// h = e + f  (repeated Grow-Expansion)
+// h = e + f in time linear in the lengths: the components of both are
+// merged by increasing magnitude and accumulated with exact sums, dropping
+// zeros (Shewchuk's Fast-Expansion-Sum with zero elimination, "Adaptive
+// Precision Floating-Point Arithmetic and Fast Robust Geometric
+// Predicates", 1997, Theorem 13; valid in round-to-nearest-even).
 template <int N, int M, int K>
 SM_HD inline void add(const Exp<M> &e, const Exp<K> &f, Exp<N> &h) {
-  Exp<N> tmp;
   h.n = 0;
   h.ovf = e.ovf || f.ovf;
-  for (int i = 0; i < e.n && i < N; ++i) h.c[h.n++] = e.c[i];
-  if (e.n > N) { h.ovf = true; return; }
-  for (int j = 0; j < f.n; ++j) {
-    grow(h, f.c[j], tmp);
-    bool o = h.ovf || tmp.ovf;
-    h = tmp;
-    h.ovf = o;
-    if (o) return;
+  if (h.ovf) return;
+  const int en = e.n, fn = f.n;
+  if (en + fn == 0) return;
+  int i = 0, j = 0;
+  // next component by magnitude (ties take e's)
+  auto take = [&](double &x) {
+    if (j >= fn || (i < en && (fabs(e.c[i]) <= fabs(f.c[j])))) x = e.c[i++];
+    else x = f.c[j++];
+  };
+  double q;
+  take(q);
+  while (i < en || j < fn) {
+    double x, s, t;
+    take(x);
+    two_sum(q, x, s, t);
+    q = s;
+    if (t != 0.0) {
+      if (h.n >= N) { h.ovf = true; return; }
+      h.c[h.n++] = t;
+    }
+  }
+  if (q != 0.0 || h.n == 0) {
+    if (h.n >= N) { h.ovf = true; return; }
+    h.c[h.n++] = q;
   }
 }
 
@@ -136,19 +155,28 @@ SM_HD inline void scale(const Exp<M> &e, double b, Exp<N> &h) {
   }
 }
 
-// h = e * f
+// h = e * f: the sum over f's components of e scaled by each, accumulated
+// with the linear-time sum above in two alternating buffers.
 template <int N, int M, int K>
 SM_HD inline void mul(const Exp<M> &e, const Exp<K> &f, Exp<N> &h) {
   h.n = 0;
   h.ovf = e.ovf || f.ovf;
-  if (h.ovf) return;
-  Exp<N> part, acc;
-  for (int j = 0; j < f.n; ++j) {
+  if (h.ovf || e.n == 0 || f.n == 0) return;
+  if (f.n == 1) {
+    scale(e, f.c[0], h);
+    return;
+  }
+  Exp<N> b0, b1, part;
+  Exp<N> *cur = &b0, *nxt = &b1;
+  scale(e, f.c[0], *cur);
+  for (int j = 1; j < f.n; ++j) {
     scale(e, f.c[j], part);
-    if (part.ovf) { h.ovf = true; return; }
-    add(h, part, acc);
-    if (acc.ovf) { h.ovf = true; return; }
-    h = acc;
+    if (part.ovf || cur->ovf) { h.ovf = true; return; }
+    Exp<N> *out = (j + 1 == f.n) ? &h : nxt;
+    add(*cur, part, *out);
+    if (out->ovf) { h.ovf = true; return; }
+    nxt = cur;
+    cur = out;
   }
 }
 
@@ -471,6 +499,64 @@ SM_HD inline int orient3d_filter(const double *a, const double *b, const double 
   return SM_PRED_UNRESOLVED;
 }
 
+// Second stage without expansions: the plain double evaluation is repeated
+// with error-free transformations, and its sign is exact when every
+// difference, product and partial sum in it was exact (all error terms
+// zero) -- the usual case for degenerate configurations of points on a
+// lattice of modest multiples of a power of two, where the filter cannot
+// decide.  The last operation needs no check: rounding a sum of exact
+// values keeps its sign.  Registers only; SM_PRED_UNRESOLVED otherwise.
+struct ExactTrack {
+  bool ok = true;
+  SM_HD double d(double x, double y) {
+    double r, e;
+    two_diff(x, y, r, e);
+    ok = ok && e == 0.0;
+    return r;
+  }
+  SM_HD double p(double x, double y) {
+    double r, e;
+    two_prod(x, y, r, e);
+    ok = ok && e == 0.0;
+    return r;
+  }
+  SM_HD double s(double x, double y) {
+    double r, e;
+    two_sum(x, y, r, e);
+    ok = ok && e == 0.0;
+    return r;
+  }
+};
+
+SM_HD inline int orient2d_if_exact(const double *a, const double *b, const double *c) {
+  ExactTrack k;
+  double acx = k.d(a[0], c[0]), acy = k.d(a[1], c[1]);
+  double bcx = k.d(b[0], c[0]), bcy = k.d(b[1], c[1]);
+  double l = k.p(acx, bcy), r = k.p(acy, bcx);
+  if (!k.ok) return SM_PRED_UNRESOLVED;
+  double det = l - r;
+  return (det > 0.0) - (det < 0.0);
+}
+
+SM_HD inline int incircle_if_exact(const double *a, const double *b, const double *c,
+                                   const double *d) {
+  ExactTrack k;
+  double adx = k.d(a[0], d[0]), ady = k.d(a[1], d[1]);
+  double bdx = k.d(b[0], d[0]), bdy = k.d(b[1], d[1]);
+  double cdx = k.d(c[0], d[0]), cdy = k.d(c[1], d[1]);
+  double bc = k.d(k.p(bdx, cdy), k.p(cdx, bdy));
+  double ca = k.d(k.p(cdx, ady), k.p(adx, cdy));
+  double ab = k.d(k.p(adx, bdy), k.p(bdx, ady));
+  double al = k.s(k.p(adx, adx), k.p(ady, ady));
+  double bl = k.s(k.p(bdx, bdx), k.p(bdy, bdy));
+  double cl = k.s(k.p(cdx, cdx), k.p(cdy, cdy));
+  double t1 = k.p(al, bc), t2 = k.p(bl, ca), t3 = k.p(cl, ab);
+  double s12 = k.s(t1, t2);
+  if (!k.ok) return SM_PRED_UNRESOLVED;
+  double det = s12 + t3;
+  return (det > 0.0) - (det < 0.0);
+}
+
 // Host entry points: same filters, exact stage with room for any input a
 // double-precision coordinate set can produce in practice.
 inline int orient2d_h(const double *a, const double *b, const double *c) {
@@ -478,6 +564,8 @@ inline int orient2d_h(const double *a, const double *b, const double *c) {
 }
 inline int incircle_h(const double *a, const double *b, const double *c, const double *d) {
   int s = incircle_filter(a, b, c, d);
+  if (s != SM_PRED_UNRESOLVED) return s;
+  s = incircle_if_exact(a, b, c, d);
   if (s != SM_PRED_UNRESOLVED) return s;
   s = incircle_exact<96>(a, b, c, d);
   if (s != SM_PRED_UNRESOLVED) return s;
