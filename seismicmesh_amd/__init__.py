@@ -10,7 +10,8 @@ from . import geometry, mesh_utils, sizing
 from .generation import generate_mesh, sliver_removal
 from .geometry import (Ball, Cube, Cylinder, Difference, Disk, Intersection, Prism,
                        Rectangle, Repeat, Torus, Union)
-from .sizing import SizeFunction, get_sizing_function_from_segy
+from .sizing import (SizeFunction, get_sizing_function_from_segy, read_velocity_model,
+                     write_velocity_model)
 
 __version__ = "0.1.0"
 
@@ -18,4 +19,5 @@ __all__ = [
     "__version__", "geometry", "mesh_utils", "sizing", "generate_mesh", "sliver_removal",
     "Ball", "Cube", "Cylinder", "Difference", "Disk", "Intersection", "Prism", "Rectangle",
     "Repeat", "Torus", "Union", "SizeFunction", "get_sizing_function_from_segy",
+    "read_velocity_model", "write_velocity_model",
 ]

@@ -282,7 +282,10 @@ def _termination(p, t, opts, verbose=1):
         p, t = mu.delete_boundary_entities(p, t, dim=2, min_qual=0.15, verbose=verbose)
         if opts["subdomains"] is None and opts["mesh_improvement"]:
             p, t = mu.laplacian2_fixed_point(p, t)
-    p, t, _ = mu.fix_mesh(p, t, dim=dim, delete_unused=True)
+    if opts.get("perform_checks"):
+        p, t = mu.linter(p, t, dim=dim)
+    else:
+        p, t, _ = mu.fix_mesh(p, t, dim=dim, delete_unused=True)
     return p, t
 
 

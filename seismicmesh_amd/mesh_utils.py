@@ -13,7 +13,7 @@ __all__ = [
     "simp_vol", "simp_qual", "fix_mesh", "unique_rows", "get_edges", "get_facets",
     "get_boundary_edges", "get_boundary_facets", "get_boundary_vertices",
     "vertex_to_entities", "get_boundary_entities", "delete_boundary_entities",
-    "laplacian2_fixed_point", "get_centroids",
+    "laplacian2_fixed_point", "get_centroids", "linter", "do_any_overlap", "is_manifold",
 ]
 
 
@@ -186,3 +186,108 @@ def laplacian2_fixed_point(p, t):
     rhs[bnd] = p[bnd]
     lu = splu(m.tocsc())
     return np.column_stack([lu.solve(rhs[:, 0]), lu.solve(rhs[:, 1])]), t
+
+
+def is_manifold(p, t, dim=2):
+    """geometry/utils.py:634-654: a boundary edge belongs to one entity and
+    boundary vertices and edges pair up."""
+    bedges = get_boundary_edges(t, dim=dim)
+    if bedges.size != p[np.unique(bedges), :].size:
+        print("Mesh has a non-manifold boundary...", flush=True)
+        return False
+    return True
+
+
+def _det4(m):
+    """Determinants of a stack of 4x4 matrices by cofactor expansion along
+    the last column (of ones for the point-in-cell test)."""
+    def d3(a):
+        return (a[:, 0, 0] * (a[:, 1, 1] * a[:, 2, 2] - a[:, 1, 2] * a[:, 2, 1])
+                - a[:, 0, 1] * (a[:, 1, 0] * a[:, 2, 2] - a[:, 1, 2] * a[:, 2, 0])
+                + a[:, 0, 2] * (a[:, 1, 0] * a[:, 2, 1] - a[:, 1, 1] * a[:, 2, 0]))
+    out = 0.0
+    for r in range(4):
+        rows = [k for k in range(4) if k != r]
+        minor = m[:, rows][:, :, :3]
+        out = out + ((-1) ** (r + 3)) * m[:, r, 3] * d3(minor)
+    return out
+
+
+def _inside(cent, q, dim):
+    """Point-in-entity tests of geometry/utils.py:657-726 for stacks of
+    (point, entity) pairs: barycentric coordinates in [0, 1] (2D), equal
+    orientation signs of the five 4x4 determinants (3D)."""
+    if dim == 2:
+        x, y = cent[:, 0], cent[:, 1]
+        x1, y1, x2, y2, x3, y3 = (q[:, 0, 0], q[:, 0, 1], q[:, 1, 0], q[:, 1, 1],
+                                  q[:, 2, 0], q[:, 2, 1])
+        den = (y2 - y3) * (x1 - x3) + (x3 - x2) * (y1 - y3)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            a = ((y2 - y3) * (x - x3) + (x3 - x2) * (y - y3)) / den
+            b = ((y3 - y1) * (x - x3) + (x1 - x3) * (y - y3)) / den
+        c = 1 - a - b
+        return (0 <= a) & (a <= 1) & (0 <= b) & (b <= 1) & (0 <= c) & (c <= 1)
+    ones = np.ones((len(q), 4, 1))
+    base = np.concatenate((q, ones), axis=2)
+    signs = [np.sign(_det4(base))]
+    for k in range(4):
+        m = base.copy()
+        m[:, k, :3] = cent
+        signs.append(np.sign(_det4(m)))
+    s = np.stack(signs, 1)
+    return (s == -1).all(1) | (s == 1).all(1)
+
+
+def do_any_overlap(p, t, dim=2):
+    """geometry/utils.py:745-817: pairs (ie, ele) where entity ie's centroid
+    lies in an entity ele of its vertex 1-ring; progress and alerts printed
+    as the reference does."""
+    vtoe, ptr = vertex_to_entities(p, t, dim=dim)
+    cents = get_centroids(p, t)
+    C = len(t)
+    k = dim + 1
+    vflat = t.reshape(-1)
+    cnt = ptr[vflat + 1] - ptr[vflat]
+    start = ptr[vflat]
+    ie = np.repeat(np.repeat(np.arange(C), k), cnt)
+    off = np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt, cnt)
+    ele = vtoe[np.repeat(start, cnt) + off]
+    keep = ele != ie
+    ie, ele = ie[keep], ele[keep]
+    hit = _inside(cents[ie], p[t[ele]], dim) if len(ie) else np.zeros(0, bool)
+    inter = []
+    hi, he = ie[hit], ele[hit]
+    j = 0
+    for blk in range(0, C, 1000):
+        print("INFO: " + str(100.0 * (blk / C)) + " % done...", flush=True)
+        while j < len(hi) and hi[j] < blk + 1000:
+            print("Alert: entity " + str(hi[j]) + " intersects with entity " + str(he[j])
+                  + ". These will be adjusted.", flush=True)
+            inter.append((int(hi[j]), int(he[j])))
+            j += 1
+    return inter
+
+
+def linter(p, t, dim=2, min_qual=0.10):
+    """geometry/utils.py:820-872: delete the lower-quality entity of every
+    overlapping pair, clean up, drop poor boundary entities (2D) and report
+    the final size and minimum quality."""
+    print("Performing mesh linting...", flush=True)
+    qual = simp_qual(p, t)
+    inter = do_any_overlap(p, t, dim=dim)
+    delete = np.array([], dtype=int)
+    for pair in inter:
+        ix = list(pair)
+        delete = np.append(delete, pair[int(np.argmin(qual[ix]))])
+    delete = np.unique(delete)
+    print("Deleting " + str(len(delete)) + " overlapped entities", flush=True)
+    t = np.delete(t, delete, axis=0)
+    p, t, _ = fix_mesh(p, t, delete_unused=True)
+    if dim == 2:
+        p, t = delete_boundary_entities(p, t, min_qual=min_qual)
+        _ = is_manifold(p, t)
+    qual = simp_qual(p, t)
+    print("There are " + str(len(p)) + " vertices and " + str(len(t))
+          + " elements in the mesh", flush=True)
+    print("The minimum element quality is " + str(np.amin(qual)), flush=True)
+    return p, t

@@ -1,8 +1,10 @@
 """Sizing functions with the reference's API; grids are interpolated on the GPU.
 
-Mirrors SeismicMesh/sizing/size_function.py:1-12 (SizeFunction) and the
-in-memory part of SeismicMesh/sizing/mesh_size_function.py:34-239
-(get_sizing_function_from_segy with ``velocity_data``).  The graded size
+Mirrors SeismicMesh/sizing/size_function.py:1-12 (SizeFunction) and
+SeismicMesh/sizing/mesh_size_function.py:34-239
+(get_sizing_function_from_segy: velocity from memory, SEG-Y or raw binary
+files, wavelength / gradient / Courant sizing, gradation, domain pads) and
+:576-646 (read_velocity_model).  The graded size
 grid is held as FP32 values on float32-linspace axes (z, x[, y]) exactly as
 the reference's RegularGridInterpolator is built (:391-408, :514-523);
 queries are linear with linear extrapolation, evaluated on the device
@@ -10,12 +12,14 @@ queries are linear with linear extrapolation, evaluated on the device
 as a separable distance transform (csrc/api.hip, sm_limgrad).
 """
 import ctypes
+import warnings
 
 import numpy as np
 
 from . import _lib
 
-__all__ = ["SizeFunction", "GridSizing", "get_sizing_function_from_segy", "limgrad"]
+__all__ = ["SizeFunction", "GridSizing", "get_sizing_function_from_segy", "limgrad",
+           "read_velocity_model", "write_velocity_model"]
 
 
 class SizeFunction:
@@ -112,32 +116,33 @@ _SZ_KEYS = {
 
 def get_sizing_function_from_segy(filename, bbox, comm=None, **kwargs):
     """Velocity model -> graded size grid -> SizeFunction
-    (mesh_size_function.py:34-239), for in-memory ``velocity_data``.
-
-    Reading SEG-Y / binary files is not part of this package: pass the
-    velocity array as ``velocity_data``.
-    """
+    (mesh_size_function.py:34-239).  The velocity comes from
+    ``velocity_data`` (an array, z-major) or from ``filename`` (SEG-Y, or a
+    raw binary with nz/nx/ny, byte_order, axes_order, axes_order_sort,
+    dtype)."""
     opts = {
         "velocity_data": None, "vp_water": 1500.0, "hmin": 150.0, "hmax": 10000.0,
         "wl": 0, "freq": 2.0, "grad": 0.0, "grade": 0.0, "stencil_size": 10.0,
         "space_order": 1, "dt": 0.0, "cr_max": 1.0, "pad_style": "edge", "domain_pad": 0.0,
-        "units": "m-s", "nz": None, "nx": None, "ny": None,
+        "units": "m-s", "nz": None, "nx": None, "ny": None, "byte_order": "byte_order",
+        "axes_order": (0, 1, 2), "axes_order_sort": "F", "dtype": "float32",
     }
-    for key in kwargs:
-        if key not in _SZ_KEYS:
-            raise ValueError("Option %s with parameter %s not recognized " % (key, kwargs[key]))
     opts.update(kwargs)
     vp = opts["velocity_data"]
-    if vp is None:
-        raise NotImplementedError(
-            "reading velocity files is not supported; pass velocity_data=<array>")
-    vp = np.array(vp)
+    if vp is not None:
+        vp = np.array(vp)
+        nz, nx = vp.shape[0], vp.shape[1]
+    else:
+        vp, nz, nx, ny = read_velocity_model(
+            filename, nz=opts["nz"], nx=opts["nx"], ny=opts["ny"],
+            byte_order=opts["byte_order"], axes_order=opts["axes_order"],
+            axes_order_sort=opts["axes_order_sort"], dtype=opts["dtype"])
     if opts["units"] == "km-s":
         print("Converting from km-s to m-s...", flush=True)
-        vp = vp * 1000.0
+        vp *= 1000.0
     elif opts["units"] == "ft-s":
         print("Converting from ft-s to m-s...", flush=True)
-        vp = vp * 0.30
+        vp *= 0.30
     pos = np.where(vp < 1e-3)
     if len(pos) > 0 and any(pos[0]):
         if opts["vp_water"] is None or opts["vp_water"] < 1300 or opts["vp_water"] > 1800:
@@ -150,31 +155,205 @@ def get_sizing_function_from_segy(filename, bbox, comm=None, **kwargs):
     else:
         raise ValueError("Dimension not supported")
     cell = np.full(vp.shape, opts["hmin"], dtype=float)
+    del dim
+    for key in kwargs:
+        if key not in _SZ_KEYS:
+            raise ValueError("Option %s with parameter %s not recognized " % (key, kwargs[key]))
     if opts["wl"] > 0 or opts["grad"] > 0:
-        wl_size = 99999 if opts["wl"] == 0.0 else _wavelength_sizing(vp, opts["wl"], opts["freq"])
-        if opts["grad"] > 0:
-            raise NotImplementedError("gradient-based sizing (grad > 0) is not supported")
-        cell = np.minimum(wl_size, 99999)
+        cell = np.minimum(_wavelength_sizing(vp, opts["wl"], opts["freq"]),
+                          _gradient_sizing(vp, opts["grad"], opts["stencil_size"]))
     cell = np.array(cell)
     print(f"Enforcing minimum edge length of {opts['hmin']}")
     cell[cell < opts["hmin"]] = opts["hmin"]
     print(f"Enforcing maximum edge length of {opts['hmax']}")
     cell[cell > opts["hmax"]] = opts["hmax"]
     cell = _enforce_courant(vp, cell, opts["cr_max"], opts["dt"], opts["space_order"])
-    if opts["grade"] < 0:
-        raise ValueError("Parameter `grade` must be > 0.0")
-    if opts["grade"] > 0:
-        print(f"Enforcing mesh size gradation of {opts['grade']} decimal percent...")
-        cell = limgrad(cell, (bbox[1] - bbox[0]) / vp.shape[0], opts["grade"])
-    if opts["domain_pad"] > 0:
-        raise NotImplementedError("domain padding is not supported")
+    cell = _enforce_gradation(cell, opts["grade"], (bbox[1] - bbox[0]) / nz)
+    cell, vp, bbox = _build_domain_pad(cell, vp, bbox, opts)
     fh = GridSizing(cell, bbox)
-    del dim
     return SizeFunction(tuple(bbox), fh, opts["hmin"])
+
+
+def _enforce_gradation(cell, grade, elen):
+    # mesh_size_function.py:469-494
+    if grade == 0.0:
+        warnings.warn("Mesh size gradient is deactiavted. This may compromise mesh quality")
+        return cell
+    if grade < 0:
+        raise ValueError("Parameter `grade` must be > 0.0")
+    if grade > 1.0:
+        warnings.warn("Parameter `grade` is set pretty high (> 1.0)!")
+    print(f"Enforcing mesh size gradation of {grade} decimal percent...")
+    return limgrad(cell, elen, grade)
+
+
+def _gradient_sizing(vp, grad, stencil_size):
+    # mesh_size_function.py:428-448: windowed variance of the velocity,
+    # normalised to [0, 1], mapped to grad / (var + 0.1)
+    if grad == 0.0:
+        return 99999
+    print("Refining mesh sizes near sharp velocity gradients...")
+    if grad < 0:
+        raise ValueError("Parameter grad must be > 0")
+    from scipy import ndimage
+
+    window = [stencil_size] * vp.ndim if np.isscalar(stencil_size) else stencil_size
+    win_mean = ndimage.uniform_filter(vp, tuple(window))
+    win_sqr_mean = ndimage.uniform_filter(vp ** 2, tuple(window))
+    win_var = win_sqr_mean - win_mean ** 2
+    win_var = np.divide(win_var, np.amax(win_var))
+    win_var -= np.amin(win_var)
+    return grad / (win_var + 0.10)
+
+
+def _dimensions(vp, bbox):
+    # mesh_size_function.py:497-511: grid spacing as extent / count
+    return [(bbox[2 * k + 1] - bbox[2 * k]) / n for k, n in enumerate(vp.shape)]
+
+
+def _build_domain_pad(cell, vp, bbox, opts):
+    # mesh_size_function.py:526-573: extend the grids by `domain_pad` metres
+    # at the bottom (z min) and on both sides of the lateral axes
+    pad = opts["domain_pad"]
+    style = opts["pad_style"]
+    if pad < 0:
+        raise ValueError("Domain extension must be >= 0")
+    if pad > 0:
+        print(f"Including a {pad} meter domain extension...")
+        d = _dimensions(vp, bbox)
+        n = [int(pad / dk) for dk in d]
+        bb = [bbox[0] - pad, bbox[1]]
+        for k in range(1, vp.ndim):
+            bb += [bbox[2 * k] - pad, bbox[2 * k + 1] + pad]
+        bbox = tuple(bb)
+        print(f"Using the pad_style: {style}")
+        padding = ((n[0], 0),) + tuple((n[k], n[k]) for k in range(1, vp.ndim))
+        cell = _pad_it(cell, padding, style, [np.amax(cell)] * 2)
+        vp = _pad_it(vp, padding, style, [np.amax(vp)] * 2)
+    return cell, vp, bbox
+
+
+def _pad_it(array, padding, style, extra):
+    # mesh_size_function.py:576-587
+    if style == "edge":
+        return np.pad(array, padding, "edge")
+    if style == "constant":
+        return np.pad(array, padding, "constant", constant_values=tuple(extra))
+    if style == "linear_ramp":
+        return np.pad(array, padding, "linear_ramp", end_values=tuple(extra))
+    raise ValueError(
+        "pad style currently not supported. Try `linear_ramp`, `edge`, or `constant`")
+
+
+def read_velocity_model(filename, nz=None, nx=None, ny=None, byte_order=None, axes_order=None,
+                        axes_order_sort=None, dtype=None):
+    """mesh_size_function.py:590-603: (vp, nz, nx, ny) from a SEG-Y file or
+    a raw binary grid; vp is z-major with z increasing upwards (flipped)."""
+    if filename.endswith(".segy"):
+        return _read_segy(filename)
+    return _read_bin(filename, nz, nx, ny, byte_order, axes_order, axes_order_sort, dtype)
+
+
+def _read_bin(filename, nz, nx, ny, byte_order, axes_order, axes_order_sort, dtype):
+    # mesh_size_function.py:606-627
+    if nz is None or nx is None or ny is None:
+        raise ValueError(
+            "Please specify the number of grid points in each dimension (e.g., `nz`, `nx`, "
+            "`ny`)...")
+    axes = [nz, nx, ny]
+    ix = np.argsort(axes_order)
+    axes = [axes[o] for o in ix]
+    print(f"Reading binary file: {filename}")
+    if byte_order == "big":
+        vp = np.fromfile(filename, dtype=np.dtype(dtype).newbyteorder(">"))
+    elif byte_order == "little":
+        vp = np.fromfile(filename, dtype=np.dtype(dtype).newbyteorder("<"))
+    else:
+        raise ValueError("Please specify byte_order as either: little or big.")
+    vp = vp.reshape(*axes, order=axes_order_sort)
+    vp = np.flipud(vp.transpose((*axes_order,)))
+    return vp, nz, nx, ny
+
+
+def _ibm_to_float32(words):
+    """IBM System/360 single precision (sign, base-16 exponent excess 64,
+    24-bit fraction) to IEEE float32; every IBM value in float32 range is
+    exact there (its fraction has at most 24 significant bits)."""
+    w = words.astype(np.uint64)
+    sign = np.where(w >> np.uint64(31), -1.0, 1.0)
+    expo = ((w >> np.uint64(24)) & np.uint64(0x7F)).astype(np.int64) - 64
+    frac = (w & np.uint64(0xFFFFFF)).astype(np.float64) / float(1 << 24)
+    return (sign * np.ldexp(frac, 4 * expo)).astype(np.float32)
+
+
+def _read_segy(filename):
+    """mesh_size_function.py:630-643 (which reads with segyio,
+    ignore_geometry=True): traces are the columns of a (samples, traces)
+    grid, flipped so depth increases downwards in the array's first axis
+    reversed.  SEG-Y rev 1 layout: 3200-byte text header, 400-byte binary
+    header (samples per trace at bytes 3221-3222, format code at 3225-3226,
+    extended text headers at 3505-3506, all big-endian), then traces of a
+    240-byte header and the samples (format 1: IBM float, 5: IEEE float)."""
+    import struct
+
+    raw = np.fromfile(filename, dtype=np.uint8)
+    if raw.size < 3600:
+        raise ValueError("%s is not a SEG-Y file" % filename)
+    bh = raw[3200:3600].tobytes()
+    ns = struct.unpack(">H", bh[20:22])[0]
+    fmt = struct.unpack(">h", bh[24:26])[0]
+    ext = max(struct.unpack(">h", bh[304:306])[0], 0)
+    if fmt not in (1, 5) or ns == 0:
+        raise ValueError("unsupported SEG-Y sample format %d (need 1 or 5)" % fmt)
+    start = 3600 + 3200 * ext
+    trace_bytes = 240 + 4 * ns
+    ntr = (raw.size - start) // trace_bytes
+    body = raw[start:start + ntr * trace_bytes].reshape(ntr, trace_bytes)[:, 240:]
+    words = body.copy().view(">u4").reshape(ntr, ns)
+    if fmt == 1:
+        samples = _ibm_to_float32(words)
+    else:
+        samples = words.view(">f4").astype(np.float32)
+    nz, nx = ns, ntr
+    vp = np.zeros(shape=(nz, nx))
+    vp[:, :] = samples.T
+    if np.amin(vp) < 1000.0:
+        warnings.warn("Velocity appear to be in km/s. Maybe pass `units` km-s key pair?")
+    return np.flipud(vp), nz, nx, 0
+
+
+def write_velocity_model(filename, ofname=None, comm=None, **kwargs):
+    """mesh_size_function.py:242-327: writes the velocity as HDF5, which
+    needs h5py (as in the reference, imported at call time)."""
+    import h5py  # noqa: F401  (raises ImportError where h5py is absent)
+
+    opts = {"nz": None, "nx": None, "ny": None, "byte_order": "byte_order",
+            "axes_order": (0, 1, 2), "axes_order_sort": "F", "dtype": "float32",
+            "domain_pad": 0.0, "pad_style": "edge"}
+    opts.update(kwargs)
+    if ofname is None:
+        warnings.warn("No output filename specified, name will be `filename`")
+        ofname = filename
+    vp, _, _, _ = read_velocity_model(filename, nz=opts["nz"], nx=opts["nx"], ny=opts["ny"],
+                                      byte_order=opts["byte_order"],
+                                      axes_order=opts["axes_order"],
+                                      axes_order_sort=opts["axes_order_sort"],
+                                      dtype=opts["dtype"])
+    if opts["domain_pad"] > 0.0:
+        print("Adding a domain pad to the velocity model...")
+        _, vp, _ = _build_domain_pad(vp, vp, opts["bbox"], opts)
+    ofname += ".hdf5"
+    print(f"Writing velocity model: {ofname}", flush=True)
+    with h5py.File(ofname, "w") as f:
+        f.create_dataset("velocity_model", data=vp, dtype="f")
+        f.attrs["shape"] = vp.shape
+        f.attrs["units"] = "m/s"
 
 
 def _wavelength_sizing(vp, wl, freq):
     # mesh_size_function.py:411-425
+    if wl == 0.0:
+        return 99999
     if wl < 0:
         raise ValueError("Parameter `wl` must be set > 0")
     if freq < 0.0:

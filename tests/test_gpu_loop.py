@@ -219,3 +219,35 @@ def test_errors_match_reference():
         sm.generate_mesh(sm.Rectangle((0, 1, 0, 1)), 0.1, verbose=0)  # int bbox
     with pytest.raises(AssertionError):
         sm.generate_mesh(sq, 0.1, delta_t=1e6, verbose=0)  # divergence assert
+
+
+_GOLDEN = __import__("os").path.join(__import__("os").path.dirname(__file__), "golden")
+
+
+def test_segy_mesh_known_answer():
+    # reference tests/test_2dmesher.py:15-47 ("should have: 7690 vertices
+    # and 15045 cells"; the reference compares with atol=100)
+    import os
+    import seismicmesh_amd as sm
+    bbox = (-10000.0, 0.0, 0.0, 10000.0)
+    ef = sm.get_sizing_function_from_segy(os.path.join(_GOLDEN, "testing.segy"), bbox=bbox,
+                                          grade=0.005, grad=50.0, wl=5, freq=5.0, hmin=100,
+                                          hmax=10e6)
+    (p, t), _ = _counts(sm.generate_mesh, sm.Rectangle(bbox), ef, h0=100, perform_checks=True)
+    assert abs(len(p) - 7690) <= 100 and abs(len(t) - 15045) <= 200
+
+
+@pytest.mark.parametrize("style,answer", [("linear_ramp", (9428, 18525)),
+                                          ("edge", (9724, 19078)),
+                                          ("constant", (9428, 18525))])
+def test_domain_extension_known_answer(style, answer):
+    # reference tests/test_2dmesher_domain_extension.py:15-60 (atol=100)
+    import os
+    import seismicmesh_amd as sm
+    bbox = (-10e3, 0.0, 0.0, 10e3)
+    ef = sm.get_sizing_function_from_segy(os.path.join(_GOLDEN, "testing.segy"), bbox=bbox,
+                                          grade=0.005, grad=50.0, stencil_size=100, wl=5,
+                                          freq=5.0, hmin=100, hmax=10e6, pad_style=style,
+                                          domain_pad=1e3)
+    (p, t), _ = _counts(sm.generate_mesh, sm.Rectangle(bbox), ef, h0=100, perform_checks=True)
+    assert abs(len(p) - answer[0]) <= 100 and abs(len(t) - answer[1]) <= 100
