@@ -13,9 +13,10 @@ from the initial points (the reference's loop performs max_iter - 1 = 49 of
 them); initialisation (lattice, first triangulation) and termination are
 outside it, as in the metric's definition.
 
-Output: one JSON line with the metric, the roofline of the dominant kernel
-phase (HIP events on the loop's own stream) and a CPU baseline (the numpy
-oracle on a bounded sample of the same workload, 1 core).
+Output: one JSON line with the metric, the roofline of the dominant single
+kernel (HIP events around its launches on the loop's own stream) and a CPU
+baseline (the numpy oracle on a bounded sample of the same workload, one
+pinned core, median of three runs).
 """
 import argparse
 import json
@@ -31,10 +32,40 @@ sys.path.insert(0, HERE)
 HBM_PEAK_GBS = 8000.0
 
 
-def alg_bytes(phase, N, E, C, T, rounds=1):
-    """Algorithmic HBM bytes of one launch of a phase (2D, FP64 points,
-    FP32 per-edge arrays, int32 ids).  Each datum is counted once:
-    gathers of point coordinates count 16 B per referenced vertex."""
+# kernels timed alone by the handle's event profiler, by phase name
+SINGLE_KERNEL = {"centroid_filter": "k_keep", "edge_geom": "k_edge_geom",
+                 "edge_force": "k_edge_force", "vertex_update": "k_vertex_update",
+                 "flip_mark": "k_mark"}
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def load_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if any."""
+    fn = os.path.join(HERE, "profiles", "traffic_%s_%s.json" % (kernel, workload))
+    try:
+        with open(fn) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def alg_bytes(phase, N, E, C, T):
+    """Algorithmic HBM bytes of one launch (2D, FP64 points, FP32 per-edge
+    arrays, int32 ids).  Each datum is counted once: gathers of point
+    coordinates count 16 B per referenced vertex."""
     if phase == "edge_geom":
         # edge ids 8 + both endpoints 32 + 4 sizing taps 16 + L, h out 8
         return E * (8 + 32 + 16 + 8)
@@ -49,18 +80,19 @@ def alg_bytes(phase, N, E, C, T, rounds=1):
         # cells read twice 24 + 3 pairs written/read 24 per cell, edges out 8
         return C * (24 + 24) + E * 8 + N * 16
     if phase == "centroid_filter":
-        # triangle ids 12 + 3 vertices 48 + keep flag 1
-        return T * (12 + 48 + 1)
-    if phase == "flip_round":
-        # orientation, mark, apply and fix passes over tri/adj: ~4 x 24 B
-        # per triangle + 3 vertices 48 B
-        return T * (4 * 24 + 48)
+        # triangle ids 12 + keep flag 1 per triangle, points 16 per vertex
+        return T * (12 + 1) + N * 16
+    if phase == "flip_mark":
+        # per triangle: ids 12, neighbours 12, orientation 4, three 64-bit
+        # candidates 24, its 64-bit claim 8; points 16 per vertex
+        return T * (12 + 12 + 4 + 24 + 8) + N * 16
     return 0
 
 
-def cpu_baseline(st, p0, iters):
+def cpu_baseline(st, p0, iters, reps=3):
     """The oracle (numpy FP64 restatement + scipy Qhull) on `iters` force
-    iterations of the same workload, single thread."""
+    iterations of the same workload: one thread pinned to one core, BLAS
+    pools limited to one thread, median of `reps` runs."""
     try:
         from threadpoolctl import threadpool_limits
     except ImportError:  # pragma: no cover
@@ -83,18 +115,27 @@ def cpu_baseline(st, p0, iters):
         ofh = lambda x: np.array([val] * len(x))  # noqa: E731
     s = dict(fd=odom.eval, fh=ofh, h0=st["h0"], dim=2, L0mult=1.2, geps=st["geps"],
              deps=st["deps"], pfix=p0[:st["nfix"]], levels=[odom.eval], delta_t=st["delta_t"])
-    p = p0.copy()
+    old_aff = os.sched_getaffinity(0)
+    core = min(old_aff)
+    os.sched_setaffinity(0, {core})
     ctx = threadpool_limits(limits=1) if threadpool_limits else None
     if ctx:
         ctx.__enter__()
-    t0 = time.perf_counter()
-    for _ in range(iters):
-        t_all = odm.delaunay(p)
-        p, _, _ = odm.force_step(p, t_all, s, list(range(st["nfix"])))
-    dt = time.perf_counter() - t0
-    if ctx:
-        ctx.__exit__(None, None, None)
-    return len(p0) * iters / dt, dt
+    times = []
+    try:
+        for _ in range(reps):
+            p = p0.copy()
+            t0 = time.perf_counter()
+            for _ in range(iters):
+                t_all = odm.delaunay(p)
+                p, _, _ = odm.force_step(p, t_all, s, list(range(st["nfix"])))
+            times.append(time.perf_counter() - t0)
+    finally:
+        if ctx:
+            ctx.__exit__(None, None, None)
+        os.sched_setaffinity(0, old_aff)
+    dt = float(np.median(times))
+    return len(p0) * iters / dt, dt, core
 
 
 def main():
@@ -107,6 +148,9 @@ def main():
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
     ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device-to-device once (a kernel of known "
+                         "traffic for scaling hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -115,6 +159,15 @@ def main():
 
     import torch  # noqa: F401  (device runtime)
     from seismicmesh_amd import generation, workloads
+
+    if args.calibrate_copy > 0:
+        n = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     if args.workload == "bp2004":
         domain, ef, info = workloads.bp2004(scale=args.scale)
@@ -157,23 +210,31 @@ def main():
     prof = m.profile_read()
     m.profile(False)
     T = 2 * N
-    best = max(prof, key=lambda k: prof[k][0])
+    best = max(SINGLE_KERNEL, key=lambda k: prof[k][0])
     ms_tot, launches = prof[best]
     avg_ms = ms_tot / max(launches, 1)
     traffic_alg = alg_bytes(best, N, E, C, T)
     achieved = traffic_alg / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
-    roofline = {"bound": "hbm", "kernel": best, "achieved": round(achieved, 2),
+    kernel = SINGLE_KERNEL[best]
+    others = sum(v[0] for k, v in prof.items() if k != "flip_mark")
+    roofline = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 2),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": None, "avg_launch_us": round(avg_ms * 1e3, 3),
+                "traffic": load_traffic(kernel, wl["workload"]),
+                "avg_launch_us": round(avg_ms * 1e3, 3), "launches": launches,
                 "alg_bytes_per_launch": traffic_alg,
-                "phases_ms": {k: round(v[0], 3) for k, v in prof.items()}}
+                "phases_ms": {k: round(v[0], 3) for k, v in prof.items()},
+                "flips_per_s": round(stats["flips"] / elapsed, 1),
+                "flip_share": round(prof["flip_round"][0] / max(others, 1e-9), 4)}
 
     cpu = None
     if not args.no_cpu:
-        rate, secs = cpu_baseline(st, p0, args.cpu_iters)
+        rate, secs, core = cpu_baseline(st, p0, args.cpu_iters)
+        model, ncpu = cpu_info()
         cpu = {"value": round(rate, 1), "unit": "vertex-updates/s", "cores": 1, "kind": "port",
                "sample": "%d force iterations of the same workload (N=%d) with the numpy "
-                         "oracle + scipy Qhull, %.1f s" % (args.cpu_iters, N, secs)}
+                         "oracle + scipy Qhull (stand-in for CGAL), one thread pinned to core "
+                         "%d of %d (%s), median of 3 runs of %.1f s"
+                         % (args.cpu_iters, N, core, ncpu, model, secs)}
 
     out = {
         "metric": "vertex-updates/s on 1 MI355X, fused+neural vs plain GPU at equal min quality",

@@ -201,7 +201,7 @@ int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats);
 /* Per-phase kernel timing with HIP events recorded on the handle's stream.
  * Phases: 0 flip rounds, 1 centroid filter, 2 unique edges + edge lists,
  * 3 bar lengths and sizing, 4 L0 sums, 5 bar forces, 6 gather + update +
- * projection.  sm_mesher_profile(m, 1) clears and enables;
+ * projection, 7 the flip-marking kernel alone (inside phase 0).  sm_mesher_profile(m, 1) clears and enables;
  * sm_mesher_profile_read fills ms[i] (total) and launches[i]. */
 #define SM_NPHASES 8
 int sm_mesher_profile(sm_mesher *m, int enable);

@@ -118,3 +118,82 @@ def sizing_from_velocity(vp, bbox, hmin, hmax=10000.0, wl=0, freq=2.0, grade=0.0
         cell = gradation(cell, grade, (bbox[1] - bbox[0]) / vp.shape[0], limiter)
     fh = build_interpolant(cell, bbox)
     return SizeFunction(tuple(float(b) for b in bbox), fh, hmin), cell
+
+
+# ---------------------------------------------------------------------------
+# velocity files, gradient sizing and domain pads
+# (mesh_size_function.py:428-448, :526-587, :630-643)
+# ---------------------------------------------------------------------------
+def read_segy(filename):
+    """SEG-Y (format 1, IBM float) as segyio reads it with
+    ignore_geometry=True (mesh_size_function.py:630-643): traces are the
+    columns, then the grid is flipped upside down.  Word-by-word decoding."""
+    import struct
+
+    with open(filename, "rb") as f:
+        data = f.read()
+    ns = struct.unpack(">H", data[3220:3222])[0]
+    fmt = struct.unpack(">h", data[3224:3226])[0]
+    assert fmt == 1, "the oracle reads IBM-float SEG-Y only"
+    ntr = (len(data) - 3600) // (240 + 4 * ns)
+    vp = np.zeros((ns, ntr))
+    for j in range(ntr):
+        base = 3600 + j * (240 + 4 * ns) + 240
+        for i in range(ns):
+            (w,) = struct.unpack(">I", data[base + 4 * i:base + 4 * i + 4])
+            sign = -1.0 if w >> 31 else 1.0
+            expo = ((w >> 24) & 0x7F) - 64
+            frac = (w & 0xFFFFFF) / 16777216.0
+            vp[i, j] = float(np.float32(sign * frac * 16.0 ** expo))
+    return np.flipud(vp), ns, ntr, 0
+
+
+def gradient_sizing(vp, grad, stencil_size):
+    """mesh_size_function.py:428-448."""
+    from scipy import ndimage
+
+    if grad == 0.0:
+        return 99999
+    window = [stencil_size] * vp.ndim
+    mean = ndimage.uniform_filter(vp, tuple(window))
+    sqr = ndimage.uniform_filter(vp ** 2, tuple(window))
+    var = sqr - mean ** 2
+    var = np.divide(var, np.amax(var))
+    var -= np.amin(var)
+    return grad / (var + 0.10)
+
+
+def domain_pad(cell, bbox, pad, style):
+    """mesh_size_function.py:526-587 (2D): pad nz rows at z min and nx
+    columns on both x sides; max value for the constant / ramp ends."""
+    nz, nx = cell.shape
+    dz = (bbox[1] - bbox[0]) / nz
+    dx = (bbox[3] - bbox[2]) / nx
+    nnz, nnx = int(pad / dz), int(pad / dx)
+    bbox = (bbox[0] - pad, bbox[1], bbox[2] - pad, bbox[3] + pad)
+    widths = ((nnz, 0), (nnx, nnx))
+    top = np.amax(cell)
+    if style == "edge":
+        cell = np.pad(cell, widths, "edge")
+    elif style == "constant":
+        cell = np.pad(cell, widths, "constant", constant_values=(top, top))
+    else:
+        cell = np.pad(cell, widths, "linear_ramp", end_values=(top, top))
+    return cell, bbox
+
+
+def sizing_from_segy(filename, bbox, hmin, hmax, wl, freq, grad, grade, stencil_size=10.0,
+                     domain_pad_m=0.0, pad_style="edge", limiter=limgrad):
+    """get_sizing_function_from_segy for an IBM-float SEG-Y file
+    (mesh_size_function.py:34-239, without Courant limiting)."""
+    vp, nz, nx, _ = read_segy(filename)
+    cell = np.minimum(vp / (freq * wl) if wl > 0 else 99999,
+                      gradient_sizing(vp, grad, stencil_size))
+    cell = np.array(cell, dtype=float)
+    cell[cell < hmin] = hmin
+    cell[cell > hmax] = hmax
+    if grade > 0:
+        cell = gradation(cell, grade, (bbox[1] - bbox[0]) / nz, limiter)
+    if domain_pad_m > 0:
+        cell, bbox = domain_pad(cell, bbox, domain_pad_m, pad_style)
+    return SizeFunction(tuple(float(b) for b in bbox), build_interpolant(cell, bbox), hmin), cell

@@ -472,8 +472,10 @@ int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double
     if (exact)
       hipLaunchKernelGGL(k_orient_exact, dim3(8), dim3(64), 0, s, m->d_tri, pts, m->d_ori,
                          m->d_queue, c);
+    const int pm = prof_begin(m);
     hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, pts,
                        m->d_claim, m->d_cand, m->d_queue + T, c, qx);
+    prof_end(m, 7, pm);
     if (exact)
       hipLaunchKernelGGL(k_mark_exact, dim3(8), dim3(64), 0, s, m->d_tri, m->d_adj, pts,
                          m->d_claim, m->d_cand, m->d_queue + T, c);

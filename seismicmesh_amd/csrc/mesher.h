@@ -98,6 +98,7 @@ struct sm_mesher {
     int a, b;  // indices into ev_pool
   };
   std::vector<Rec> recs;
+  int ev_next = 0;  // next free event pair in ev_pool
   double prof_ms[SM_NPHASES] = {};
   int64_t prof_n[SM_NPHASES] = {};
 };

@@ -202,7 +202,9 @@ namespace smk {
 
 int prof_begin(sm_mesher *m) {
   if (!m->prof) return -1;
-  int idx = (int)m->recs.size() * 2;
+  // event pairs are handed out in order (phases may nest)
+  int idx = m->ev_next;
+  m->ev_next += 2;
   while ((int)m->ev_pool.size() < idx + 2) {
     hipEvent_t e;
     if (hipEventCreate(&e) != hipSuccess) return -1;
@@ -228,6 +230,7 @@ int prof_drain(sm_mesher *m) {
     m->prof_n[r.phase] += 1;
   }
   m->recs.clear();
+  m->ev_next = 0;
   return SM_OK;
 }
 

@@ -220,7 +220,7 @@ class Mesher:
         return out
 
     PHASES = ("flip_round", "centroid_filter", "unique_edges", "edge_geom", "l0_sums",
-              "edge_force", "vertex_update")
+              "edge_force", "vertex_update", "flip_mark")
 
     def set_graph(self, enable=True):
         """Run FP32 iterations as one captured graph launch each (default)."""

@@ -137,6 +137,6 @@ def test_3d_sliver_cylinder():
     dh = og.calc_dihedral_angles(p, t)
     bad = np.argwhere((dh[:, 0] < 10 * math.pi / 180) | (dh[:, 0] > math.pi))
     assert len(bad) == 0
-    # reference tests/test_3dmesher_SDF.py:45: volume of the unit cylinder
+    # Cylinder(h, r) spans z in [-h/2, h/2]: volume pi r^2 h
     from seismicmesh_amd import mesh_utils as mu
-    assert abs(mu.simp_vol(p, t).sum() - 6.28) < h
+    assert abs(mu.simp_vol(p, t).sum() - math.pi) < h
