@@ -170,6 +170,21 @@ int sm_mesher_create(const sm_mesher_params *prm, const sm_node *const *tapes,
                      const int *tape_lens, const sm_grid *sizing,
                      sm_host_sizing_fn host_sizing, void *host_sizing_ctx, int device,
                      sm_mesher **out);
+/* A handle meshing `ndom` independent 2D domains at once, each with a
+ * uniform size: domain d owns the points [voff[d], voff[d+1]) of the point
+ * array (voff[0] = 0), its SDF program tapes[d] (tape_lens[d] nodes), its
+ * size h[d] (its h0) and fixes its first nfix[d] points.  Each domain is
+ * iterated exactly as its own handle would be (its own triangulation, L0
+ * scale, centroid filter and projection), all domains in the same kernel
+ * launches.  prm: dim 2, precision 32, delta_t and L0mult as usual; geps
+ * and deps are multipliers of each domain's h (0.1 and sqrt(eps) in the
+ * reference); h0, nfix and nlevels are ignored.  The max movement that
+ * sm_mesher_run reports is in units of each domain's h0.  No counterpart
+ * in the reference, which meshes one domain per call; replaces looping
+ * generate_mesh (mesh_generator.py:291-529) over many domains. */
+int sm_mesher_create_batch(const sm_mesher_params *prm, int ndom, const sm_node *const *tapes,
+                           const int *tape_lens, const double *h, const int32_t *nfix,
+                           const int64_t *voff, int device, sm_mesher **out);
 int sm_mesher_destroy(sm_mesher *m);
 /* Uploads points (host, (n, dim) FP64); 2D builds the first triangulation
  * (3D re-tetrahedralises at every triangulation step). */

@@ -97,6 +97,8 @@ _PROTOS = {
     "sm_cell_edges": (_int, [_vp, _i64, _int, _i64, _vp, _vp, _vp, _sz, _vp]),
     "sm_mesher_create": (_int, [ctypes.POINTER(MesherParams), _vp, _vp, ctypes.POINTER(Grid),
                                 HOST_SIZING_FN, _vp, _int, ctypes.POINTER(_vp)]),
+    "sm_mesher_create_batch": (_int, [ctypes.POINTER(MesherParams), _int, _vp, _vp, _vp, _vp,
+                                      _vp, _int, ctypes.POINTER(_vp)]),
     "sm_mesher_destroy": (_int, [_vp]),
     "sm_mesher_set_points": (_int, [_vp, _vp, _i64]),
     "sm_mesher_run": (_int, [_vp, _int, ctypes.POINTER(_dbl)]),

@@ -25,6 +25,9 @@
 // with no flip and no inverted triangle, every edge is locally Delaunay and
 // the triangulation is the Delaunay triangulation of the points; otherwise
 // the host rebuilds it from scratch with exact predicates.
+#include <atomic>
+#include <thread>
+
 #include "common.h"
 #include "mesher.h"
 #include "predicates.h"
@@ -547,6 +550,49 @@ int snapshot_valid(sm_mesher *m) {
   return SM_OK;
 }
 
+// Batch handles: each domain's triangulated sphere is built on its own
+// vertex range (worker threads over the domains) and the pieces are
+// concatenated with their vertex and triangle ids offset, so no triangle
+// or neighbour link ever joins two domains.
+static int64_t batch_delaunay(sm_mesher *m, const std::vector<double> &hp,
+                              std::vector<int32_t> &tri, std::vector<int32_t> &adj) {
+  const int nd = m->ndom;
+  std::vector<std::vector<int32_t>> dt(nd), da(nd);
+  std::vector<int64_t> cnt(nd, 0);
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    std::vector<uint8_t> ins;
+    for (int d; (d = next.fetch_add(1)) < nd;) {
+      const int64_t b = m->h_voff[d], n = m->h_voff[d + 1] - b;
+      const int64_t cap = 2 * n + 16;
+      dt[d].resize(cap * 3);
+      da[d].resize(cap * 3);
+      ins.resize(n);
+      cnt[d] = sm_host_delaunay2d(hp.data() + 2 * b, n, dt[d].data(), da[d].data(), cap,
+                                  ins.data());
+    }
+  };
+  unsigned nw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  nw = std::min<unsigned>(nw, (unsigned)nd);
+  std::vector<std::thread> pool;
+  for (unsigned i = 1; i < nw; ++i) pool.emplace_back(work);
+  work();
+  for (auto &t : pool) t.join();
+  int64_t T = 0;
+  for (int d = 0; d < nd; ++d) {
+    if (cnt[d] < 0) return cnt[d];
+    if (T + cnt[d] > m->capT) return SM_ERR_CAPACITY;
+    const int32_t vb = (int32_t)m->h_voff[d], tb = (int32_t)T;
+    for (int64_t k = 0; k < 3 * cnt[d]; ++k) {
+      const int32_t v = dt[d][k];
+      tri[3 * T + k] = v < 0 ? v : v + vb;
+      adj[3 * T + k] = da[d][k] + tb;
+    }
+    T += cnt[d];
+  }
+  return T;
+}
+
 int host_rebuild2d(sm_mesher *m) {
   const int64_t N = m->N;
   std::vector<double> hp((size_t)N * 2);
@@ -555,7 +601,12 @@ int host_rebuild2d(sm_mesher *m) {
   SM_HIP(hipStreamSynchronize(m->stream));
   std::vector<int32_t> tri((size_t)m->capT * 3), adj((size_t)m->capT * 3);
   std::vector<uint8_t> ins((size_t)N);
-  int64_t T = sm_host_delaunay2d(hp.data(), N, tri.data(), adj.data(), m->capT, ins.data());
+  int64_t T = 0;
+  if (m->ndom == 0) {
+    T = sm_host_delaunay2d(hp.data(), N, tri.data(), adj.data(), m->capT, ins.data());
+  } else {
+    T = batch_delaunay(m, hp, tri, adj);
+  }
   if (T < 0) return sm_set_error((int)T, "host Delaunay rebuild failed (%lld)", (long long)T);
   m->T = T;
   SM_HIP(hipMemcpyAsync(m->d_tri, tri.data(), T * 3 * sizeof(int32_t), hipMemcpyHostToDevice,

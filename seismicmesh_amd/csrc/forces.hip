@@ -55,6 +55,32 @@ __global__ void k_keep(int64_t T, const int32_t *tri, const double *p, const sm_
   keep[t] = sdf_eval<DIM, MODE>(tape, ntape, x) < -geps;
 }
 
+// Batch handles: a triangle belongs to the domain of its vertices (the
+// triangulations of different domains never share a triangle); its
+// centroid is tested against that domain's program and geps.
+template <int MODE>
+__global__ void k_keep_batch(int64_t T, const int32_t *tri, const double *p, BatchView bv,
+                             uint8_t *keep) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  const int32_t *c = tri + 3 * t;
+  if (c[0] < 0 || c[1] < 0 || c[2] < 0) {
+    keep[t] = 0;
+    return;
+  }
+  const int dom = bv.vdom[c[0]];
+  double x[2];
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    double s = p[(int64_t)c[0] * 2 + d];
+    s = s + p[(int64_t)c[1] * 2 + d];
+    s = s + p[(int64_t)c[2] * 2 + d];
+    x[d] = s / 3.0;
+  }
+  const double geps = bv.geps_mult * bv.h[dom];
+  keep[t] = sdf_eval<2, MODE>(bv.tapes + bv.tape_off[dom], bv.tape_len[dom], x) < -geps;
+}
+
 // second-endpoint lists: count, fill, order
 __global__ void k_count_second(const int32_t *Ep, const int32_t *edges, int32_t *cnt) {
   const int64_t E = *Ep;
@@ -102,8 +128,8 @@ __device__ inline double cube_rn(double x) {
 // (FP64) for the fast path, or the per-edge powers for the exact path.
 template <int DIM, typename T, bool EXACT>
 __global__ void k_edge_geom(const int32_t *Ep, const int32_t *edges, const double *p,
-                            GridView grid, const double *h_ext, T *L_out, T *h_out, double *Lp,
-                            double *hp, double *part) {
+                            GridView grid, const double *h_ext, BatchView bv, T *L_out, T *h_out,
+                            double *Lp, double *hp, double *part) {
   const int64_t E = *Ep;
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double sl = 0.0, sh = 0.0;
@@ -121,7 +147,8 @@ __global__ void k_edge_geom(const int32_t *Ep, const int32_t *edges, const doubl
     T L = sqrt(L2);
     if (L == (T)0) L = zero_len<T>();
     T h;
-    if (h_ext) h = (T)h_ext[e];
+    if (bv.vdom) h = (T)bv.h[bv.vdom[a]];
+    else if (h_ext) h = (T)h_ext[e];
     else h = (T)grid_eval<DIM>(grid, mid);
     L_out[e] = L;
     h_out[e] = h;
@@ -278,6 +305,37 @@ __global__ void k_np_chunks(const double *a, int64_t n, double *chunk_sums) {
   }
 }
 
+// Batch handles: one workgroup per domain sums L^d and h^d over the
+// domain's edges (a contiguous range of the sorted edge list: the rows of
+// its vertex range) in a fixed order, then scale = (sum L^d / sum h^d)^(1/d).
+template <typename T>
+__global__ void k_domain_sums(const int32_t *eptr, BatchView bv, const T *L, const T *h,
+                              int dim) {
+  __shared__ double rl[256], rh[256];
+  const int dom = blockIdx.x;
+  const int32_t es = eptr[bv.voff[dom]], ee = eptr[bv.voff[dom + 1]];
+  double sl = 0.0, sh = 0.0;
+  for (int32_t e = es + (int32_t)threadIdx.x; e < ee; e += (int32_t)blockDim.x) {
+    const double Le = L[e], he = h[e];
+    sl += dim == 2 ? Le * Le : Le * Le * Le;
+    sh += dim == 2 ? he * he : he * he * he;
+  }
+  rl[threadIdx.x] = sl;
+  rh[threadIdx.x] = sh;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      rl[threadIdx.x] += rl[threadIdx.x + w];
+      rh[threadIdx.x] += rh[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double r = rh[0] > 0.0 ? rl[0] / rh[0] : 0.0;
+    bv.scale[dom] = dim == 2 ? sqrt(r) : cbrt(r);
+  }
+}
+
 __global__ void k_np_total(const double *chunk_sums, int64_t nchunks, double *out) {
   double r = chunk_sums[0];
   for (int64_t i = 1; i < nchunks; ++i) r += chunk_sums[i];
@@ -286,13 +344,13 @@ __global__ void k_np_total(const double *chunk_sums, int64_t nchunks, double *ou
 
 template <int DIM, typename T>
 __global__ void k_edge_force(const int32_t *Ep, const int32_t *edges, const double *p,
-                             const T *L, const T *h, const double *scale_ptr, double L0mult,
-                             T *F) {
+                             const T *L, const T *h, const double *scale_ptr, BatchView bv,
+                             double L0mult, T *F) {
   const int64_t E = *Ep;
   int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
-  const T scale = (T)scale_ptr[2];
   const int a = edges[2 * e], b = edges[2 * e + 1];
+  const T scale = bv.vdom ? (T)bv.scale[bv.vdom[a]] : (T)scale_ptr[2];
   const T Le = L[e];
   T L0 = h[e] * (T)L0mult * scale;
   T f = L0 - Le;
@@ -332,7 +390,7 @@ __global__ void k_vertex_update(int64_t N, double *p, const int32_t *first_ptr,
                                 const int32_t *second_ptr, const int32_t *second, const T *F,
                                 int nfix, double dt, const sm_node *tapes, const int *tape_off,
                                 const int *tape_len, int nlevels, double deps, double hmin,
-                                uint64_t *maxF) {
+                                BatchView bv, uint64_t *maxF) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= N) return;
   T f[DIM];
@@ -347,7 +405,13 @@ __global__ void k_vertex_update(int64_t N, double *p, const int32_t *first_ptr,
 #pragma unroll
     for (int d = 0; d < DIM; ++d) f[d] = f[d] + F[(int64_t)e * DIM + d];
   }
-  if (v < nfix) {
+  int dom = 0;
+  bool fixed = v < nfix;
+  if (bv.vdom) {
+    dom = bv.vdom[v];
+    fixed = v - bv.voff[dom] < bv.nfix[dom];
+  }
+  if (fixed) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) f[d] = (T)0;
   }
@@ -355,12 +419,20 @@ __global__ void k_vertex_update(int64_t N, double *p, const int32_t *first_ptr,
 #pragma unroll
   for (int d = 1; d < DIM; ++d) s2 = s2 + (double)f[d] * (double)f[d];
   double s = sqrt(s2);
+  // batches track the movement relative to each domain's h0
+  if (bv.vdom) s = s / bv.h[dom];
   atomicMax((unsigned long long *)maxF, (unsigned long long)__double_as_longlong(s));
   double x[3];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) x[d] = p[v * DIM + d] + dt * (double)f[d];
-  for (int l = 0; l < nlevels; ++l)
-    project_level<DIM, MODE>(x, tapes + tape_off[l], tape_len[l], deps, hmin, l);
+  if (bv.vdom) {
+    const double h0 = bv.h[dom];
+    project_level<DIM, MODE>(x, bv.tapes + bv.tape_off[dom], bv.tape_len[dom],
+                             bv.deps_mult * h0, h0, 0);
+  } else {
+    for (int l = 0; l < nlevels; ++l)
+      project_level<DIM, MODE>(x, tapes + tape_off[l], tape_len[l], deps, hmin, l);
+  }
 #pragma unroll
   for (int d = 0; d < DIM; ++d) p[v * DIM + d] = x[d];
 }
@@ -377,6 +449,17 @@ static void launch_keep(sm_mesher *m) {
 
 int centroid_keep(sm_mesher *m) {
   if (m->T == 0) return SM_OK;
+  if (m->ndom > 0) {
+    const int B = 256;
+    if (m->deep_sdf)
+      hipLaunchKernelGGL((k_keep_batch<kSdfDeep>), dim3(blocks_for(m->T, B)), dim3(B), 0,
+                         m->stream, m->T, m->d_tri, m->d_p, batch_view(m), m->d_keep);
+    else
+      hipLaunchKernelGGL((k_keep_batch<kSdfReg>), dim3(blocks_for(m->T, B)), dim3(B), 0,
+                         m->stream, m->T, m->d_tri, m->d_p, batch_view(m), m->d_keep);
+    SM_LAUNCH_CHECK();
+    return SM_OK;
+  }
   if (m->dim == 2) {
     if (m->deep_sdf) launch_keep<2, kSdfDeep>(m);
     else launch_keep<2, kSdfReg>(m);
@@ -449,19 +532,25 @@ static int forces_t(sm_mesher *m) {
   const unsigned ge = blocks_for(edge_launch_bound(m), B);
   const int32_t *Ep = m->d_ctr + kCtrE;
   GridView g = m->grid;
+  const BatchView bv = batch_view(m);
   int pb = prof_begin(m);
   if (exact)
     hipLaunchKernelGGL((k_edge_geom<DIM, T, true>), dim3(ge), dim3(B), 0, s, Ep, m->d_edges,
-                       m->d_p, g, m->d_hext, (T *)m->d_L, (T *)m->d_h, m->d_Lp, m->d_hp,
+                       m->d_p, g, m->d_hext, bv, (T *)m->d_L, (T *)m->d_h, m->d_Lp, m->d_hp,
                        m->d_part);
   else
     hipLaunchKernelGGL((k_edge_geom<DIM, T, false>), dim3(ge), dim3(B), 0, s, Ep, m->d_edges,
-                       m->d_p, g, m->d_hext, (T *)m->d_L, (T *)m->d_h, m->d_Lp, m->d_hp,
+                       m->d_p, g, m->d_hext, bv, (T *)m->d_L, (T *)m->d_h, m->d_Lp, m->d_hp,
                        m->d_part);
   SM_LAUNCH_CHECK();
   prof_end(m, 3, pb);
   pb = prof_begin(m);
-  if (exact) {
+  if (m->ndom > 0) {
+    hipLaunchKernelGGL((k_domain_sums<T>), dim3(m->ndom), dim3(256), 0, s,
+                       (const int32_t *)eptr_of(m), bv, (const T *)m->d_L, (const T *)m->d_h,
+                       DIM);
+    SM_LAUNCH_CHECK();
+  } else if (exact) {
     int64_t nch = (E + kNpChunk - 1) / kNpChunk;
     if (nch < 1) nch = 1;
     if (E > 0) {
@@ -489,7 +578,7 @@ static int forces_t(sm_mesher *m) {
   prof_end(m, 4, pb);
   pb = prof_begin(m);
   hipLaunchKernelGGL((k_edge_force<DIM, T>), dim3(ge), dim3(B), 0, s, Ep, m->d_edges, m->d_p,
-                     (const T *)m->d_L, (const T *)m->d_h, m->d_sums, m->prm.L0mult,
+                     (const T *)m->d_L, (const T *)m->d_h, m->d_sums, bv, m->prm.L0mult,
                      (T *)m->d_F);
   SM_LAUNCH_CHECK();
   prof_end(m, 5, pb);
@@ -500,12 +589,14 @@ static int forces_t(sm_mesher *m) {
     hipLaunchKernelGGL((k_vertex_update<DIM, T, kSdfDeep>), dim3(blocks_for(N, B)), dim3(B), 0,
                        s, N, m->d_p, eptr_of(m), m->d_sptr, m->d_slist, (const T *)m->d_F,
                        m->prm.nfix, m->prm.delta_t, m->d_tape_all, m->d_tape_off_dev,
-                       m->d_tape_len_dev, m->prm.nlevels, m->prm.deps, m->prm.h0, m->d_maxF);
+                       m->d_tape_len_dev, m->prm.nlevels, m->prm.deps, m->prm.h0, bv,
+                       m->d_maxF);
   else
     hipLaunchKernelGGL((k_vertex_update<DIM, T, kSdfReg>), dim3(blocks_for(N, B)), dim3(B), 0,
                        s, N, m->d_p, eptr_of(m), m->d_sptr, m->d_slist, (const T *)m->d_F,
                        m->prm.nfix, m->prm.delta_t, m->d_tape_all, m->d_tape_off_dev,
-                       m->d_tape_len_dev, m->prm.nlevels, m->prm.deps, m->prm.h0, m->d_maxF);
+                       m->d_tape_len_dev, m->prm.nlevels, m->prm.deps, m->prm.h0, bv,
+                       m->d_maxF);
   SM_LAUNCH_CHECK();
   prof_end(m, 6, pb);
   return SM_OK;

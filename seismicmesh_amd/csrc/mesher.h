@@ -1,4 +1,5 @@
-// Device state of one force-loop handle (2D path).
+// Device state of one force-loop handle (one domain, or a batch of
+// independent 2D domains).
 #pragma once
 
 #include <vector>
@@ -19,6 +20,18 @@ struct sm_mesher {
   std::vector<int> tape_off;
   int *d_tape_off_dev = nullptr, *d_tape_len_dev = nullptr;
   bool deep_sdf = false;  // some program needs the scratch-stack interpreter
+
+  // batch of independent 2D domains (ndom > 0): domain d owns the vertex
+  // range [voff[d], voff[d+1]), the SDF program tape_off[d] (tapes above,
+  // one per domain instead of one per level), one uniform size h[d] (= its
+  // h0) and fixes the first nfix[d] vertices of its range
+  int ndom = 0;
+  std::vector<int64_t> h_voff;
+  std::vector<double> h_dh;
+  int32_t *d_vdom = nullptr;                      // (N) domain of each vertex
+  int32_t *d_voff = nullptr, *d_dnfix = nullptr;  // (ndom + 1), (ndom)
+  double *d_dh = nullptr;                         // (ndom)
+  double *d_dscale = nullptr;                     // (ndom) L0 scale per iteration
 
   // host sizing callback (for sizing given as a Python callable): called
   // with the edge midpoints, fills h; the device path then uses d_hext
@@ -104,6 +117,34 @@ struct sm_mesher {
 };
 
 namespace smk {
+
+// What the force kernels need to treat a batch handle's domains apart
+// (all pointers null for a single-domain handle).
+struct BatchView {
+  const int32_t *vdom = nullptr, *voff = nullptr, *nfix = nullptr;
+  const double *h = nullptr;
+  double *scale = nullptr;
+  const sm_node *tapes = nullptr;
+  const int *tape_off = nullptr, *tape_len = nullptr;
+  double geps_mult = 0.0, deps_mult = 0.0;
+};
+
+inline BatchView batch_view(const sm_mesher *m) {
+  BatchView b;
+  if (m->ndom == 0) return b;
+  b.vdom = m->d_vdom;
+  b.voff = m->d_voff;
+  b.nfix = m->d_dnfix;
+  b.h = m->d_dh;
+  b.scale = m->d_dscale;
+  b.tapes = m->d_tape_all;
+  b.tape_off = m->d_tape_off_dev;
+  b.tape_len = m->d_tape_len_dev;
+  b.geps_mult = m->prm.geps;
+  b.deps_mult = m->prm.deps;
+  return b;
+}
+
 // d_ctr layout (int32): [4] unique edge count, [16] last round id,
 // [24..31] status of the last batch of flip rounds, [32..) one 8-int block
 // per flip round of a batch

@@ -57,6 +57,7 @@ void free_point_state(sm_mesher *m) {
   dfree(m->d_adj_s);
   dfree(m->d_pvalid);
   dfree(m->d_pstage);
+  dfree(m->d_vdom);
   dfree(m->d_sl_rank);
   dfree(m->d_sl_win);
   dfree(m->d_sl_total);
@@ -119,7 +120,17 @@ int alloc_point_state(sm_mesher *m, int64_t N) {
   if ((rc = dalloc(&m->d_tri_s, T * (dim + 1)))) return rc;
   if ((rc = dalloc(&m->d_adj_s, T * (dim + 1)))) return rc;
   if ((rc = fill<int32_t>(m->d_mark, T, 0, m->stream))) return rc;
+  if (m->ndom > 0 && (rc = dalloc(&m->d_vdom, N))) return rc;
   return SM_OK;
+}
+
+// domain of each vertex from the batch's vertex offsets
+__global__ void k_vertex_domains(int ndom, const int32_t *voff, int32_t *vdom) {
+  const int d = blockIdx.y;
+  const int32_t b = voff[d], e = voff[d + 1];
+  for (int32_t v = b + (int32_t)(blockIdx.x * blockDim.x + threadIdx.x); v < e;
+       v += (int32_t)(gridDim.x * blockDim.x))
+    vdom[v] = d;
 }
 
 __global__ void k_midpoints(int64_t E, int dim, const int32_t *edges, const double *p,
@@ -332,6 +343,79 @@ extern "C" int sm_mesher_create(const sm_mesher_params *prm, const sm_node *cons
   return SM_OK;
 }
 
+extern "C" int sm_mesher_create_batch(const sm_mesher_params *prm, int ndom,
+                                      const sm_node *const *tapes, const int *tape_lens,
+                                      const double *h, const int32_t *nfix,
+                                      const int64_t *voff, int device, sm_mesher **out) {
+  SM_CHECK(prm && tapes && tape_lens && h && nfix && voff && out, SM_ERR_ARG,
+           "mesher_create_batch: null argument");
+  SM_CHECK(prm->dim == 2, SM_ERR_ARG, "mesher_create_batch: batches are 2D");
+  SM_CHECK(prm->precision == 32, SM_ERR_ARG, "mesher_create_batch: batches use FP32 forces");
+  SM_CHECK(ndom >= 1 && ndom <= 65535, SM_ERR_ARG, "mesher_create_batch: bad domain count %d",
+           ndom);
+  SM_CHECK(voff[0] == 0, SM_ERR_ARG, "mesher_create_batch: voff[0] must be 0");
+  for (int d = 0; d < ndom; ++d) {
+    SM_CHECK(voff[d + 1] - voff[d] >= 3, SM_ERR_ARG,
+             "mesher_create_batch: domain %d has fewer than 3 points", d);
+    SM_CHECK(h[d] > 0.0, SM_ERR_ARG, "mesher_create_batch: h of domain %d must be > 0", d);
+    SM_CHECK(nfix[d] >= 0 && nfix[d] <= voff[d + 1] - voff[d], SM_ERR_ARG,
+             "mesher_create_batch: bad nfix of domain %d", d);
+    SM_CHECK(tapes[d] && tape_lens[d] > 0, SM_ERR_ARG, "mesher_create_batch: empty program");
+  }
+  SM_CHECK(voff[ndom] < (int64_t)1 << 30, SM_ERR_ARG, "mesher_create_batch: too many points");
+  // one program per domain takes the place of the levels; sizes are the
+  // domains' own
+  sm_mesher_params p1 = *prm;
+  p1.nlevels = 1;
+  p1.nfix = 0;
+  sm_grid g{};
+  g.dim = 2;
+  g.hconst = 1.0;
+  int rc = sm_mesher_create(&p1, tapes, tape_lens, &g, nullptr, nullptr, device, out);
+  if (rc) return rc;
+  sm_mesher *m = *out;
+  // replace the level table by the per-domain table
+  dfree(m->d_tape_all);
+  dfree(m->d_tape_off_dev);
+  dfree(m->d_tape_len_dev);
+  m->d_tapes.clear();
+  m->tape_off.clear();
+  m->tape_len.clear();
+  m->deep_sdf = false;
+  int total = 0;
+  for (int d = 0; d < ndom; ++d) {
+    m->tape_off.push_back(total);
+    m->tape_len.push_back(tape_lens[d]);
+    total += tape_lens[d];
+    if (tapes[d][0].pad[0] != 0.0) m->deep_sdf = true;
+  }
+  SM_HIP(hipMalloc(&m->d_tape_all, total * sizeof(sm_node)));
+  for (int d = 0; d < ndom; ++d) {
+    SM_HIP(hipMemcpy(m->d_tape_all + m->tape_off[d], tapes[d], tape_lens[d] * sizeof(sm_node),
+                     hipMemcpyHostToDevice));
+    m->d_tapes.push_back(m->d_tape_all + m->tape_off[d]);
+  }
+  SM_HIP(hipMalloc(&m->d_tape_off_dev, ndom * sizeof(int)));
+  SM_HIP(hipMalloc(&m->d_tape_len_dev, ndom * sizeof(int)));
+  SM_HIP(hipMemcpy(m->d_tape_off_dev, m->tape_off.data(), ndom * sizeof(int),
+                   hipMemcpyHostToDevice));
+  SM_HIP(hipMemcpy(m->d_tape_len_dev, m->tape_len.data(), ndom * sizeof(int),
+                   hipMemcpyHostToDevice));
+  m->ndom = ndom;
+  m->h_voff.assign(voff, voff + ndom + 1);
+  m->h_dh.assign(h, h + ndom);
+  std::vector<int32_t> v32(ndom + 1);
+  for (int d = 0; d <= ndom; ++d) v32[d] = (int32_t)voff[d];
+  SM_HIP(hipMalloc(&m->d_voff, (ndom + 1) * sizeof(int32_t)));
+  SM_HIP(hipMalloc(&m->d_dnfix, ndom * sizeof(int32_t)));
+  SM_HIP(hipMalloc(&m->d_dh, ndom * sizeof(double)));
+  SM_HIP(hipMalloc(&m->d_dscale, ndom * sizeof(double)));
+  SM_HIP(hipMemcpy(m->d_voff, v32.data(), (ndom + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+  SM_HIP(hipMemcpy(m->d_dnfix, nfix, ndom * sizeof(int32_t), hipMemcpyHostToDevice));
+  SM_HIP(hipMemcpy(m->d_dh, h, ndom * sizeof(double), hipMemcpyHostToDevice));
+  return SM_OK;
+}
+
 extern "C" int sm_mesher_destroy(sm_mesher *m) {
   if (!m) return SM_OK;
   (void)hipSetDevice(m->device);
@@ -340,6 +424,10 @@ extern "C" int sm_mesher_destroy(sm_mesher *m) {
   dfree(m->d_tape_all);
   dfree(m->d_tape_off_dev);
   dfree(m->d_tape_len_dev);
+  dfree(m->d_voff);
+  dfree(m->d_dnfix);
+  dfree(m->d_dh);
+  dfree(m->d_dscale);
   for (int k = 0; k < 3; ++k) dfree(m->d_axis[k]);
   dfree(m->d_values);
   dfree(m->d_ctr);
@@ -366,6 +454,14 @@ extern "C" int sm_mesher_set_points(sm_mesher *m, const double *p, int64_t n) {
   SM_HIP(hipMemcpyAsync(m->d_p, p, n * m->dim * sizeof(double), hipMemcpyHostToDevice,
                         m->stream));
   m->have_tri = false;
+  if (m->ndom > 0) {
+    SM_CHECK(n == m->h_voff[m->ndom], SM_ERR_ARG,
+             "mesher_set_points: a batch of %d domains holds %lld points (got %lld)", m->ndom,
+             (long long)m->h_voff[m->ndom], (long long)n);
+    hipLaunchKernelGGL(k_vertex_domains, dim3(64, m->ndom), dim3(256), 0, m->stream, m->ndom,
+                       m->d_voff, m->d_vdom);
+    SM_LAUNCH_CHECK();
+  }
   // 3D rebuilds its tetrahedralisation at every triangulate()
   int rc = m->dim == 2 ? host_rebuild2d(m) : SM_OK;
   if (rc) return rc;
@@ -390,7 +486,9 @@ int check_movement(sm_mesher *m) {
   double mf;
   std::memcpy(&mf, m->h_maxF, sizeof(double));
   m->last_maxdp = m->prm.delta_t * mf;
-  if (!(m->last_maxdp < 1000.0 * m->prm.h0))
+  // batches record the movement in units of each domain's h0
+  const double h0 = m->ndom > 0 ? 1.0 : m->prm.h0;
+  if (!(m->last_maxdp < 1000.0 * h0))
     return sm_set_error(SM_ERR_NUMERIC,
                         "max movement indicates there's a convergence problem (%g)",
                         m->last_maxdp);

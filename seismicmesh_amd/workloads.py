@@ -57,3 +57,55 @@ def bp2004(scale=1.0, seed=2004, quiet=True):
 
 def disk(h=0.02):
     return geo.Disk([0.0, 0.0], 1.0), h, {"bbox": (-1.0, 1.0, -1.0, 1.0)}
+
+
+def batch_domain(i, seed0=1000):
+    """Domain i of the many-domains workload: a seeded CSG of 1-6 primitives
+    in [-2, 2]^2 -- disks with r ~ U[0.2, 1] and axis-aligned rectangles with
+    sides ~ U[0.3, 2] -- the first kept, each next one added (union) or cut
+    out (difference)."""
+    rng = np.random.default_rng(seed0 + i)
+    k = int(rng.integers(1, 7))
+    prims = []
+    for _ in range(k):
+        if rng.random() < 0.5:
+            r = rng.uniform(0.2, 1.0)
+            c = rng.uniform(-2.0 + r, 2.0 - r, 2)
+            prims.append(geo.Disk([float(c[0]), float(c[1])], float(r)))
+        else:
+            w, hgt = rng.uniform(0.3, 2.0, 2)
+            c = rng.uniform([-2.0 + w / 2, -2.0 + hgt / 2], [2.0 - w / 2, 2.0 - hgt / 2])
+            prims.append(geo.Rectangle((float(c[0] - w / 2), float(c[0] + w / 2),
+                                        float(c[1] - hgt / 2), float(c[1] + hgt / 2))))
+    ops = rng.random(k) < 0.65
+    dom = prims[0]
+    for prim, union in zip(prims[1:], ops[1:]):
+        dom = geo.Union([dom, prim]) if union else geo.Difference([dom, prim])
+    return dom, rng
+
+
+def domain_area(dom, n=400):
+    """Area of {sdf < 0} from an n x n sample of [-2, 2]^2."""
+    ax = (np.arange(n) + 0.5) / n * 4.0 - 2.0
+    X, Y = np.meshgrid(ax, ax, indexing="ij")
+    d = dom.eval(np.column_stack((X.ravel(), Y.ravel())))
+    return float((d < 0).mean() * 16.0)
+
+
+def many_domains(n=512, seed0=1000, nmin=500, nmax=2.0e5):
+    """The many-domains workload: n seeded CSG domains with uniform sizes
+    h = sqrt(area / N_i), N_i log-uniform in [nmin, nmax].  A domain whose
+    cut-outs leave less than 0.1 of area keeps its first primitive only."""
+    domains, hs = [], []
+    for i in range(n):
+        dom, rng = batch_domain(i, seed0)
+        target = float(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        area = domain_area(dom)
+        if area < 0.1:
+            dom, rng2 = batch_domain(i, seed0)
+            while isinstance(dom, (geo.Union, geo.Difference)):
+                dom = dom.domains[0]
+            area = domain_area(dom)
+        domains.append(dom)
+        hs.append(float(np.sqrt(area / target)))
+    return domains, hs
