@@ -138,12 +138,72 @@ def cpu_baseline(st, p0, iters, reps=3):
     return len(p0) * iters / dt, dt, core
 
 
+def _oracle_domain(dom):
+    """The oracle's copy of a product geometry object (Disk, Rectangle and
+    their unions / differences)."""
+    from oracle import geometry as og
+    name = type(dom).__name__
+    if name == "Disk":
+        return og.Disk([dom.xc, dom.yc], dom.r)
+    if name == "Rectangle":
+        return og.Rectangle(dom.bbox0)
+    if name in ("Union", "Difference"):
+        return getattr(og, name)([_oracle_domain(d) for d in dom.domains])
+    raise ValueError("no oracle copy of %s" % name)
+
+
+def cpu_baseline_many(sts, iters, reps=3, min_points=50000):
+    """The oracle on the first domains of the many-domains workload (at
+    least `min_points` points), `iters` iterations each, one pinned thread,
+    median of `reps` runs."""
+    try:
+        from threadpoolctl import threadpool_limits
+    except ImportError:  # pragma: no cover
+        threadpool_limits = None
+    from oracle import distmesh as odm
+    cases = []
+    total = 0
+    for st in sts:
+        od = _oracle_domain(st["dom"])
+        h = st["h0"]
+        s = dict(fd=od.eval, fh=(lambda x, h=h: np.array([h] * len(x))), h0=h, dim=2,
+                 L0mult=1.2, geps=st["geps"], deps=st["deps"], pfix=st["p"][:st["nfix"]],
+                 levels=[od.eval], delta_t=st["delta_t"])
+        cases.append((st["p"], s, st["nfix"]))
+        total += len(st["p"])
+        if total >= min_points:
+            break
+    old_aff = os.sched_getaffinity(0)
+    core = min(old_aff)
+    os.sched_setaffinity(0, {core})
+    ctx = threadpool_limits(limits=1) if threadpool_limits else None
+    if ctx:
+        ctx.__enter__()
+    times = []
+    try:
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            for p0, s, nfix in cases:
+                p = p0.copy()
+                for _ in range(iters):
+                    p, _, _ = odm.force_step(p, odm.delaunay(p), s, list(range(nfix)))
+            times.append(time.perf_counter() - t0)
+    finally:
+        if ctx:
+            ctx.__exit__(None, None, None)
+        os.sched_setaffinity(0, old_aff)
+    dt = float(np.median(times))
+    return total * iters / dt, dt, core, total
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=49)
     ap.add_argument("--warmup", type=int, default=49)
-    ap.add_argument("--workload", default="bp2004", choices=["bp2004", "disk"])
+    ap.add_argument("--workload", default="bp2004", choices=["bp2004", "disk", "many"])
+    ap.add_argument("--domains", type=int, default=512,
+                    help="number of domains of the many-domains workload")
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
     ap.add_argument("--cpu-iters", type=int, default=10)
@@ -169,17 +229,30 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    if args.workload == "bp2004":
-        domain, ef, info = workloads.bp2004(scale=args.scale)
-        wl = {"workload": "bp2004_2d", "grid": list(info["grid"]), "bbox_m": list(info["bbox"]),
-              "hmin_m": 75.0, "grade": 0.15, "max_iter": 50}
+    sts = None
+    if args.workload == "many":
+        from seismicmesh_amd import batch
+        if args.precision != 32:
+            sys.exit("the many-domains workload runs FP32 forces")
+        doms, hs = workloads.many_domains(n=args.domains)
+        sts = batch.prepare_batch(doms, hs)
+        p0 = np.vstack([s_["p"] for s_ in sts])
+        m = batch.make_batch_mesher(sts)
+        st = None
+        wl = {"workload": "many_domains_2d", "domains": args.domains, "seed0": 1000,
+              "n_per_domain": "log-uniform [500, 2e5]", "max_iter": 50}
     else:
-        domain, ef, info = workloads.disk()
-        wl = {"workload": "unit_disk_h0.02", "max_iter": 50}
-    st = generation.prepare(domain, ef, verbose=0, precision=args.precision)
-    p0 = st["p"]
+        if args.workload == "bp2004":
+            domain, ef, info = workloads.bp2004(scale=args.scale)
+            wl = {"workload": "bp2004_2d", "grid": list(info["grid"]),
+                  "bbox_m": list(info["bbox"]), "hmin_m": 75.0, "grade": 0.15, "max_iter": 50}
+        else:
+            domain, ef, info = workloads.disk()
+            wl = {"workload": "unit_disk_h0.02", "max_iter": 50}
+        st = generation.prepare(domain, ef, verbose=0, precision=args.precision)
+        p0 = st["p"]
+        m = generation.make_mesher(st)
     N = len(p0)
-    m = generation.make_mesher(st)
 
     def timed(graph):
         m.set_graph(graph)
@@ -227,7 +300,15 @@ def main():
                 "flip_share": round(prof["flip_round"][0] / max(others, 1e-9), 4)}
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and sts is not None:
+        rate, secs, core, nsamp = cpu_baseline_many(sts, args.cpu_iters)
+        model, ncpu = cpu_info()
+        cpu = {"value": round(rate, 1), "unit": "vertex-updates/s", "cores": 1, "kind": "port",
+               "sample": "%d force iterations of the first domains of the workload (%d points) "
+                         "with the numpy oracle + scipy Qhull (stand-in for CGAL), one thread "
+                         "pinned to core %d of %d (%s), median of 3 runs of %.1f s"
+                         % (args.cpu_iters, nsamp, core, ncpu, model, secs)}
+    elif not args.no_cpu:
         rate, secs, core = cpu_baseline(st, p0, args.cpu_iters)
         model, ncpu = cpu_info()
         cpu = {"value": round(rate, 1), "unit": "vertex-updates/s", "cores": 1, "kind": "port",

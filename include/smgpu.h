@@ -160,6 +160,15 @@ typedef struct {
   int32_t nfix;           /* the first nfix points are fixed (pfix)          */
   int32_t nlevels;        /* number of level sets (domain + subdomains)      */
   double h0, delta_t, geps, deps, L0mult;
+  int32_t far_square;     /* 2D: 1 = triangulate inside a fixed square of four
+                             far vertices (100 extents from the points)
+                             instead of the convex hull of the points: no
+                             sequential hull-flip chains along straight
+                             boundaries; hull slivers whose circumcircle
+                             reaches that far (kept by the reference on
+                             degenerate initial lattices) are not formed.
+                             Batches always use it.  0 = convex hull.       */
+  int32_t reserved;
 } sm_mesher_params;
 
 /* Creates a handle on device `device`.  `tapes`/`tape_lens` give one SDF
@@ -211,7 +220,8 @@ int sm_mesher_get_edges(sm_mesher *m, int32_t *edges, int64_t cap);
  * [3] host rebuilds (every 3D triangulation is one), [4] exact predicate
  * evaluations, [5] speculative (graph) iterations redone because their flips
  * had not converged, [6] kinetic repairs (points moved in sub-steps from the
- * last valid triangulation). */
+ * last valid triangulation), [7] predicates the device's exact stage could
+ * not settle (each forces a host rebuild). */
 int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats);
 /* Per-phase kernel timing with HIP events recorded on the handle's stream.
  * Phases: 0 flip rounds, 1 centroid filter, 2 unique edges + edge lists,

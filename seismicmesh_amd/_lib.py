@@ -72,6 +72,8 @@ class MesherParams(ctypes.Structure):
         ("geps", ctypes.c_double),
         ("deps", ctypes.c_double),
         ("L0mult", ctypes.c_double),
+        ("far_square", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 

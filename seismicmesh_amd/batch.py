@@ -44,6 +44,7 @@ class BatchMesher(gen.Mesher):
         prm.geps = geps_mult
         prm.deps = float(np.sqrt(np.finfo(np.double).eps))
         prm.L0mult = 1 + 0.4 / 2 ** (2 - 1)
+        prm.far_square = 1
         hd = ctypes.c_void_p()
         _lib.check(_lib.lib.sm_mesher_create_batch(
             ctypes.byref(prm), nd, ptrs, lens, self.hsize.ctypes.data, self.nfix.ctypes.data,

@@ -37,6 +37,10 @@ namespace smk {
 namespace {
 
 constexpr int kGhostOri = 2;
+// vertex id of an unused triangle slot (batch handles keep each domain's
+// triangles in a fixed slot range; a slot the domain does not use is dead:
+// vertices kDeadVertex, neighbours itself, never flipped or kept)
+constexpr int32_t kDeadVertex = -2;
 
 __device__ inline const double *PT(const double *p, int v) { return p + 2 * (int64_t)v; }
 
@@ -197,6 +201,10 @@ __device__ inline int valid_new(const double *p, int a, int b, int c) {
   return s > 0;
 }
 
+__device__ inline bool near_inverted(const int32_t *adj, const int32_t *ori, int64_t t) {
+  return ori[adj[3 * t]] <= 0 || ori[adj[3 * t + 1]] <= 0 || ori[adj[3 * t + 2]] <= 0;
+}
+
 __global__ void k_mark(int64_t T, const int32_t *tri, const int32_t *adj, const int32_t *ori,
                        const double *p, uint64_t *claim, uint64_t *cand, int32_t *queue,
                        int32_t *ctr, int queue_exact) {
@@ -227,6 +235,10 @@ __global__ void k_mark(int64_t T, const int32_t *tri, const int32_t *adj, const 
           ok = valid_new(p, q.x, q.a, q.y) && valid_new(p, q.y, q.b, q.x);
         }
         if (ok) prio = (2ull << 32) | scramble((uint32_t)(3 * t + k), round);
+      } else if (near_inverted(adj, ori, t) || near_inverted(adj, ori, o)) {
+        // no Lawson flips next to an inverted triangle: in a tangled
+        // neighbourhood they can fold two triangles onto each other; the
+        // untangling flips there go first
       } else {
         int s = in_circle_of(p, q.x, q.a, q.b, q.y);
         if (s == SM_PRED_UNRESOLVED) {
@@ -345,7 +357,40 @@ __global__ void k_flip_status(const int32_t *ctr, int nrounds, int32_t *status,
   status[3] = last >= 0 ? ctr[8 * last + 1] : 0;
   status[4] = unres;
   status[5] = exact;
+  status[6] = last >= 0 ? ctr[8 * last + 6] : 0;
   if (last >= 0) *round_counter = ctr[8 * last + 6];
+}
+
+// After a flip batch that did not end valid: flags the domains (one flag
+// for a single-domain handle) holding an inverted or degenerate real
+// triangle, a triangle flipped by the last round, or two vertices at the
+// same position; dup is set for the latter (flips cannot separate them).
+__global__ void k_domain_fail(int64_t T, const int32_t *tri, const double *p,
+                              const int32_t *mark, const int32_t *status,
+                              const int32_t *vdom, int32_t *dflag, int32_t *dup) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  const int32_t a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
+  if (a == kDeadVertex) return;
+  const int last = status[6];
+  bool bad = last > 0 && mark[t] == last;
+  if (a >= 0 && b >= 0 && c >= 0) {
+    const double *pa = PT(p, a), *pb = PT(p, b), *pc = PT(p, c);
+    const bool same = (pa[0] == pb[0] && pa[1] == pb[1]) || (pb[0] == pc[0] && pb[1] == pc[1]) ||
+                      (pa[0] == pc[0] && pa[1] == pc[1]);
+    if (same) {
+      atomicOr(dup, 1);
+      bad = true;
+    } else {
+      int o = smpred::orient2d_filter(pa, pb, pc);
+      if (o == SM_PRED_UNRESOLVED) o = smpred::orient2d_if_exact(pa, pb, pc);
+      if (o == SM_PRED_UNRESOLVED) o = smpred::orient2d_exact(pa, pb, pc);
+      bad |= o <= 0;
+    }
+  }
+  if (!bad) return;
+  const int32_t v = a >= 0 ? a : (b >= 0 ? b : c);
+  dflag[vdom ? vdom[v] : 0] = 1;
 }
 
 }  // namespace
@@ -353,6 +398,28 @@ __global__ void k_flip_status(const int32_t *ctr, int nrounds, int32_t *status,
 __global__ void k_lerp(int64_t n, const double *a, const double *b, double s, double *out) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = a[i] + s * (b[i] - a[i]);
+}
+
+// Batch handles: sub-step only the vertices of masked domains (the others
+// keep their current positions) ...
+__global__ void k_lerp_masked(int64_t n2, const double *a, const double *b, double s,
+                              double *out, const int32_t *vdom, const int32_t *mask) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n2) return;
+  out[i] = mask[vdom[i / 2]] ? a[i] + s * (b[i] - a[i]) : b[i];
+}
+
+// ... and copy only their triangle slots (tdom: domain of each slot).
+__global__ void k_copy_tri_masked(int64_t T, const int32_t *tri_src, const int32_t *adj_src,
+                                  int32_t *tri_dst, int32_t *adj_dst, const int32_t *tdom,
+                                  const int32_t *mask) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T || !mask[tdom[t]]) return;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    tri_dst[3 * t + k] = tri_src[3 * t + k];
+    adj_dst[3 * t + k] = adj_src[3 * t + k];
+  }
 }
 
 // Copies the triangulation and the points it is valid for into the
@@ -373,11 +440,11 @@ __global__ void k_snapshot_if_valid(const int32_t *status, int64_t T, const int3
 }
 
 int enqueue_snapshot(sm_mesher *m, const int32_t *status) {
-  const int64_t work = std::max<int64_t>(3 * m->T, m->N * 2);
+  const int64_t work = std::max<int64_t>(3 * m->T, m->NA * 2);
   unsigned g = blocks_for(work, 256);
   if (g > 2048) g = 2048;
   hipLaunchKernelGGL(k_snapshot_if_valid, dim3(g), dim3(256), 0, m->stream, status, m->T,
-                     m->d_tri, m->d_adj, m->d_tri_s, m->d_adj_s, m->N * 2, m->d_p, m->d_pvalid);
+                     m->d_tri, m->d_adj, m->d_tri_s, m->d_adj_s, m->NA * 2, m->d_p, m->d_pvalid);
   SM_LAUNCH_CHECK();
   return SM_OK;
 }
@@ -413,15 +480,28 @@ int repair_on(sm_mesher *m, const double *pts, int max_rounds) {
 // from the last good state with half the step.  Returns true on success.
 bool kinetic_repair(sm_mesher *m) {
   hipStream_t s = m->stream;
-  const int64_t n2 = m->N * 2;
+  const int64_t n2 = m->NA * 2;
   double s0 = 0.0, step = 0.5;
+  // batches sub-step only the domains flagged by flag_failed_domains
+  const bool masked = m->ndom > 0 && m->d_dfail;
+  const unsigned gT = blocks_for(m->T, 256);
   auto restore = [&]() {
+    if (masked) {
+      hipLaunchKernelGGL(k_copy_tri_masked, dim3(gT), dim3(256), 0, s, m->T, m->d_tri_s,
+                         m->d_adj_s, m->d_tri, m->d_adj, m->d_tdom, m->d_dfail);
+      return;
+    }
     (void)hipMemcpyAsync(m->d_tri, m->d_tri_s, m->T * 3 * sizeof(int32_t),
                          hipMemcpyDeviceToDevice, s);
     (void)hipMemcpyAsync(m->d_adj, m->d_adj_s, m->T * 3 * sizeof(int32_t),
                          hipMemcpyDeviceToDevice, s);
   };
   auto save = [&]() {
+    if (masked) {
+      hipLaunchKernelGGL(k_copy_tri_masked, dim3(gT), dim3(256), 0, s, m->T, m->d_tri,
+                         m->d_adj, m->d_tri_s, m->d_adj_s, m->d_tdom, m->d_dfail);
+      return;
+    }
     (void)hipMemcpyAsync(m->d_tri_s, m->d_tri, m->T * 3 * sizeof(int32_t),
                          hipMemcpyDeviceToDevice, s);
     (void)hipMemcpyAsync(m->d_adj_s, m->d_adj, m->T * 3 * sizeof(int32_t),
@@ -433,8 +513,12 @@ bool kinetic_repair(sm_mesher *m) {
     double s1 = std::min(1.0, s0 + step);
     const double *pts = m->d_p;
     if (s1 < 1.0) {
-      hipLaunchKernelGGL(k_lerp, dim3(blocks_for(n2, 256)), dim3(256), 0, s, n2, m->d_pvalid,
-                         m->d_p, s1, m->d_pstage);
+      if (masked)
+        hipLaunchKernelGGL(k_lerp_masked, dim3(blocks_for(n2, 256)), dim3(256), 0, s, n2,
+                           m->d_pvalid, m->d_p, s1, m->d_pstage, m->d_vdom, m->d_dfail);
+      else
+        hipLaunchKernelGGL(k_lerp, dim3(blocks_for(n2, 256)), dim3(256), 0, s, n2,
+                           m->d_pvalid, m->d_p, s1, m->d_pstage);
       pts = m->d_pstage;
     }
     int ok = repair_on(m, pts, 64);
@@ -506,6 +590,7 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
   *need_rebuild = false;
   int32_t *dstat = m->d_ctr + kCtrStatus;
   bool ok = false, exact_failed = false;
+  int best_inverted = INT32_MAX, stale = 0;
   for (int done = 0; done < max_rounds;) {
     const int batch = 4;
     const int pb = prof_begin(m);
@@ -517,6 +602,7 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
     account_flips(m, m->h_ctr);
     done += m->h_ctr[0];
     if (m->h_ctr[4] > 0) {  // an exact predicate did not fit: rebuild exactly
+      m->stats[7] += m->h_ctr[4];
       exact_failed = true;
       break;
     }
@@ -526,13 +612,75 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
       ok = m->h_ctr[3] == 0;
       break;
     }
+    // inverted triangles that stop decreasing will not be untangled
+    if (m->h_ctr[3] > 0) {
+      if (m->h_ctr[3] < best_inverted) {
+        best_inverted = m->h_ctr[3];
+        stale = 0;
+      } else if (++stale >= 3) {
+        break;
+      }
+    }
   }
-  if (!ok && !exact_failed && m->have_snap) ok = kinetic_repair(m);
+  m->rebuild_flagged = false;
+  if (!ok && !exact_failed) {
+    // which domains failed, and whether coincident vertices are the cause
+    // (flips and sub-steps cannot separate them; the host rebuild drops
+    // the duplicate, as the reference's CGAL insert does)
+    int64_t bad_points = 0;
+    bool dup = false;
+    int rc = flag_failed_domains(m, &bad_points, &dup);
+    if (rc) return rc;
+    // kinetic repair: restart the failed domains from their last valid
+    // state and move their points in sub-steps.  A batch with only a few
+    // failed domains rebuilds those on the host straight away (the usual
+    // cause there is a vertex projected next to a fixed corner, which the
+    // sub-steps cannot get past either)
+    // Sub-steps pay off for large moves (the first iterations); after a
+    // move of less than half an edge length the failure is a local
+    // degeneracy (a vertex driven onto a fixed corner or a straight piece
+    // of boundary) and the host rebuild is cheaper.
+    const double h0 = m->ndom > 0 ? 1.0 : m->prm.h0;
+    const bool large_move = !(m->last_maxdp < 0.5 * h0);
+    const bool kinetic = !dup && m->have_snap && large_move &&
+                         (m->ndom == 0 || 8 * bad_points > m->N);
+    if (kinetic) ok = kinetic_repair(m);
+    // rebuild on the host only the flagged domains (the others were valid
+    // and the kinetic repair left them alone)
+    if (!ok && m->ndom > 0) m->rebuild_flagged = true;
+  }
   if (!ok) {
     *need_rebuild = true;
     return SM_OK;
   }
   return snapshot_valid(m);
+}
+
+// Runs k_domain_fail on the current triangulation; fills m->h_dfail and the
+// number of points of the flagged domains.
+int flag_failed_domains(sm_mesher *m, int64_t *bad_points, bool *dup) {
+  hipStream_t s = m->stream;
+  const int nd = std::max(m->ndom, 1);
+  if ((int)m->h_dfail.size() != nd) m->h_dfail.assign(nd, 0);
+  if (!m->d_dfail) SM_HIP(hipMalloc(&m->d_dfail, (nd + 1) * sizeof(int32_t)));
+  int rc = fill<int32_t>(m->d_dfail, nd + 1, 0, s);
+  if (rc) return rc;
+  const int B = 256;
+  hipLaunchKernelGGL(k_domain_fail, dim3(blocks_for(m->T, B)), dim3(B), 0, s, m->T, m->d_tri,
+                     m->d_p, m->d_mark, (const int32_t *)(m->d_ctr + kCtrStatus), m->d_vdom,
+                     m->d_dfail, m->d_dfail + nd);
+  SM_LAUNCH_CHECK();
+  std::vector<int32_t> fl(nd + 1);
+  SM_HIP(hipMemcpyAsync(fl.data(), m->d_dfail, (nd + 1) * sizeof(int32_t),
+                        hipMemcpyDeviceToHost, s));
+  SM_HIP(hipStreamSynchronize(s));
+  *dup = fl[nd] != 0;
+  *bad_points = 0;
+  for (int d = 0; d < nd; ++d) {
+    m->h_dfail[d] = fl[d] != 0;
+    if (fl[d]) *bad_points += m->ndom > 0 ? m->h_voff[d + 1] - m->h_voff[d] : m->N;
+  }
+  return SM_OK;
 }
 
 // Records the current triangulation and points as the last valid state.
@@ -555,57 +703,102 @@ int snapshot_valid(sm_mesher *m) {
 // concatenated with their vertex and triangle ids offset, so no triangle
 // or neighbour link ever joins two domains.
 static int64_t batch_delaunay(sm_mesher *m, const std::vector<double> &hp,
-                              std::vector<int32_t> &tri, std::vector<int32_t> &adj) {
+                              std::vector<int32_t> &tri, std::vector<int32_t> &adj,
+                              const std::vector<uint8_t> *only) {
   const int nd = m->ndom;
-  std::vector<std::vector<int32_t>> dt(nd), da(nd);
+  std::vector<int> todo;
+  for (int d = 0; d < nd; ++d)
+    if (!only || (*only)[d]) todo.push_back(d);
   std::vector<int64_t> cnt(nd, 0);
   std::atomic<int> next{0};
+  const int64_t N = m->N;
   auto work = [&]() {
     std::vector<uint8_t> ins;
-    for (int d; (d = next.fetch_add(1)) < nd;) {
+    std::vector<double> loc;
+    for (int k; (k = next.fetch_add(1)) < (int)todo.size();) {
+      const int d = todo[k];
       const int64_t b = m->h_voff[d], n = m->h_voff[d + 1] - b;
-      const int64_t cap = 2 * n + 16;
-      dt[d].resize(cap * 3);
-      da[d].resize(cap * 3);
-      ins.resize(n);
-      cnt[d] = sm_host_delaunay2d(hp.data() + 2 * b, n, dt[d].data(), da[d].data(), cap,
-                                  ins.data());
+      // the domain's points, then its four far vertices
+      loc.assign(hp.begin() + 2 * b, hp.begin() + 2 * (b + n));
+      loc.insert(loc.end(), hp.begin() + 2 * (N + 4 * d), hp.begin() + 2 * (N + 4 * d + 4));
+      const int64_t na = n + 4, tb = m->h_tbase[d], cap = m->h_tbase[d + 1] - tb;
+      int32_t *td = tri.data() + 3 * tb, *ad = adj.data() + 3 * tb;
+      ins.resize(na);
+      int64_t c = sm_host_delaunay2d(loc.data(), na, td, ad, cap, ins.data());
+      cnt[d] = c;
+      if (c < 0) continue;
+      for (int64_t j = 0; j < 3 * c; ++j) {
+        const int32_t v = td[j];
+        if (v >= 0) td[j] = v < n ? (int32_t)(b + v) : (int32_t)(N + 4 * d + (v - n));
+        ad[j] += (int32_t)tb;
+      }
+      for (int64_t t = c; t < cap; ++t)
+        for (int j = 0; j < 3; ++j) {
+          td[3 * t + j] = kDeadVertex;
+          ad[3 * t + j] = (int32_t)(tb + t);
+        }
     }
   };
   unsigned nw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  nw = std::min<unsigned>(nw, (unsigned)nd);
+  nw = std::min<unsigned>(nw, (unsigned)std::max<size_t>(todo.size(), 1));
   std::vector<std::thread> pool;
   for (unsigned i = 1; i < nw; ++i) pool.emplace_back(work);
   work();
   for (auto &t : pool) t.join();
-  int64_t T = 0;
-  for (int d = 0; d < nd; ++d) {
+  for (int d : todo)
     if (cnt[d] < 0) return cnt[d];
-    if (T + cnt[d] > m->capT) return SM_ERR_CAPACITY;
-    const int32_t vb = (int32_t)m->h_voff[d], tb = (int32_t)T;
-    for (int64_t k = 0; k < 3 * cnt[d]; ++k) {
-      const int32_t v = dt[d][k];
-      tri[3 * T + k] = v < 0 ? v : v + vb;
-      adj[3 * T + k] = da[d][k] + tb;
-    }
-    T += cnt[d];
+  return m->h_tbase[nd];
+}
+
+// Batch handles: rebuild only the domains flagged by flag_failed_domains
+// and upload their slot ranges; the other domains keep their (valid)
+// triangulations.
+static int rebuild_flagged_domains(sm_mesher *m, const std::vector<double> &hp) {
+  m->rebuild_flagged = false;
+  std::vector<int32_t> tri((size_t)m->capT * 3), adj((size_t)m->capT * 3);
+  int64_t T = batch_delaunay(m, hp, tri, adj, &m->h_dfail);
+  if (T < 0) return sm_set_error((int)T, "host Delaunay rebuild failed (%lld)", (long long)T);
+  for (int d = 0; d < m->ndom; ++d) {
+    if (!m->h_dfail[d]) continue;
+    const int64_t b = m->h_tbase[d], n = m->h_tbase[d + 1] - b;
+    SM_HIP(hipMemcpyAsync(m->d_tri + 3 * b, tri.data() + 3 * b, 3 * n * sizeof(int32_t),
+                          hipMemcpyHostToDevice, m->stream));
+    SM_HIP(hipMemcpyAsync(m->d_adj + 3 * b, adj.data() + 3 * b, 3 * n * sizeof(int32_t),
+                          hipMemcpyHostToDevice, m->stream));
+    m->stats[3] += 1;
   }
-  return T;
+  SM_HIP(hipStreamSynchronize(m->stream));
+  return snapshot_valid(m);
 }
 
 int host_rebuild2d(sm_mesher *m) {
-  const int64_t N = m->N;
-  std::vector<double> hp((size_t)N * 2);
-  SM_HIP(hipMemcpyAsync(hp.data(), m->d_p, N * 2 * sizeof(double), hipMemcpyDeviceToHost,
+  const int64_t NA = m->NA;
+  std::vector<double> hp((size_t)NA * 2);
+  SM_HIP(hipMemcpyAsync(hp.data(), m->d_p, NA * 2 * sizeof(double), hipMemcpyDeviceToHost,
                         m->stream));
   SM_HIP(hipStreamSynchronize(m->stream));
+  if (m->ndom > 0 && m->rebuild_flagged && m->have_tri) return rebuild_flagged_domains(m, hp);
   std::vector<int32_t> tri((size_t)m->capT * 3), adj((size_t)m->capT * 3);
-  std::vector<uint8_t> ins((size_t)N);
+  std::vector<uint8_t> ins((size_t)NA);
   int64_t T = 0;
   if (m->ndom == 0) {
-    T = sm_host_delaunay2d(hp.data(), N, tri.data(), adj.data(), m->capT, ins.data());
+    T = sm_host_delaunay2d(hp.data(), NA, tri.data(), adj.data(), m->capT, ins.data());
   } else {
-    T = batch_delaunay(m, hp, tri, adj);
+    // fixed slot ranges: domain d owns 2 (n_d + 4) slots from h_tbase[d]
+    if ((int)m->h_tbase.size() != m->ndom + 1) {
+      m->h_tbase.assign(m->ndom + 1, 0);
+      std::vector<int32_t> tdom;
+      tdom.reserve(m->capT);
+      for (int d = 0; d < m->ndom; ++d) {
+        const int64_t sz = 2 * (m->h_voff[d + 1] - m->h_voff[d] + 4);
+        m->h_tbase[d + 1] = m->h_tbase[d] + sz;
+        tdom.insert(tdom.end(), sz, d);
+      }
+      SM_CHECK(m->h_tbase[m->ndom] <= m->capT, SM_ERR_CAPACITY, "batch slots exceed capacity");
+      SM_HIP(hipMemcpy(m->d_tdom, tdom.data(), tdom.size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+    }
+    T = batch_delaunay(m, hp, tri, adj, nullptr);
   }
   if (T < 0) return sm_set_error((int)T, "host Delaunay rebuild failed (%lld)", (long long)T);
   m->T = T;

@@ -28,10 +28,17 @@ struct sm_mesher {
   int ndom = 0;
   std::vector<int64_t> h_voff;
   std::vector<double> h_dh;
-  int32_t *d_vdom = nullptr;                      // (N) domain of each vertex
+  int32_t *d_vdom = nullptr;                      // (NA) domain of each vertex
+  int32_t *d_tdom = nullptr;                      // (capT) domain of each slot
+  std::vector<int64_t> h_tbase;                   // (ndom + 1) slot ranges
   int32_t *d_voff = nullptr, *d_dnfix = nullptr;  // (ndom + 1), (ndom)
   double *d_dh = nullptr;                         // (ndom)
   double *d_dscale = nullptr;                     // (ndom) L0 scale per iteration
+  // domains whose triangulation the flips could not repair, and whether the
+  // next host rebuild may be limited to them
+  int32_t *d_dfail = nullptr;
+  std::vector<uint8_t> h_dfail;
+  bool rebuild_flagged = false;
 
   // host sizing callback (for sizing given as a Python callable): called
   // with the edge midpoints, fills h; the device path then uses d_hext
@@ -44,8 +51,15 @@ struct sm_mesher {
   double *d_axis[3] = {nullptr, nullptr, nullptr};
   float *d_values = nullptr;
 
-  // points
-  int64_t N = 0, capN = 0;
+  // points: N moving points, then (2D batches) nfar fixed far vertices --
+  // four per domain, at the corners of a square 200 times the domain's
+  // extent -- so each domain's triangulated region is a fixed square
+  // instead of the convex hull of its points (whose straight, nearly
+  // collinear boundaries would otherwise need long sequential chains of
+  // hull flips; only triangles with circumcircles wider than ~100 extents,
+  // which the centroid filter never keeps, differ); NA = N + nfar
+  int64_t N = 0, capN = 0, NA = 0;
+  int nfar = 0;
   double *d_p = nullptr;
 
   // triangulation (real + ghost triangles; ghost vertex = -1)
@@ -170,6 +184,7 @@ void prof_end(sm_mesher *m, int phase, int begin_idx);
 int prof_drain(sm_mesher *m);
 // flip2d.hip
 int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild);
+int flag_failed_domains(sm_mesher *m, int64_t *bad_points, bool *dup);
 int host_rebuild2d(sm_mesher *m);
 // forces.hip: `sync_edges` reads the edge count back to the host (needed
 // by the exact-order sums); otherwise every launch is sized for the

@@ -58,6 +58,8 @@ void free_point_state(sm_mesher *m) {
   dfree(m->d_pvalid);
   dfree(m->d_pstage);
   dfree(m->d_vdom);
+  dfree(m->d_tdom);
+  m->h_tbase.clear();
   dfree(m->d_sl_rank);
   dfree(m->d_sl_win);
   dfree(m->d_sl_total);
@@ -78,15 +80,17 @@ int alloc_point_state(sm_mesher *m, int64_t N) {
   free_point_state(m);
   const int dim = m->dim;
   m->capN = N;
-  // 2D: real + ghost triangles of the triangulated sphere, 2N - 2;
+  m->nfar = dim == 2 && (m->ndom > 0 || m->prm.far_square) ? 4 * std::max(m->ndom, 1) : 0;
+  const int64_t NA = N + m->nfar;
+  // 2D: real + ghost triangles of the triangulated sphere, 2 NA - 2;
   // 3D: tetrahedra of a Delaunay tetrahedralisation, ~6.5N for well-spaced
   // points, bounded here generously
-  m->capT = dim == 2 ? 2 * N + 16 : 10 * N + 64;
+  m->capT = dim == 2 ? 2 * NA + 16 : 10 * N + 64;
   const int64_t T = m->capT;
   const int64_t npairs = T * (dim == 2 ? 3 : 6);
   m->capE = npairs;
   int rc;
-  if ((rc = dalloc(&m->d_p, N * dim))) return rc;
+  if ((rc = dalloc(&m->d_p, NA * dim))) return rc;
   if ((rc = dalloc(&m->d_tri, T * (dim + 1)))) return rc;
   if ((rc = dalloc(&m->d_adj, T * (dim + 1)))) return rc;
   if ((rc = dalloc(&m->d_ori, T))) return rc;
@@ -114,13 +118,14 @@ int alloc_point_state(sm_mesher *m, int64_t N) {
     if ((rc = dalloc(&m->d_mid, m->capE * dim))) return rc;
     if ((rc = dalloc(&m->d_hext, m->capE))) return rc;
   }
-  if ((rc = dalloc(&m->d_psnap, N * dim))) return rc;
-  if ((rc = dalloc(&m->d_pvalid, N * dim))) return rc;
-  if ((rc = dalloc(&m->d_pstage, N * dim))) return rc;
+  if ((rc = dalloc(&m->d_psnap, NA * dim))) return rc;
+  if ((rc = dalloc(&m->d_pvalid, NA * dim))) return rc;
+  if ((rc = dalloc(&m->d_pstage, NA * dim))) return rc;
   if ((rc = dalloc(&m->d_tri_s, T * (dim + 1)))) return rc;
   if ((rc = dalloc(&m->d_adj_s, T * (dim + 1)))) return rc;
   if ((rc = fill<int32_t>(m->d_mark, T, 0, m->stream))) return rc;
-  if (m->ndom > 0 && (rc = dalloc(&m->d_vdom, N))) return rc;
+  if (m->ndom > 0 && (rc = dalloc(&m->d_vdom, NA))) return rc;
+  if (m->ndom > 0 && (rc = dalloc(&m->d_tdom, T))) return rc;
   return SM_OK;
 }
 
@@ -428,6 +433,7 @@ extern "C" int sm_mesher_destroy(sm_mesher *m) {
   dfree(m->d_dnfix);
   dfree(m->d_dh);
   dfree(m->d_dscale);
+  dfree(m->d_dfail);
   for (int k = 0; k < 3; ++k) dfree(m->d_axis[k]);
   dfree(m->d_values);
   dfree(m->d_ctr);
@@ -450,14 +456,45 @@ extern "C" int sm_mesher_set_points(sm_mesher *m, const double *p, int64_t n) {
     int rc = alloc_point_state(m, n);
     if (rc) return rc;
   }
-  m->N = n;
-  SM_HIP(hipMemcpyAsync(m->d_p, p, n * m->dim * sizeof(double), hipMemcpyHostToDevice,
-                        m->stream));
-  m->have_tri = false;
-  if (m->ndom > 0) {
+  if (m->ndom > 0)
     SM_CHECK(n == m->h_voff[m->ndom], SM_ERR_ARG,
              "mesher_set_points: a batch of %d domains holds %lld points (got %lld)", m->ndom,
              (long long)m->h_voff[m->ndom], (long long)n);
+  m->N = n;
+  m->NA = n + m->nfar;
+  SM_HIP(hipMemcpyAsync(m->d_p, p, n * m->dim * sizeof(double), hipMemcpyHostToDevice,
+                        m->stream));
+  m->have_tri = false;
+  if (m->nfar > 0) {
+    // far square of each domain (of the whole point set for one domain)
+    const int nd = std::max(m->ndom, 1);
+    std::vector<double> far((size_t)m->nfar * 2);
+    std::vector<int32_t> fdom((size_t)m->nfar);
+    for (int d = 0; d < nd; ++d) {
+      const int64_t b = m->ndom > 0 ? m->h_voff[d] : 0, e = m->ndom > 0 ? m->h_voff[d + 1] : n;
+      double lo[2] = {p[2 * b], p[2 * b + 1]}, hi[2] = {lo[0], lo[1]};
+      for (int64_t i = b; i < e; ++i)
+        for (int k = 0; k < 2; ++k) {
+          lo[k] = std::min(lo[k], p[2 * i + k]);
+          hi[k] = std::max(hi[k], p[2 * i + k]);
+        }
+      const double ext = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), 1e-300);
+      const double c[2] = {(lo[0] + hi[0]) / 2, (lo[1] + hi[1]) / 2};
+      const double K = 100.0 * ext;
+      const double sx[4] = {-1, 1, 1, -1}, sy[4] = {-1, -1, 1, 1};
+      for (int k = 0; k < 4; ++k) {
+        far[2 * (4 * d + k)] = c[0] + K * sx[k];
+        far[2 * (4 * d + k) + 1] = c[1] + K * sy[k];
+        fdom[4 * d + k] = d;
+      }
+    }
+    SM_HIP(hipMemcpy(m->d_p + 2 * n, far.data(), far.size() * sizeof(double),
+                     hipMemcpyHostToDevice));
+    if (m->ndom > 0)
+      SM_HIP(hipMemcpy(m->d_vdom + n, fdom.data(), fdom.size() * sizeof(int32_t),
+                       hipMemcpyHostToDevice));
+  }
+  if (m->ndom > 0) {
     hipLaunchKernelGGL(k_vertex_domains, dim3(64, m->ndom), dim3(256), 0, m->stream, m->ndom,
                        m->d_voff, m->d_vdom);
     SM_LAUNCH_CHECK();
@@ -540,7 +577,7 @@ int capture_iteration(sm_mesher *m) {
   SM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int rc = SM_OK;
   do {
-    if (hipMemcpyAsync(m->d_psnap, m->d_p, m->N * m->dim * sizeof(double),
+    if (hipMemcpyAsync(m->d_psnap, m->d_p, m->NA * m->dim * sizeof(double),
                        hipMemcpyDeviceToDevice, s) != hipSuccess) {
       rc = sm_set_error(SM_ERR_HIP, "capture: snapshot copy failed");
       break;
@@ -593,7 +630,7 @@ int iterate_graph(sm_mesher *m) {
     return check_movement(m);
   }
   m->stats[5] += 1;  // speculative iteration rolled back
-  SM_HIP(hipMemcpyAsync(m->d_p, m->d_psnap, m->N * m->dim * sizeof(double),
+  SM_HIP(hipMemcpyAsync(m->d_p, m->d_psnap, m->NA * m->dim * sizeof(double),
                         hipMemcpyDeviceToDevice, m->stream));
   // more flip rounds (settling uncertain predicates exactly, which the
   // captured rounds leave to this path) or a kinetic repair from the last

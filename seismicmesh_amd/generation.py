@@ -128,7 +128,7 @@ class Mesher:
     """Owns one libsmgpu force-loop handle (2D triangles or 3D tetrahedra)."""
 
     def __init__(self, levels, sizing, h0, delta_t, geps, deps, nfix, precision=64, device=0,
-                 dim=2):
+                 dim=2, far_square=False):
         _lib.require_gpu()
         self.dim = dim
         self._tapes = [geo.tape_array(l.tape()) for l in levels]
@@ -144,6 +144,7 @@ class Mesher:
         prm.geps = geps
         prm.deps = deps
         prm.L0mult = 1 + 0.4 / 2 ** (dim - 1)
+        prm.far_square = int(bool(far_square))
         self._sizing = sizing
         g = _lib.Grid()
         g.dim = dim
@@ -240,7 +241,7 @@ class Mesher:
         s = (ctypes.c_int64 * 8)()
         _lib.check(_lib.lib.sm_mesher_stats(self.h, s, 8), "sm_mesher_stats")
         keys = ("iterations", "flip_rounds", "flips", "host_rebuilds", "exact_predicates",
-                "rollbacks", "kinetic_repairs")
+                "rollbacks", "kinetic_repairs", "unresolved")
         return {k: int(s[i]) for i, k in enumerate(keys)}
 
 
@@ -366,10 +367,11 @@ def prepare(domain, edge_length, **kwargs):
                 nfix=nfix, p=p, levels=levels, max_iter=max_iter, dim=dim)
 
 
-def make_mesher(st):
+def make_mesher(st, far_square=False):
     """A force-loop handle for a prepared problem (initial points not set)."""
     return Mesher(st["levels"], st["fh"], st["h0"], st["delta_t"], st["geps"], st["deps"],
-                  st["nfix"], precision=st["opts"]["precision"], dim=st["dim"])
+                  st["nfix"], precision=st["opts"]["precision"], dim=st["dim"],
+                  far_square=far_square)
 
 
 Mesher2D = Mesher
