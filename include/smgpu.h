@@ -221,7 +221,8 @@ int sm_mesher_get_edges(sm_mesher *m, int32_t *edges, int64_t cap);
  * evaluations, [5] speculative (graph) iterations redone because their flips
  * had not converged, [6] kinetic repairs (points moved in sub-steps from the
  * last valid triangulation), [7] predicates the device's exact stage could
- * not settle (each forces a host rebuild). */
+ * not settle (each forces a host rebuild), [8] vertices taken out and put
+ * back on the host to untangle what flips could not. */
 int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats);
 /* Per-phase kernel timing with HIP events recorded on the handle's stream.
  * Phases: 0 flip rounds, 1 centroid filter, 2 unique edges + edge lists,

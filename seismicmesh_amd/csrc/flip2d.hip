@@ -31,6 +31,7 @@
 #include "common.h"
 #include "mesher.h"
 #include "predicates.h"
+#include "relocate2d.h"
 
 namespace smk {
 
@@ -623,6 +624,37 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
     }
   }
   m->rebuild_flagged = false;
+  // A move of less than half an edge length leaves local tangles (a vertex
+  // slid across a flat triangle along a straight boundary, or driven almost
+  // onto a corner): relocate one vertex of each inverted triangle on the
+  // host, then flip again.  Large moves (the first iterations) go to the
+  // kinetic repair below.
+  const double h0m = m->ndom > 0 ? 1.0 : m->prm.h0;
+  const bool large_move = !(m->last_maxdp < 0.5 * h0m);
+  for (int pass = 0; pass < 3 && !ok && !exact_failed && !large_move && m->h_ctr[3] > 0 &&
+                     m->h_ctr[3] <= 4096 && m->have_snap;
+       ++pass) {
+    int64_t moved = 0;
+    int rc = relocate_inverted(m, &moved);
+    if (rc) return rc;
+    if (moved <= 0) break;
+    for (int done = 0; done < max_rounds;) {
+      rc = enqueue_flip_rounds(m, 4, dstat, m->d_p, true);
+      if (rc) return rc;
+      SM_HIP(hipMemcpyAsync(m->h_ctr, dstat, 8 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+      SM_HIP(hipStreamSynchronize(s));
+      account_flips(m, m->h_ctr);
+      done += m->h_ctr[0];
+      if (m->h_ctr[4] > 0) {
+        exact_failed = true;
+        break;
+      }
+      if (m->h_ctr[2]) {
+        ok = m->h_ctr[3] == 0;
+        break;
+      }
+    }
+  }
   if (!ok && !exact_failed) {
     // which domains failed, and whether coincident vertices are the cause
     // (flips and sub-steps cannot separate them; the host rebuild drops
@@ -640,8 +672,6 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
     // move of less than half an edge length the failure is a local
     // degeneracy (a vertex driven onto a fixed corner or a straight piece
     // of boundary) and the host rebuild is cheaper.
-    const double h0 = m->ndom > 0 ? 1.0 : m->prm.h0;
-    const bool large_move = !(m->last_maxdp < 0.5 * h0);
     const bool kinetic = !dup && m->have_snap && large_move &&
                          (m->ndom == 0 || 8 * bad_points > m->N);
     if (kinetic) ok = kinetic_repair(m);
@@ -654,6 +684,69 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
     return SM_OK;
   }
   return snapshot_valid(m);
+}
+
+// Host relocation (relocate2d.cpp) of a vertex of every inverted triangle:
+// the whole triangulation for one domain, the failed domains' slot ranges
+// for a batch.  *moved: relocations done.
+int relocate_inverted(sm_mesher *m, int64_t *moved) {
+  hipStream_t s = m->stream;
+  *moved = 0;
+  std::vector<std::pair<int64_t, int64_t>> ranges;
+  if (m->ndom > 0) {
+    int64_t bad = 0;
+    bool dup = false;
+    int rc = flag_failed_domains(m, &bad, &dup);
+    if (rc) return rc;
+    for (int d = 0; d < m->ndom; ++d)
+      if (m->h_dfail[d]) ranges.push_back({m->h_tbase[d], m->h_tbase[d + 1] - m->h_tbase[d]});
+  } else {
+    ranges.push_back({0, m->T});
+  }
+  if (ranges.empty()) return SM_OK;
+  const int64_t NA = m->NA;
+  if ((int64_t)m->h_prel.size() != 4 * NA) m->h_prel.assign(4 * NA, 0.0);
+  double *hp = m->h_prel.data(), *hold = hp + 2 * NA;
+  if (m->ndom > 0) {
+    // points of the failed domains and their far vertices
+    for (int d = 0; d < m->ndom; ++d) {
+      if (!m->h_dfail[d]) continue;
+      const int64_t b = m->h_voff[d], n = m->h_voff[d + 1] - b, f = m->N + 4 * d;
+      SM_HIP(hipMemcpyAsync(hp + 2 * b, m->d_p + 2 * b, 2 * n * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+      SM_HIP(hipMemcpyAsync(hold + 2 * b, m->d_pvalid + 2 * b, 2 * n * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+      SM_HIP(hipMemcpyAsync(hp + 2 * f, m->d_p + 2 * f, 8 * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+      SM_HIP(hipMemcpyAsync(hold + 2 * f, m->d_pvalid + 2 * f, 8 * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    }
+  } else {
+    SM_HIP(hipMemcpyAsync(hp, m->d_p, 2 * NA * sizeof(double), hipMemcpyDeviceToHost, s));
+    SM_HIP(hipMemcpyAsync(hold, m->d_pvalid, 2 * NA * sizeof(double), hipMemcpyDeviceToHost, s));
+  }
+  std::vector<int32_t> tri, adj;
+  for (const auto &r : ranges) {
+    tri.resize(3 * r.second);
+    adj.resize(3 * r.second);
+    SM_HIP(hipMemcpyAsync(tri.data(), m->d_tri + 3 * r.first, 3 * r.second * sizeof(int32_t),
+                          hipMemcpyDeviceToHost, s));
+    SM_HIP(hipMemcpyAsync(adj.data(), m->d_adj + 3 * r.first, 3 * r.second * sizeof(int32_t),
+                          hipMemcpyDeviceToHost, s));
+    SM_HIP(hipStreamSynchronize(s));
+    int64_t failed = 0;
+    const int64_t n =
+        sm_relocate2d(hp, hold, tri.data(), adj.data(), r.first, r.second, &failed);
+    if (n == 0) continue;
+    *moved += n;
+    m->stats[8] += n;
+    SM_HIP(hipMemcpyAsync(m->d_tri + 3 * r.first, tri.data(), 3 * r.second * sizeof(int32_t),
+                          hipMemcpyHostToDevice, s));
+    SM_HIP(hipMemcpyAsync(m->d_adj + 3 * r.first, adj.data(), 3 * r.second * sizeof(int32_t),
+                          hipMemcpyHostToDevice, s));
+    SM_HIP(hipStreamSynchronize(s));
+  }
+  return SM_OK;
 }
 
 // Runs k_domain_fail on the current triangulation; fills m->h_dfail and the

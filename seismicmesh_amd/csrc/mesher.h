@@ -39,6 +39,7 @@ struct sm_mesher {
   int32_t *d_dfail = nullptr;
   std::vector<uint8_t> h_dfail;
   bool rebuild_flagged = false;
+  std::vector<double> h_prel;  // host points for relocations (p, then pvalid)
 
   // host sizing callback (for sizing given as a Python callable): called
   // with the edge midpoints, fills h; the device path then uses d_hext
@@ -112,7 +113,7 @@ struct sm_mesher {
   int64_t sl_capT = 0, sl_capN = 0;
   int64_t nsl = -1;  // selected slivers, -1 before a selection
 
-  int64_t stats[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   int round = 0;
   bool have_tri = false;
   double last_maxdp = 0.0;
@@ -185,6 +186,7 @@ int prof_drain(sm_mesher *m);
 // flip2d.hip
 int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild);
 int flag_failed_domains(sm_mesher *m, int64_t *bad_points, bool *dup);
+int relocate_inverted(sm_mesher *m, int64_t *moved);
 int host_rebuild2d(sm_mesher *m);
 // forces.hip: `sync_edges` reads the edge count back to the host (needed
 // by the exact-order sums); otherwise every launch is sized for the

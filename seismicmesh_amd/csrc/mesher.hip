@@ -729,7 +729,7 @@ extern "C" int sm_mesher_get_edges(sm_mesher *m, int32_t *edges, int64_t cap) {
 
 extern "C" int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats) {
   SM_CHECK(m && stats, SM_ERR_ARG, "mesher_stats: null argument");
-  for (int i = 0; i < nstats && i < 8; ++i) stats[i] = m->stats[i];
+  for (int i = 0; i < nstats && i < 9; ++i) stats[i] = m->stats[i];
   return SM_OK;
 }
 

@@ -238,10 +238,10 @@ class Mesher:
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.PHASES)}
 
     def stats(self):
-        s = (ctypes.c_int64 * 8)()
-        _lib.check(_lib.lib.sm_mesher_stats(self.h, s, 8), "sm_mesher_stats")
+        s = (ctypes.c_int64 * 9)()
+        _lib.check(_lib.lib.sm_mesher_stats(self.h, s, 9), "sm_mesher_stats")
         keys = ("iterations", "flip_rounds", "flips", "host_rebuilds", "exact_predicates",
-                "rollbacks", "kinetic_repairs", "unresolved")
+                "rollbacks", "kinetic_repairs", "unresolved", "relocations")
         return {k: int(s[i]) for i, k in enumerate(keys)}
 
 
