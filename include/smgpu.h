@@ -32,6 +32,8 @@
  *   sm_mesher_find_slivers,
  *   sm_mesher_perturb_slivers         <- generation/mesh_generator.py:222-272,
  *                                       :532-540 (sliver_removal loop body)
+ *   sm_mlp_*, sm_gemm_f32,
+ *   sm_mesher_set_surrogate           <- no counterpart (neural SDF surrogate)
  *   sm_host_delaunay2d                <- generation/cpp/delaunay_class.cpp
  *                                       :108-121, :232-298 (CGAL insert and
  *                                       finite cells/vertices)
@@ -112,6 +114,79 @@ typedef struct {
  * copy of `ntape` nodes. */
 int sm_sdf_eval(const sm_node *tape, int ntape, int dim, const double *x, int64_t n,
                 double *d, void *stream);
+
+/* ------------------------------------------------- neural SDF surrogate
+ * No counterpart in the reference (it evaluates the CSG tape exactly,
+ * signed_distance_functions.py:283-623).  A multilayer perceptron
+ * in_dim -> width -> ... -> width -> 1 with SiLU activations approximates
+ * the signed distance:  x' = (x - shift) * scale  (per axis),
+ * a_0 = silu(W_in x' + b_in), a_l = silu(W_l a_{l-1} + b_l) for
+ * l = 1..hidden, d = out_scale * (w_out . a_hidden + b_out).
+ * The hidden weights are BF16 (the inference kernel multiplies them on the
+ * matrix cores with FP32 accumulation; activations are rounded to BF16
+ * between layers); the input and output layers stay FP32.  Every pointer
+ * is device memory. */
+typedef struct {
+  int32_t in_dim;      /* 2 or 3                                              */
+  int32_t width;       /* 256                                                 */
+  int32_t hidden;      /* number of width x width layers (3 for 3->256x4->1)  */
+  int32_t reserved;
+  const float *w_in;   /* [width][in_dim]                                     */
+  const float *b_in;   /* [width]                                             */
+  const void *w_hid;   /* bf16 [hidden][width (out)][width (in)], 16-B aligned */
+  const float *b_hid;  /* [hidden][width]                                     */
+  const float *w_out;  /* [width]                                             */
+  float b_out;
+  float reserved2;
+  double shift[3];
+  double scale[3];
+  double out_scale;
+} sm_mlp;
+
+/* d[i] = mlp(x[i]) (FP32 result), x (n, in_dim) FP64 row-major. */
+int sm_mlp_eval(const sm_mlp *net, const double *x, int64_t n, float *d, void *stream);
+/* d[t] = mlp(centroid of cell t), cells (ncells, in_dim + 1) int32; a cell
+ * with a negative vertex id gets +inf. */
+int sm_mlp_eval_centroids(const sm_mlp *net, const int32_t *cells, int64_t ncells,
+                          const double *p, float *d, void *stream);
+
+/* Training kernels (FP32).  The forward and backward passes of the
+ * network are sequences of these calls on caller-owned buffers.
+ *
+ * C[m, n] = alpha * sum_k A(m, k) B(k, n) + beta * C[m, n] with
+ * A(m, k) = A[m * sam + k * sak], B(k, n) = B[k * sbk + n * sbn] and C
+ * row-major with leading dimension ldc.  With nsplit > 1 the k range is cut
+ * into nsplit slices whose partial products go to `work`
+ * (nsplit * M * N floats) and are summed in slice order, so the result does
+ * not depend on scheduling. */
+int sm_gemm_f32(int M, int N, int K, float alpha, const float *A, int64_t sam, int64_t sak,
+                const float *B, int64_t sbk, int64_t sbn, float beta, float *C, int64_t ldc,
+                int nsplit, float *work, void *stream);
+/* z[r, c] += b[c]; a[r, c] = silu(z[r, c]) unless a is NULL. */
+int sm_mlp_bias_silu(int64_t rows, int cols, float *z, const float *b, float *a, void *stream);
+/* dz = da * silu'(z) elementwise. */
+int sm_mlp_silu_backward(int64_t count, const float *z, const float *da, float *dz,
+                         void *stream);
+/* Training inputs: for each of the n samples x (FP64, (n, dim)), rows
+ * b * n + i of X (FP32, ((2 dim + 1) n, dim)) hold x' = (x - shift) * scale
+ * (b = 0) and x' +- h e_k (b = 2k + 1, 2k + 2) when eik_h > 0; only b = 0
+ * when eik_h == 0. */
+int sm_mlp_make_inputs(int64_t n, int dim, const double *x, const double *shift,
+                       const double *scale, float eik_h, float *X, void *stream);
+/* Loss of the surrogate and its gradient with respect to the network
+ * outputs y (same row blocks as sm_mlp_make_inputs):
+ *   L = mean_i (y_i - t_i)^2 + lam * mean_i (|g_i| - 1)^2,
+ *   g_i,k = (y[(2k+1) n + i] - y[(2k+2) n + i]) / (2 eik_h)
+ * (a central-difference eikonal term; lam = 0 drops it).  part[b] receives
+ * the loss partial of block b of 256 samples (ceil(n / 256) floats). */
+int sm_mlp_sdf_loss(int64_t n, int dim, const float *y, const float *t, float eik_h, float lam,
+                    float *grad, float *part, void *stream);
+/* Adam (Kingma & Ba): m1 = b1 m1 + (1-b1) g, m2 = b2 m2 + (1-b2) g^2,
+ * w -= lr (m1 / (1 - b1^step)) / (sqrt(m2 / (1 - b2^step)) + eps). */
+int sm_mlp_adam(int64_t count, float *w, const float *g, float *m1, float *m2, float lr,
+                float b1, float b2, float eps, int step, void *stream);
+/* dst (bf16) = round-to-nearest-even(src). */
+int sm_f32_to_bf16(int64_t count, const float *src, void *dst, void *stream);
 
 /* --------------------------------------------------------- sizing grids */
 typedef struct {
@@ -247,6 +322,22 @@ int sm_mesher_find_slivers(sm_mesher *m, double min_dh, double max_dh, int64_t *
  * selected cells share a first vertex the last one in cell order is
  * applied.  The selection is consumed; triangulate again before the next. */
 int sm_mesher_perturb_slivers(sm_mesher *m, double scale);
+
+/* Neural surrogate for the handle's SDF (no counterpart in the reference;
+ * single-domain handles without subdomains).  From the next iteration on
+ * the centroid filter and the projection test first evaluate `net` (the
+ * struct is copied; the weights it points to must stay alive): a cell
+ * centroid with d_nn <= -band is kept and one with d_nn >= band dropped
+ * without the exact program, a point with d_nn <= -band is not projected;
+ * everything else goes through the exact program as before.  The result
+ * is the exact path's whenever |d_nn - d| < band - geps at every skipped
+ * evaluation.  net = NULL switches the surrogate off. */
+int sm_mesher_set_surrogate(sm_mesher *m, const sm_mlp *net, double band);
+/* Exact evaluations skipped and made since the surrogate was set, and the
+ * largest |d_nn - d| over the evaluations made (the network's error in
+ * the band, judged by the exact program). */
+int sm_mesher_surrogate_stats(sm_mesher *m, int64_t *skipped, int64_t *exact,
+                              double *max_err);
 
 /* ------------------------------------------------------- host triangulation */
 /* Delaunay triangulation of the convex hull of p (n, 2), exact predicates.

@@ -85,6 +85,7 @@ _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
 _int = ctypes.c_int
 _dbl = ctypes.c_double
+_flt = ctypes.c_float
 _sz = ctypes.c_size_t
 
 _PROTOS = {
@@ -118,6 +119,19 @@ _PROTOS = {
     "sm_mesher_stream": (_int, [_vp, ctypes.POINTER(_vp)]),
     "sm_mesher_find_slivers": (_int, [_vp, _dbl, _dbl, ctypes.POINTER(_i64)]),
     "sm_mesher_perturb_slivers": (_int, [_vp, _dbl]),
+    "sm_mesher_set_surrogate": (_int, [_vp, _vp, _dbl]),
+    "sm_mesher_surrogate_stats": (_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
+                                         ctypes.POINTER(_dbl)]),
+    "sm_mlp_eval": (_int, [_vp, _vp, _i64, _vp, _vp]),
+    "sm_mlp_eval_centroids": (_int, [_vp, _vp, _i64, _vp, _vp, _vp]),
+    "sm_gemm_f32": (_int, [_int, _int, _int, _flt, _vp, _i64, _i64, _vp, _i64, _i64, _flt,
+                           _vp, _i64, _int, _vp, _vp]),
+    "sm_mlp_bias_silu": (_int, [_i64, _int, _vp, _vp, _vp, _vp]),
+    "sm_mlp_silu_backward": (_int, [_i64, _vp, _vp, _vp, _vp]),
+    "sm_mlp_make_inputs": (_int, [_i64, _int, _vp, _vp, _vp, _flt, _vp, _vp]),
+    "sm_mlp_sdf_loss": (_int, [_i64, _int, _vp, _vp, _flt, _flt, _vp, _vp, _vp]),
+    "sm_mlp_adam": (_int, [_i64, _vp, _vp, _vp, _vp, _flt, _flt, _flt, _flt, _int, _vp]),
+    "sm_f32_to_bf16": (_int, [_i64, _vp, _vp, _vp]),
     "sm_host_delaunay2d": (_i64, [_vp, _i64, _vp, _vp, _i64, _vp]),
     "sm_host_delaunay3d": (_i64, [_vp, _i64, _vp, _i64, _vp]),
     "sm_dihedral_angles": (_int, [_vp, _vp, _i64, _vp, _vp]),

@@ -364,11 +364,11 @@ __global__ void k_edge_force(const int32_t *Ep, const int32_t *edges, const doub
 }
 
 template <int DIM, int MODE>
-__device__ inline void project_level(double *x, const sm_node *tape, int ntape, double deps,
-                                     double hmin, int level) {
+__device__ inline double project_level(double *x, const sm_node *tape, int ntape, double deps,
+                                       double hmin, int level) {
   double d = sdf_eval<DIM, MODE>(tape, ntape, x);
   bool in = level == 0 ? (d > 0.0) : (d > 0.0 && d < hmin / 1.5);
-  if (!in) return;
+  if (!in) return d;
   double g[DIM];
 #pragma unroll
   for (int i = 0; i < DIM; ++i) {
@@ -383,6 +383,7 @@ __device__ inline void project_level(double *x, const sm_node *tape, int ntape, 
   if (g2 < deps) g2 = deps;
 #pragma unroll
   for (int i = 0; i < DIM; ++i) x[i] = x[i] - (d * g[i]) / g2;
+  return d;
 }
 
 template <int DIM, typename T, int MODE>
@@ -390,7 +391,7 @@ __global__ void k_vertex_update(int64_t N, double *p, const int32_t *first_ptr,
                                 const int32_t *second_ptr, const int32_t *second, const T *F,
                                 int nfix, double dt, const sm_node *tapes, const int *tape_off,
                                 const int *tape_len, int nlevels, double deps, double hmin,
-                                BatchView bv, uint64_t *maxF) {
+                                BatchView bv, uint64_t *maxF, int project) {
   int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= N) return;
   T f[DIM];
@@ -425,7 +426,9 @@ __global__ void k_vertex_update(int64_t N, double *p, const int32_t *first_ptr,
   double x[3];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) x[d] = p[v * DIM + d] + dt * (double)f[d];
-  if (bv.vdom) {
+  if (!project) {
+    // the surrogate path projects in k_project_surrogate
+  } else if (bv.vdom) {
     const double h0 = bv.h[dom];
     project_level<DIM, MODE>(x, bv.tapes + bv.tape_off[dom], bv.tape_len[dom],
                              bv.deps_mult * h0, h0, 0);
@@ -435,6 +438,80 @@ __global__ void k_vertex_update(int64_t N, double *p, const int32_t *first_ptr,
   }
 #pragma unroll
   for (int d = 0; d < DIM; ++d) p[v * DIM + d] = x[d];
+}
+
+// counters of the surrogate: one atomic per wave
+__device__ inline void wave_count(uint64_t *ctr, bool pred) {
+  const uint64_t act = __ballot(1);
+  const uint64_t hit = __ballot(pred);
+  if ((int)__lane_id() == __ffsll((long long)act) - 1 && hit)
+    atomicAdd((unsigned long long *)ctr, (unsigned long long)__popcll(hit));
+}
+
+__device__ inline void record_error(uint64_t *ctr, double dn, double d) {
+  const double err = fabs(dn - d);
+  if (err == err) atomicMax((unsigned long long *)ctr, (unsigned long long)__double_as_longlong(err));
+}
+
+// Centroid filter with the surrogate: outside the band |d_nn| < band the
+// network's sign decides (a centroid with d_nn <= -band is kept, one with
+// d_nn >= band dropped); inside the band, or where d_nn is not finite, the
+// exact program decides as in k_keep.  The decisions agree with k_keep
+// wherever |d_nn - d| < band - geps.
+template <int DIM, int MODE>
+__global__ void k_keep_surrogate(int64_t T, const int32_t *tri, const double *p,
+                                 const sm_node *tape, int ntape, double geps, const float *dnn,
+                                 double band, uint8_t *keep, uint64_t *sg) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= T) return;
+  const int32_t *c = tri + (DIM + 1) * t;
+  bool real = true;
+#pragma unroll
+  for (int k = 0; k <= DIM; ++k) real &= c[k] >= 0;
+  const double dn = real ? (double)dnn[t] : 0.0;
+  const bool exact = real && !(fabs(dn) >= band);
+  wave_count(sg + 0, real && !exact);
+  wave_count(sg + 1, exact);
+  if (!real) {
+    keep[t] = 0;
+  } else if (!exact) {
+    keep[t] = dn < 0.0;
+  } else {
+    double x[3];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      double s = p[(int64_t)c[0] * DIM + d];
+#pragma unroll
+      for (int k = 1; k <= DIM; ++k) s = s + p[(int64_t)c[k] * DIM + d];
+      x[d] = s / (double)(DIM + 1);
+    }
+    const double d = sdf_eval<DIM, MODE>(tape, ntape, x);
+    record_error(sg + 2, dn, d);
+    keep[t] = d < -geps;
+  }
+}
+
+// Projection with the surrogate: points with d_nn <= -band are inside
+// (no projection); every other point is tested and projected with the
+// exact program, as in k_vertex_update.
+template <int DIM, int MODE>
+__global__ void k_project_surrogate(int64_t N, double *p, const float *dnn, double band,
+                                    const sm_node *tape, int ntape, double deps, double hmin,
+                                    uint64_t *sg) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= N) return;
+  const double dn = (double)dnn[v];
+  const bool exact = !(dn <= -band);
+  wave_count(sg + 0, !exact);
+  wave_count(sg + 1, exact);
+  if (!exact) return;
+  double x[3];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) x[d] = p[v * DIM + d];
+  const double d = project_level<DIM, MODE>(x, tape, ntape, deps, hmin, 0);
+  record_error(sg + 2, dn, d);
+#pragma unroll
+  for (int k = 0; k < DIM; ++k) p[v * DIM + k] = x[k];
 }
 
 }  // namespace
@@ -447,6 +524,26 @@ static void launch_keep(sm_mesher *m) {
                      m->d_keep);
 }
 
+template <int DIM, int MODE>
+static void launch_keep_surrogate(sm_mesher *m) {
+  const int B = 256;
+  hipLaunchKernelGGL((k_keep_surrogate<DIM, MODE>), dim3(blocks_for(m->T, B)), dim3(B), 0,
+                     m->stream, m->T, m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0],
+                     m->prm.geps, m->d_dnn, m->band, m->d_keep, m->d_sg);
+}
+
+template <int DIM, int MODE>
+static int project_surrogate(sm_mesher *m) {
+  int rc = mlp_eval_points(&m->net, m->d_p, m->N, m->d_dnn, m->stream);
+  if (rc) return rc;
+  const int B = 256;
+  hipLaunchKernelGGL((k_project_surrogate<DIM, MODE>), dim3(blocks_for(m->N, B)), dim3(B), 0,
+                     m->stream, m->N, m->d_p, m->d_dnn, m->band, m->d_tapes[0], m->tape_len[0],
+                     m->prm.deps, m->prm.h0, m->d_sg);
+  SM_LAUNCH_CHECK();
+  return SM_OK;
+}
+
 int centroid_keep(sm_mesher *m) {
   if (m->T == 0) return SM_OK;
   if (m->ndom > 0) {
@@ -457,6 +554,19 @@ int centroid_keep(sm_mesher *m) {
     else
       hipLaunchKernelGGL((k_keep_batch<kSdfReg>), dim3(blocks_for(m->T, B)), dim3(B), 0,
                          m->stream, m->T, m->d_tri, m->d_p, batch_view(m), m->d_keep);
+    SM_LAUNCH_CHECK();
+    return SM_OK;
+  }
+  if (m->surrogate) {
+    int rc = mlp_eval_cells(&m->net, m->d_tri, m->T, m->d_p, m->d_dnn, m->stream);
+    if (rc) return rc;
+    if (m->dim == 2) {
+      if (m->deep_sdf) launch_keep_surrogate<2, kSdfDeep>(m);
+      else launch_keep_surrogate<2, kSdfReg>(m);
+    } else {
+      if (m->deep_sdf) launch_keep_surrogate<3, kSdfDeep>(m);
+      else launch_keep_surrogate<3, kSdfReg>(m);
+    }
     SM_LAUNCH_CHECK();
     return SM_OK;
   }
@@ -590,14 +700,18 @@ static int forces_t(sm_mesher *m) {
                        s, N, m->d_p, eptr_of(m), m->d_sptr, m->d_slist, (const T *)m->d_F,
                        m->prm.nfix, m->prm.delta_t, m->d_tape_all, m->d_tape_off_dev,
                        m->d_tape_len_dev, m->prm.nlevels, m->prm.deps, m->prm.h0, bv,
-                       m->d_maxF);
+                       m->d_maxF, m->surrogate ? 0 : 1);
   else
     hipLaunchKernelGGL((k_vertex_update<DIM, T, kSdfReg>), dim3(blocks_for(N, B)), dim3(B), 0,
                        s, N, m->d_p, eptr_of(m), m->d_sptr, m->d_slist, (const T *)m->d_F,
                        m->prm.nfix, m->prm.delta_t, m->d_tape_all, m->d_tape_off_dev,
                        m->d_tape_len_dev, m->prm.nlevels, m->prm.deps, m->prm.h0, bv,
-                       m->d_maxF);
+                       m->d_maxF, m->surrogate ? 0 : 1);
   SM_LAUNCH_CHECK();
+  if (m->surrogate) {
+    rc = m->deep_sdf ? project_surrogate<DIM, kSdfDeep>(m) : project_surrogate<DIM, kSdfReg>(m);
+    if (rc) return rc;
+  }
   prof_end(m, 6, pb);
   return SM_OK;
 }

@@ -113,6 +113,17 @@ struct sm_mesher {
   int64_t sl_capT = 0, sl_capN = 0;
   int64_t nsl = -1;  // selected slivers, -1 before a selection
 
+  // neural SDF surrogate (single domain, one level): the network decides
+  // the centroid filter and the projection test wherever |d_nn| >= band;
+  // inside the band the exact program runs.  d_sg: [skipped exact
+  // evaluations, exact evaluations, max |d_nn - d| (double bits) over them]
+  bool surrogate = false;
+  sm_mlp net{};
+  double band = 0.0;
+  float *d_dnn = nullptr;
+  int64_t dnn_cap = 0;
+  uint64_t *d_sg = nullptr;
+
   int64_t stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   int round = 0;
   bool have_tri = false;
@@ -194,4 +205,9 @@ int host_rebuild2d(sm_mesher *m);
 int centroid_keep(sm_mesher *m);
 int build_edges(sm_mesher *m, bool sync_edges);
 int forces_and_update(sm_mesher *m);
+// mlp.hip: surrogate inference (points, or centroids of cells)
+int mlp_check(const sm_mlp *net);
+int mlp_eval_points(const sm_mlp *net, const double *x, int64_t n, float *d, hipStream_t s);
+int mlp_eval_cells(const sm_mlp *net, const int32_t *cells, int64_t ncells, const double *p,
+                   float *d, hipStream_t s);
 }  // namespace smk

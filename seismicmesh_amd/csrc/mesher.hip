@@ -59,6 +59,8 @@ void free_point_state(sm_mesher *m) {
   dfree(m->d_pstage);
   dfree(m->d_vdom);
   dfree(m->d_tdom);
+  dfree(m->d_dnn);
+  m->dnn_cap = 0;
   m->h_tbase.clear();
   dfree(m->d_sl_rank);
   dfree(m->d_sl_win);
@@ -126,6 +128,10 @@ int alloc_point_state(sm_mesher *m, int64_t N) {
   if ((rc = fill<int32_t>(m->d_mark, T, 0, m->stream))) return rc;
   if (m->ndom > 0 && (rc = dalloc(&m->d_vdom, NA))) return rc;
   if (m->ndom > 0 && (rc = dalloc(&m->d_tdom, T))) return rc;
+  if (m->surrogate) {
+    m->dnn_cap = std::max(T, NA);
+    if ((rc = dalloc(&m->d_dnn, m->dnn_cap))) return rc;
+  }
   return SM_OK;
 }
 
@@ -439,6 +445,7 @@ extern "C" int sm_mesher_destroy(sm_mesher *m) {
   dfree(m->d_ctr);
   dfree(m->d_sums);
   dfree(m->d_maxF);
+  dfree(m->d_sg);
   if (m->h_ctr) (void)hipHostFree(m->h_ctr);
   if (m->h_status) (void)hipHostFree(m->h_status);
   if (m->h_sums) (void)hipHostFree(m->h_sums);
@@ -730,6 +737,58 @@ extern "C" int sm_mesher_get_edges(sm_mesher *m, int32_t *edges, int64_t cap) {
 extern "C" int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats) {
   SM_CHECK(m && stats, SM_ERR_ARG, "mesher_stats: null argument");
   for (int i = 0; i < nstats && i < 9; ++i) stats[i] = m->stats[i];
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_set_surrogate(sm_mesher *m, const sm_mlp *net, double band) {
+  SM_CHECK(m, SM_ERR_ARG, "mesher_set_surrogate: null handle");
+  SM_HIP(hipSetDevice(m->device));
+  if (!net) {
+    m->surrogate = false;
+  } else {
+    int rc = mlp_check(net);
+    if (rc) return rc;
+    SM_CHECK(net->in_dim == m->dim, SM_ERR_ARG, "mesher_set_surrogate: network takes %dD points",
+             net->in_dim);
+    SM_CHECK(band > 0.0, SM_ERR_ARG, "mesher_set_surrogate: band must be positive");
+    SM_CHECK(m->ndom == 0 && m->prm.nlevels == 1 && !m->host_sizing, SM_ERR_STATE,
+             "mesher_set_surrogate: single-domain handles without subdomains only");
+    m->net = *net;
+    m->band = band;
+    m->surrogate = true;
+    if (!m->d_sg) {
+      int rc2 = dalloc(&m->d_sg, 4);
+      if (rc2) return rc2;
+    }
+    int rc3 = fill<uint64_t>(m->d_sg, 4, 0ull, m->stream);
+    if (rc3) return rc3;
+    const int64_t need = std::max(m->capT, m->NA);
+    if (m->capN > 0 && m->dnn_cap < need) {
+      dfree(m->d_dnn);
+      m->dnn_cap = need;
+      int rc4 = dalloc(&m->d_dnn, need);
+      if (rc4) return rc4;
+    }
+  }
+  // the captured iteration no longer matches the kernels to run
+  if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->gexec = nullptr;
+  m->graph = nullptr;
+  m->graph_T = -1;
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_surrogate_stats(sm_mesher *m, int64_t *skipped, int64_t *exact,
+                                         double *max_err) {
+  SM_CHECK(m && skipped && exact && max_err, SM_ERR_ARG, "mesher_surrogate_stats: null argument");
+  SM_CHECK(m->d_sg, SM_ERR_STATE, "mesher_surrogate_stats: no surrogate was set");
+  uint64_t v[4];
+  SM_HIP(hipMemcpyAsync(v, m->d_sg, sizeof(v), hipMemcpyDeviceToHost, m->stream));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  *skipped = (int64_t)v[0];
+  *exact = (int64_t)v[1];
+  std::memcpy(max_err, &v[2], sizeof(double));
   return SM_OK;
 }
 

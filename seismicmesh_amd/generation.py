@@ -33,7 +33,7 @@ _KNOWN = {
     "verbose", "max_iter", "seed", "perform_checks", "pfix", "axis", "points", "domain",
     "edge_length", "bbox", "min_dh_angle_bound", "max_dh_angle_bound", "delta_t", "h0",
     "geps_mult", "subdomains", "gamma", "preserve", "mesh_improvement", "r0m_is_h0",
-    "precision",
+    "precision", "surrogate", "surrogate_band",
 }
 
 
@@ -237,6 +237,30 @@ class Mesher:
         _lib.check(_lib.lib.sm_mesher_profile_read(self.h, ms, n, 8), "sm_mesher_profile_read")
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.PHASES)}
 
+    def set_surrogate(self, net, band):
+        """Let `net` (a neural.NeuralSDF of this handle's dimension) answer
+        the centroid filter and the projection test where |d_nn| >= band;
+        net=None switches it off.  The handle keeps the weights the network
+        has now (train before, not after)."""
+        if net is None:
+            self._net = None
+            _lib.check(_lib.lib.sm_mesher_set_surrogate(self.h, None, 0.0),
+                       "sm_mesher_set_surrogate")
+            return
+        st = net.struct()
+        self._net = (net, st)
+        _lib.check(_lib.lib.sm_mesher_set_surrogate(self.h, ctypes.byref(st), float(band)),
+                   "sm_mesher_set_surrogate")
+
+    def surrogate_stats(self):
+        """(exact evaluations skipped, exact evaluations made, max |d_nn - d|
+        over the evaluations made)."""
+        a, b, e = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_double()
+        _lib.check(_lib.lib.sm_mesher_surrogate_stats(self.h, ctypes.byref(a), ctypes.byref(b),
+                                                      ctypes.byref(e)),
+                   "sm_mesher_surrogate_stats")
+        return {"skipped": a.value, "exact": b.value, "max_err": e.value}
+
     def stats(self):
         s = (ctypes.c_int64 * 9)()
         _lib.check(_lib.lib.sm_mesher_stats(self.h, s, 9), "sm_mesher_stats")
@@ -384,6 +408,9 @@ def generate_mesh(domain, edge_length, comm=None, **kwargs):
     v1 = opts["verbose"] >= 1
     v2 = opts["verbose"] > 1
     m = make_mesher(st)
+    if opts.get("surrogate") is not None:
+        # band |d_nn| < surrogate_band * h0 falls back to the exact SDF
+        m.set_surrogate(opts["surrogate"], opts.get("surrogate_band", 2.0) * h0)
     m.set_points(st["p"])
     updates = max(st["max_iter"] - 1, 0)
     if v2:

@@ -109,3 +109,43 @@ def many_domains(n=512, seed0=1000, nmin=500, nmax=2.0e5):
         domains.append(dom)
         hs.append(float(np.sqrt(area / target)))
     return domains, hs
+
+
+def csg3d(seed=3, nballs=48, nboxes=16, ncyl=8, ntori=4):
+    """The many-primitive 3D CSG: ((nballs balls U nboxes boxes) minus ncyl
+    cylinders) U ntori tori, all inside [-1, 1]^3, seeded.  Balls have
+    r ~ U[0.05, 0.3]; boxes sides ~ U[0.1, 0.5]; cylinders (axis z)
+    r ~ U[0.05, 0.15], length ~ U[0.5, 1.5]; tori r1 ~ U[0.15, 0.35],
+    r2 ~ U[0.03, 0.08].  Returns (domain, bbox)."""
+    rng = np.random.default_rng(seed)
+    solids = []
+    for _ in range(nballs):
+        r = rng.uniform(0.05, 0.3)
+        c = rng.uniform(-1.0 + r, 1.0 - r, 3)
+        solids.append(geo.Ball([float(v) for v in c], float(r)))
+    for _ in range(nboxes):
+        s = rng.uniform(0.1, 0.5, 3)
+        c = rng.uniform(-1.0 + s / 2, 1.0 - s / 2)
+        solids.append(geo.Cube(tuple(float(v) for k in range(3)
+                                     for v in (c[k] - s[k] / 2, c[k] + s[k] / 2))))
+    cuts = []
+    for _ in range(ncyl):
+        r = rng.uniform(0.05, 0.15)
+        ln = rng.uniform(0.5, 1.5)
+        c = rng.uniform(-1.0 + r, 1.0 - r, 3)
+        c[2] = rng.uniform(-1.0 + ln / 2, 1.0 - ln / 2)
+        cuts.append(geo.Cylinder(h=float(ln), r=float(r), translate=[float(v) for v in c]))
+    tori = []
+    for _ in range(ntori):
+        r1 = rng.uniform(0.15, 0.35)
+        r2 = rng.uniform(0.03, 0.08)
+        c = rng.uniform(-1.0 + r1 + r2, 1.0 - r1 - r2, 3)
+        tori.append(geo.Torus(float(r1), float(r2), translate=[float(v) for v in c]))
+    body = geo.Union(solids) if len(solids) > 1 else solids[0]
+    if cuts:
+        body = geo.Difference([body] + cuts)
+    if tori:
+        body = geo.Union([body] + tori)
+    bbox = (-1.0, 1.0, -1.0, 1.0, -1.0, 1.0)
+    body.bbox = bbox
+    return body, bbox
