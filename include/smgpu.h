@@ -133,8 +133,8 @@ typedef struct {
   int32_t reserved;
   const float *w_in;   /* [width][in_dim]                                     */
   const float *b_in;   /* [width]                                             */
-  const void *w_hid;   /* bf16 [hidden][width (out)][width (in)], 16-B aligned */
-  const float *b_hid;  /* [hidden][width]                                     */
+  const void *w_hid;   /* bf16 fragments from sm_mlp_pack_hidden, 16-B aligned */
+  const float *b_hid;  /* [hidden][width], 16-B aligned                      */
   const float *w_out;  /* [width]                                             */
   float b_out;
   float reserved2;
@@ -185,6 +185,10 @@ int sm_mlp_sdf_loss(int64_t n, int dim, const float *y, const float *t, float ei
  * w -= lr (m1 / (1 - b1^step)) / (sqrt(m2 / (1 - b2^step)) + eps). */
 int sm_mlp_adam(int64_t count, float *w, const float *g, float *m1, float *m2, float lr,
                 float b1, float b2, float eps, int step, void *stream);
+/* Hidden weights w (FP32, [hidden][width (out)][width (in)]) rounded to
+ * BF16 (nearest even) and stored in the matrix-core fragment order that
+ * sm_mlp.w_hid expects (hidden * width * width values, 16-B aligned). */
+int sm_mlp_pack_hidden(int hidden, int width, const float *w, void *packed, void *stream);
 /* dst (bf16) = round-to-nearest-even(src). */
 int sm_f32_to_bf16(int64_t count, const float *src, void *dst, void *stream);
 

@@ -132,6 +132,7 @@ _PROTOS = {
     "sm_mlp_sdf_loss": (_int, [_i64, _int, _vp, _vp, _flt, _flt, _vp, _vp, _vp]),
     "sm_mlp_adam": (_int, [_i64, _vp, _vp, _vp, _vp, _flt, _flt, _flt, _flt, _int, _vp]),
     "sm_f32_to_bf16": (_int, [_i64, _vp, _vp, _vp]),
+    "sm_mlp_pack_hidden": (_int, [_int, _int, _vp, _vp, _vp]),
     "sm_host_delaunay2d": (_i64, [_vp, _i64, _vp, _vp, _i64, _vp]),
     "sm_host_delaunay3d": (_i64, [_vp, _i64, _vp, _i64, _vp]),
     "sm_dihedral_angles": (_int, [_vp, _vp, _i64, _vp, _vp]),

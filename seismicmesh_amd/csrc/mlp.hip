@@ -57,10 +57,18 @@ __device__ inline bool mlp_input(const double *x, const int32_t *cells, int64_t 
   return true;
 }
 
+// SiLU with the hardware reciprocal (1 ulp) for the inference kernel, whose
+// activations are rounded to BF16 (8 bits) anyway.
+__device__ inline float silu_fast(float z) {
+  return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
 template <int IN, bool CELLS>
 __global__ __launch_bounds__(256) void k_mlp_eval(sm_mlp net, const double *x,
                                                   const int32_t *cells, int64_t n, float *out) {
-  __shared__ __bf16 hs[kWaves][32][kRow];
+  __shared__ __align__(16) __bf16 hs[kWaves][32][kRow];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int r = lane % 32, half = lane / 32;
   const int64_t pi = ((int64_t)blockIdx.x * kWaves + wave) * 32 + r;
@@ -74,56 +82,81 @@ __global__ __launch_bounds__(256) void k_mlp_eval(sm_mlp net, const double *x,
     for (int c = 0; c < IN; ++c)
       xin[c] = valid ? (float)((q[c] - net.shift[c]) * net.scale[c]) : 0.0f;
   }
-  // input layer: this lane computes features [128 half, 128 half + 128) of its point
-  for (int j = 0; j < kW / 2; ++j) {
-    const int f = half * (kW / 2) + j;
-    float z = net.b_in[f];
+  // input layer: this lane computes features [128 half, 128 half + 128) of
+  // its point, stored 8 at a time
+  for (int j0 = 0; j0 < kW / 2; j0 += 8) {
+    bf16x8 v;
 #pragma unroll
-    for (int c = 0; c < IN; ++c) z = fmaf(net.w_in[f * IN + c], xin[c], z);
-    hs[wave][r][f] = (__bf16)silu(z);
+    for (int j = 0; j < 8; ++j) {
+      const int f = half * (kW / 2) + j0 + j;
+      float z = net.b_in[f];
+#pragma unroll
+      for (int c = 0; c < IN; ++c) z = fmaf(net.w_in[f * IN + c], xin[c], z);
+      v[j] = (__bf16)silu_fast(z);
+    }
+    *reinterpret_cast<bf16x8 *>(&hs[wave][r][half * (kW / 2) + j0]) = v;
   }
   __syncthreads();
-  const __bf16 *whid = reinterpret_cast<const __bf16 *>(net.w_hid);
+  // Hidden layers as C^T = W A^T: the A operand is a weight fragment (32
+  // features x 16 k: lane holds feature 32 nb + r, k = 8 half .. 8 half + 7,
+  // packed by k_pack_hidden), the B operand the activations (16 k x 32
+  // points: lane holds point r, same k), and the accumulator gives lane r
+  // the features 32 nb + 8 (i / 4) + 4 half + i % 4 (i = 0..15) of point r,
+  // four consecutive features per group -- one 8-byte LDS store each.
+  const bf16x8 *whid = reinterpret_cast<const bf16x8 *>(net.w_hid);
+  float s = 0.0f;
   for (int l = 0; l < net.hidden; ++l) {
-    const __bf16 *W = whid + (size_t)l * kW * kW;
+    const bf16x8 *W = whid + (size_t)l * (kW * kW / 8) + lane;
     floatx16 acc[8];
 #pragma unroll
     for (int nb = 0; nb < 8; ++nb)
 #pragma unroll
       for (int i = 0; i < 16; ++i) acc[nb][i] = 0.0f;
-    // A (32 points x 16 k): lane holds row r, k = 8 half .. 8 half + 7;
-    // B (16 k x 32 features): lane holds feature r of block nb, same k
+    // the next k-step's weight fragments are loaded before this step's products
+    bf16x8 wc[8], wn[8];
+#pragma unroll
+    for (int nb = 0; nb < 8; ++nb) wc[nb] = W[nb * 64];
     for (int ks = 0; ks < kW / 16; ++ks) {
-      const int k0 = 16 * ks + 8 * half;
-      const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&hs[wave][r][k0]);
+      if (ks + 1 < kW / 16) {
 #pragma unroll
-      for (int nb = 0; nb < 8; ++nb) {
-        const bf16x8 b =
-            *reinterpret_cast<const bf16x8 *>(W + (size_t)(nb * 32 + r) * kW + k0);
-        acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[nb], 0, 0, 0);
+        for (int nb = 0; nb < 8; ++nb) wn[nb] = W[((ks + 1) * 8 + nb) * 64];
       }
+      const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&hs[wave][r][16 * ks + 8 * half]);
+#pragma unroll
+      for (int nb = 0; nb < 8; ++nb)
+        acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wc[nb], a, acc[nb], 0, 0, 0);
+#pragma unroll
+      for (int nb = 0; nb < 8; ++nb) wc[nb] = wn[nb];
     }
-    __syncthreads();
-    // C (32 points x 32 features): lane holds feature r of block nb, points
-    // 8 (i / 4) + 4 half + i % 4 for i = 0..15
     const float *bias = net.b_hid + (size_t)l * kW;
+    const bool last = l + 1 == net.hidden;
+    if (!last) __syncthreads();  // every lane has read the layer's input
 #pragma unroll
-    for (int nb = 0; nb < 8; ++nb) {
-      const int col = nb * 32 + r;
-      const float bc = bias[col];
+    for (int nb = 0; nb < 8; ++nb)
 #pragma unroll
-      for (int i = 0; i < 16; ++i) {
-        const int row = 8 * (i / 4) + 4 * half + (i % 4);
-        hs[wave][row][col] = (__bf16)silu(acc[nb][i] + bc);
+      for (int g = 0; g < 4; ++g) {
+        const int f0 = 32 * nb + 8 * g + 4 * half;
+        const float4 b4 = *reinterpret_cast<const float4 *>(bias + f0);
+        const float bb[4] = {b4.x, b4.y, b4.z, b4.w};
+        bf16x4 v;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = (__bf16)silu_fast(acc[nb][4 * g + e] + bb[e]);
+        if (last) {
+          // output layer straight from the registers (FP32 weights over the
+          // BF16-rounded activations)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) s = fmaf((float)v[e], net.w_out[f0 + e], s);
+        } else {
+          *reinterpret_cast<bf16x4 *>(&hs[wave][r][f0]) = v;
+        }
       }
-    }
-    __syncthreads();
+    if (!last) __syncthreads();
   }
-  // output layer (FP32 weights over the BF16 activations)
-  float s = 0.0f;
-  for (int j = 0; j < kW / 2; ++j) {
-    const int f = half * (kW / 2) + j;
-    s = fmaf((float)hs[wave][r][f], net.w_out[f], s);
+  if (net.hidden == 0) {
+    for (int j = 0; j < kW / 2; ++j) {
+      const int f = half * (kW / 2) + j;
+      s = fmaf((float)hs[wave][r][f], net.w_out[f], s);
+    }
   }
   s += __shfl_xor(s, 32);
   if (half == 0 && valid) out[pi] = (float)(net.out_scale * (double)(s + net.b_out));
@@ -295,6 +328,24 @@ __global__ void k_adam(int64_t count, float *w, const float *g, float *m1, float
   w[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
 }
 
+// Hidden weights W[l][out][in] (FP32) -> BF16 fragments in the order the
+// inference kernel reads them: index ((l * 16 + ks) * 8 + nb) * 64 + lane
+// holds W[l][32 nb + lane % 32][16 ks + 8 (lane / 32) + j], j = 0..7.
+__global__ void k_pack_hidden(int hidden, const float *w, bf16x8 *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t per_layer = kW * kW / 8;
+  if (i >= hidden * per_layer) return;
+  const int l = (int)(i / per_layer);
+  const int rem = (int)(i % per_layer);
+  const int lane = rem % 64, nb = (rem / 64) % 8, ks = rem / 512;
+  const float *src = w + (size_t)l * kW * kW + (size_t)(32 * nb + lane % 32) * kW + 16 * ks +
+                     8 * (lane / 32);
+  bf16x8 v;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) v[j] = (__bf16)src[j];
+  out[i] = v;
+}
+
 __global__ void k_to_bf16(int64_t count, const float *src, __bf16 *dst) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) dst[i] = (__bf16)src[i];
@@ -326,7 +377,8 @@ int mlp_check(const sm_mlp *net) {
   SM_CHECK(net->w_in && net->b_in && net->w_out &&
                (net->hidden == 0 || (net->w_hid && net->b_hid)),
            SM_ERR_ARG, "mlp: null weight pointer");
-  SM_CHECK(((uintptr_t)net->w_hid & 15) == 0, SM_ERR_ARG, "mlp: w_hid not 16-byte aligned");
+  SM_CHECK(((uintptr_t)net->w_hid & 15) == 0 && ((uintptr_t)net->b_hid & 15) == 0, SM_ERR_ARG,
+           "mlp: w_hid and b_hid must be 16-byte aligned");
   return SM_OK;
 }
 
@@ -438,6 +490,20 @@ extern "C" int sm_mlp_adam(int64_t count, float *w, const float *g, float *m1, f
   const float c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
   hipLaunchKernelGGL(k_adam, dim3(blocks_for(count, 256)), dim3(256), 0, stream_of(stream),
                      count, w, g, m1, m2, lr, b1, b2, eps, c1, c2);
+  SM_LAUNCH_CHECK();
+  return SM_OK;
+}
+
+extern "C" int sm_mlp_pack_hidden(int hidden, int width, const float *w, void *packed,
+                                  void *stream) {
+  SM_CHECK(width == kW, SM_ERR_ARG, "mlp_pack_hidden: width must be %d (got %d)", kW, width);
+  SM_CHECK(hidden >= 0 && hidden <= kMaxHidden && (hidden == 0 || (w && packed)), SM_ERR_ARG,
+           "mlp_pack_hidden: bad argument");
+  SM_CHECK(((uintptr_t)packed & 15) == 0, SM_ERR_ARG, "mlp_pack_hidden: output not 16-byte aligned");
+  const int64_t n = (int64_t)hidden * kW * kW / 8;
+  if (n == 0) return SM_OK;
+  hipLaunchKernelGGL(k_pack_hidden, dim3(blocks_for(n, 256)), dim3(256), 0, stream_of(stream),
+                     hidden, w, reinterpret_cast<bf16x8 *>(packed));
   SM_LAUNCH_CHECK();
   return SM_OK;
 }

@@ -137,13 +137,12 @@ class NeuralSDF:
 
     def struct(self):
         """The sm_mlp view of the current weights (hidden weights rounded to
-        BF16 on the device when they changed)."""
+        BF16 and packed in fragment order on the device when they changed)."""
         if self._dirty:
-            off, n = self.layout["w_hid"]
-            if n:
-                _lib.check(_lib.lib.sm_f32_to_bf16(n, self._p("w_hid"),
-                                                   _ptr(self.w_hid_bf16), self._stream()),
-                           "sm_f32_to_bf16")
+            if self.hidden:
+                _lib.check(_lib.lib.sm_mlp_pack_hidden(self.hidden, self.width, self._p("w_hid"),
+                                                       _ptr(self.w_hid_bf16), self._stream()),
+                           "sm_mlp_pack_hidden")
             self._dirty = False
         st = MLPStruct()
         st.in_dim, st.width, st.hidden = self.dim, self.width, self.hidden
