@@ -311,6 +311,13 @@ int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats);
 #define SM_NPHASES 8
 int sm_mesher_profile(sm_mesher *m, int enable);
 int sm_mesher_profile_read(sm_mesher *m, double *ms, int64_t *launches, int nphases);
+/* Average duration (ms) of `reps` back-to-back launches of one phase's
+ * kernel on the handle's current state, timed with HIP events on the
+ * handle's stream: a roofline measurement of the kernel alone (no
+ * counterpart in the reference).  Only phase 7 (the flip-marking kernel of
+ * a 2D handle) is supported; its launches leave nothing that the next flip
+ * round does not recompute. */
+int sm_mesher_time_kernel(sm_mesher *m, int phase, int reps, double *ms);
 /* Device pointers of the state, for zero-copy access from the host side. */
 int sm_mesher_device_points(sm_mesher *m, double **p);
 /* Stream used by the handle (hipStream_t as void*). */

@@ -538,6 +538,40 @@ bool kinetic_repair(sm_mesher *m) {
   return true;
 }
 
+int time_flip_mark(sm_mesher *m, int reps, double *ms) {
+  SM_CHECK(m->dim == 2 && m->have_tri && m->T > 0, SM_ERR_STATE,
+           "time_kernel: needs a 2D handle with a triangulation");
+  SM_CHECK(reps >= 1 && reps <= 100000, SM_ERR_ARG, "time_kernel: bad repetition count");
+  hipStream_t s = m->stream;
+  if (!m->d_tctr) SM_HIP(hipMalloc((void **)&m->d_tctr, 8 * sizeof(int32_t)));
+  // counters of an open round (not finished, round id 0); uncertain
+  // predicates are only counted, so the launches write claims and
+  // candidates, which every flip round recomputes before use
+  int rc = fill<int32_t>(m->d_tctr, 8, 0, s);
+  if (rc) return rc;
+  const int64_t T = m->T;
+  const int B = 256;
+  const unsigned g = blocks_for(T, B);
+  hipEvent_t e0, e1;
+  SM_HIP(hipEventCreate(&e0));
+  SM_HIP(hipEventCreate(&e1));
+  hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, m->d_p,
+                     m->d_claim, m->d_cand, m->d_queue + T, m->d_tctr, 0);
+  SM_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; ++i)
+    hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, m->d_p,
+                       m->d_claim, m->d_cand, m->d_queue + T, m->d_tctr, 0);
+  SM_HIP(hipEventRecord(e1, s));
+  SM_LAUNCH_CHECK();
+  SM_HIP(hipEventSynchronize(e1));
+  float el = 0.0f;
+  SM_HIP(hipEventElapsedTime(&el, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = (double)el / reps;
+  return SM_OK;
+}
+
 int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double *pts,
                         bool exact) {
   SM_CHECK(nrounds >= 1 && nrounds <= kMaxBatchRounds, SM_ERR_ARG,

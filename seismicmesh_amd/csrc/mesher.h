@@ -124,6 +124,8 @@ struct sm_mesher {
   int64_t dnn_cap = 0;
   uint64_t *d_sg = nullptr;
 
+  int32_t *d_tctr = nullptr;  // private counter block of sm_mesher_time_kernel
+
   int64_t stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   int round = 0;
   bool have_tri = false;
@@ -199,6 +201,10 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild);
 int flag_failed_domains(sm_mesher *m, int64_t *bad_points, bool *dup);
 int relocate_inverted(sm_mesher *m, int64_t *moved);
 int host_rebuild2d(sm_mesher *m);
+// average duration of `reps` back-to-back launches of the flip-marking
+// kernel on the current triangulation (timing only; no state changes that
+// the next flip round does not overwrite)
+int time_flip_mark(sm_mesher *m, int reps, double *ms);
 // forces.hip: `sync_edges` reads the edge count back to the host (needed
 // by the exact-order sums); otherwise every launch is sized for the
 // largest possible edge count and reads the count on the device

@@ -446,6 +446,7 @@ extern "C" int sm_mesher_destroy(sm_mesher *m) {
   dfree(m->d_sums);
   dfree(m->d_maxF);
   dfree(m->d_sg);
+  dfree(m->d_tctr);
   if (m->h_ctr) (void)hipHostFree(m->h_ctr);
   if (m->h_status) (void)hipHostFree(m->h_status);
   if (m->h_sums) (void)hipHostFree(m->h_sums);
@@ -777,6 +778,14 @@ extern "C" int sm_mesher_set_surrogate(sm_mesher *m, const sm_mlp *net, double b
   m->graph = nullptr;
   m->graph_T = -1;
   return SM_OK;
+}
+
+extern "C" int sm_mesher_time_kernel(sm_mesher *m, int phase, int reps, double *ms) {
+  SM_CHECK(m && ms, SM_ERR_ARG, "mesher_time_kernel: null argument");
+  SM_CHECK(phase == 7, SM_ERR_ARG, "mesher_time_kernel: only phase 7 (flip marking) is timed");
+  SM_HIP(hipSetDevice(m->device));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  return time_flip_mark(m, reps, ms);
 }
 
 extern "C" int sm_mesher_surrogate_stats(sm_mesher *m, int64_t *skipped, int64_t *exact,

@@ -261,6 +261,14 @@ class Mesher:
                    "sm_mesher_surrogate_stats")
         return {"skipped": a.value, "exact": b.value, "max_err": e.value}
 
+    def time_kernel(self, phase="flip_mark", reps=200):
+        """Average ms of `reps` back-to-back launches of a phase's kernel on
+        the current state (HIP events on the handle's stream)."""
+        ms = ctypes.c_double()
+        _lib.check(_lib.lib.sm_mesher_time_kernel(self.h, self.PHASES.index(phase), int(reps),
+                                                  ctypes.byref(ms)), "sm_mesher_time_kernel")
+        return ms.value
+
     def stats(self):
         s = (ctypes.c_int64 * 9)()
         _lib.check(_lib.lib.sm_mesher_stats(self.h, s, 9), "sm_mesher_stats")
