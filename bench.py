@@ -343,7 +343,7 @@ def main():
     # the dominant kernel alone: back-to-back launches on the final state
     # (HIP events on the loop's stream), so that launch gaps do not count
     reps = 200
-    if best == "flip_mark" and sts is None:
+    if best == "flip_mark":
         avg_ms = m.time_kernel("flip_mark", reps)
         launches = reps
     else:

@@ -26,6 +26,10 @@
 // the triangulation is the Delaunay triangulation of the points; otherwise
 // the host rebuilds it from scratch with exact predicates.
 #include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
 #include <thread>
 
 #include "common.h"
@@ -60,6 +64,35 @@ __global__ void k_batch_begin(int32_t *c0, const int32_t *round_counter) {
   c0[7] = 0;
 }
 
+// Per-domain activity of a batch handle's flip rounds.  act holds four
+// arrays of ndom flags: flipped in a round of parity 0 / 1, then has an
+// inverted or uncertain triangle in a round of parity 0 / 1.  Round r
+// (parity p) marks and applies flips only in domains that flipped in round
+// r - 1 or have an inverted triangle now: a domain without a flip in round
+// r - 1 had no flip candidate then (its best candidate would have won both
+// its triangles, which no other domain shares), and without a change to
+// its triangles or points it has none in round r either.  Null for a
+// single-domain handle (every triangle active).
+struct ActView {
+  const int32_t *tdom = nullptr;
+  int32_t *act = nullptr;
+  int ndom = 0;
+  __device__ bool active(int64_t t, int par) const {
+    if (!act) return true;
+    const int d = tdom[t];
+    return act[(1 - par) * ndom + d] || act[(2 + par) * ndom + d];
+  }
+};
+
+// Opens a batch: every domain counts as having flipped in the round before
+// round 0 (parity 1), and round 0's inverted flags start clear.
+__global__ void k_act_begin(int32_t *act, int ndom) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= ndom) return;
+  act[1 * ndom + d] = 1;
+  act[2 * ndom + d] = 0;
+}
+
 // Neighbour links of triangle s after the flips of round `round`: a link to
 // a flipped triangle that no longer holds the shared edge now belongs to its
 // partner.
@@ -86,7 +119,8 @@ __device__ inline void fix_links(int64_t s, const int32_t *tri, int32_t *adj,
 __global__ void k_fix_orient(int64_t T, const int32_t *tri, int32_t *adj,
                              const int32_t *partner, const int32_t *mark, const double *p,
                              int32_t *ori, uint64_t *claim, int32_t *queue,
-                             const int32_t *prev, int32_t *c, int queue_exact) {
+                             const int32_t *prev, int32_t *c, int queue_exact, ActView av,
+                             int par) {
   int skip = c[7], id = c[6];
   if (prev) {
     skip = prev[7] || prev[0] == 0 || prev[5] > 0;
@@ -97,7 +131,15 @@ __global__ void k_fix_orient(int64_t T, const int32_t *tri, int32_t *adj,
     c[6] = id;
     c[7] = skip;
   }
+  // this round's flip flags start clear (round r - 1 read the other parity)
+  if (av.act && t < av.ndom) av.act[par * av.ndom + t] = 0;
   if (t >= T) return;
+  // a domain without flips and without inverted triangles in round r - 1
+  // is unchanged: no links to fix, orientations and claims stand
+  if (av.act) {
+    const int d = av.tdom[t];
+    if (!av.act[(1 - par) * av.ndom + d] && !av.act[(3 - par) * av.ndom + d]) return;
+  }
   if (prev && !prev[7]) fix_links(t, tri, adj, partner, mark, prev[6]);
   if (skip) return;
   claim[t] = 0ull;
@@ -116,10 +158,14 @@ __global__ void k_fix_orient(int64_t T, const int32_t *tri, int32_t *adj,
       atomicAdd(&c[5], 1);
     }
     ori[t] = 0;  // provisional, settled by k_orient_exact
+    if (av.act) av.act[(2 + par) * av.ndom + av.tdom[t]] = 1;
     return;
   }
   ori[t] = o;
-  if (o <= 0) atomicAdd(&c[1], 1);
+  if (o <= 0) {
+    atomicAdd(&c[1], 1);
+    if (av.act) av.act[(2 + par) * av.ndom + av.tdom[t]] = 1;
+  }
 }
 
 // The link fix of a batch's last round.
@@ -208,11 +254,14 @@ __device__ inline bool near_inverted(const int32_t *adj, const int32_t *ori, int
 
 __global__ void k_mark(int64_t T, const int32_t *tri, const int32_t *adj, const int32_t *ori,
                        const double *p, uint64_t *claim, uint64_t *cand, int32_t *queue,
-                       int32_t *ctr, int queue_exact) {
+                       int32_t *ctr, int queue_exact, ActView av, int par) {
   if (ctr[7]) return;
   const int round = ctr[6];
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= T) return;
+  // an idle domain's claims stay 0 (reset by k_fix_orient), so its stale
+  // candidates never match in k_apply
+  if (!av.active(t, par)) return;
   for (int k = 0; k < 3; ++k) {
     uint64_t prio = 0;
     int64_t o = adj[3 * t + k];
@@ -300,12 +349,15 @@ __global__ void k_mark_exact(const int32_t *tri, const int32_t *adj, const doubl
 
 __global__ void k_apply(int64_t T, int32_t *tri, int32_t *adj, const uint64_t *claim,
                         const uint64_t *cand, int32_t *partner, int32_t *mark, int32_t *ctr,
-                        int32_t *next) {
+                        int32_t *next, ActView av, int par) {
   if (next && blockIdx.x == 0 && threadIdx.x < 6) next[threadIdx.x] = 0;
   if (ctr[7]) return;
   const int round = ctr[6];
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // the next round's inverted flags start clear (nothing reads them before)
+  if (av.act && t < av.ndom) av.act[(3 - par) * av.ndom + t] = 0;
   if (t >= T) return;
+  if (!av.active(t, par)) return;
   for (int k = 0; k < 3; ++k) {
     uint64_t pr = cand[3 * t + k];
     if (!pr) continue;
@@ -333,6 +385,7 @@ __global__ void k_apply(int64_t T, int32_t *tri, int32_t *adj, const uint64_t *c
     mark[t] = round;
     mark[o] = round;
     atomicAdd(&ctr[0], 1);
+    if (av.act) av.act[par * av.ndom + av.tdom[t]] = 1;
     break;
   }
 }
@@ -556,11 +609,11 @@ int time_flip_mark(sm_mesher *m, int reps, double *ms) {
   SM_HIP(hipEventCreate(&e0));
   SM_HIP(hipEventCreate(&e1));
   hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, m->d_p,
-                     m->d_claim, m->d_cand, m->d_queue + T, m->d_tctr, 0);
+                     m->d_claim, m->d_cand, m->d_queue + T, m->d_tctr, 0, ActView(), 0);
   SM_HIP(hipEventRecord(e0, s));
   for (int i = 0; i < reps; ++i)
     hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, m->d_p,
-                       m->d_claim, m->d_cand, m->d_queue + T, m->d_tctr, 0);
+                       m->d_claim, m->d_cand, m->d_queue + T, m->d_tctr, 0, ActView(), 0);
   SM_HIP(hipEventRecord(e1, s));
   SM_LAUNCH_CHECK();
   SM_HIP(hipEventSynchronize(e1));
@@ -583,26 +636,35 @@ int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double
   int32_t *base = m->d_ctr + kCtrRounds;
   int32_t *rc = m->d_ctr + kCtrRoundId;
   const int qx = exact ? 1 : 0;
+  ActView av;
+  if (m->ndom > 0 && m->d_act) {
+    av.tdom = m->d_tdom;
+    av.act = m->d_act;
+    av.ndom = m->ndom;
+    hipLaunchKernelGGL(k_act_begin, dim3(blocks_for(m->ndom, 256)), dim3(256), 0, s, m->d_act,
+                       m->ndom);
+  }
   hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(1), 0, s, base, (const int32_t *)rc);
   for (int r = 0; r < nrounds; ++r) {
     int32_t *c = base + 8 * r;
     const int32_t *prev = r == 0 ? nullptr : c - 8;
     int32_t *next = r + 1 < nrounds ? c + 8 : nullptr;
+    const int par = r % 2;
     hipLaunchKernelGGL(k_fix_orient, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj,
                        m->d_partner, m->d_mark, pts, m->d_ori, m->d_claim, m->d_queue, prev, c,
-                       qx);
+                       qx, av, par);
     if (exact)
       hipLaunchKernelGGL(k_orient_exact, dim3(8), dim3(64), 0, s, m->d_tri, pts, m->d_ori,
                          m->d_queue, c);
     const int pm = prof_begin(m);
     hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, pts,
-                       m->d_claim, m->d_cand, m->d_queue + T, c, qx);
+                       m->d_claim, m->d_cand, m->d_queue + T, c, qx, av, par);
     prof_end(m, 7, pm);
     if (exact)
       hipLaunchKernelGGL(k_mark_exact, dim3(8), dim3(64), 0, s, m->d_tri, m->d_adj, pts,
                          m->d_claim, m->d_cand, m->d_queue + T, c);
     hipLaunchKernelGGL(k_apply, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_claim,
-                       m->d_cand, m->d_partner, m->d_mark, c, next);
+                       m->d_cand, m->d_partner, m->d_mark, c, next, av, par);
     SM_LAUNCH_CHECK();
   }
   const int32_t *lastc = base + 8 * (nrounds - 1);
@@ -612,6 +674,21 @@ int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double
                      status, rc);
   SM_LAUNCH_CHECK();
   return SM_OK;
+}
+
+// SMGPU_TRACE_FLIPS=1 prints the outcome of every batch of repair rounds
+static bool trace_flips() {
+  static const bool on = [] {
+    const char *e = std::getenv("SMGPU_TRACE_FLIPS");
+    return e && *e && *e != '0';
+  }();
+  return on;
+}
+
+static double wall_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
 }
 
 void account_flips(sm_mesher *m, const int32_t *st) {
@@ -626,6 +703,21 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
   int32_t *dstat = m->d_ctr + kCtrStatus;
   bool ok = false, exact_failed = false;
   int best_inverted = INT32_MAX, stale = 0;
+  // A move of less than half an edge length leaves local tangles (a vertex
+  // slid across a flat triangle along a straight boundary, or driven almost
+  // onto a corner) that the host relocation below resolves in one pass
+  // where untangling flips need dozens of rounds: with it available, the
+  // flips get two batches.
+  const double h0m = m->ndom > 0 ? 1.0 : m->prm.h0;
+  const bool large_move = !(m->last_maxdp < 0.5 * h0m);
+  // (single-domain handles only: in a batch the failed domains are rebuilt
+  // on the host in parallel, which costs less than relocation passes that
+  // each need further flip rounds over the whole batch)
+  const auto relocatable = [&]() {
+    return m->ndom == 0 && !large_move && m->have_snap && m->h_ctr[3] > 0 &&
+           m->h_ctr[3] <= 4096;
+  };
+  const double t_start = wall_ms();
   for (int done = 0; done < max_rounds;) {
     const int batch = 4;
     const int pb = prof_begin(m);
@@ -636,6 +728,10 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
     SM_HIP(hipStreamSynchronize(s));
     account_flips(m, m->h_ctr);
     done += m->h_ctr[0];
+    if (trace_flips())
+      std::fprintf(stderr, "flip_repair2d: rounds %d flips %d last-round-idle %d inverted %d "
+                   "exact %d\n", m->h_ctr[0], m->h_ctr[1], m->h_ctr[2], m->h_ctr[3],
+                   m->h_ctr[5]);
     if (m->h_ctr[4] > 0) {  // an exact predicate did not fit: rebuild exactly
       m->stats[7] += m->h_ctr[4];
       exact_failed = true;
@@ -647,9 +743,11 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
       ok = m->h_ctr[3] == 0;
       break;
     }
-    // inverted triangles that stop decreasing will not be untangled
+    if (done >= 8 && relocatable()) break;
+    // inverted triangles that stop decreasing (by at least 5 % per batch)
+    // will not be untangled by more rounds
     if (m->h_ctr[3] > 0) {
-      if (m->h_ctr[3] < best_inverted) {
+      if (m->h_ctr[3] < best_inverted - best_inverted / 20) {
         best_inverted = m->h_ctr[3];
         stale = 0;
       } else if (++stale >= 3) {
@@ -658,20 +756,16 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
     }
   }
   m->rebuild_flagged = false;
-  // A move of less than half an edge length leaves local tangles (a vertex
-  // slid across a flat triangle along a straight boundary, or driven almost
-  // onto a corner): relocate one vertex of each inverted triangle on the
-  // host, then flip again.  Large moves (the first iterations) go to the
-  // kinetic repair below.
-  const double h0m = m->ndom > 0 ? 1.0 : m->prm.h0;
-  const bool large_move = !(m->last_maxdp < 0.5 * h0m);
-  for (int pass = 0; pass < 3 && !ok && !exact_failed && !large_move && m->h_ctr[3] > 0 &&
-                     m->h_ctr[3] <= 4096 && m->have_snap;
-       ++pass) {
+  const double t_flips = wall_ms();
+  // Relocate one vertex of each inverted triangle on the host, then flip
+  // again.  Large moves (the first iterations) go to the kinetic repair
+  // below.
+  for (int pass = 0; pass < 3 && !ok && !exact_failed && relocatable(); ++pass) {
     int64_t moved = 0;
     int rc = relocate_inverted(m, &moved);
     if (rc) return rc;
     if (moved <= 0) break;
+    int best = INT32_MAX, idle = 0;
     for (int done = 0; done < max_rounds;) {
       rc = enqueue_flip_rounds(m, 4, dstat, m->d_p, true);
       if (rc) return rc;
@@ -679,6 +773,10 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
       SM_HIP(hipStreamSynchronize(s));
       account_flips(m, m->h_ctr);
       done += m->h_ctr[0];
+      if (trace_flips())
+        std::fprintf(stderr, "flip_repair2d: after relocation: rounds %d flips %d "
+                     "last-round-idle %d inverted %d\n", m->h_ctr[0], m->h_ctr[1], m->h_ctr[2],
+                     m->h_ctr[3]);
       if (m->h_ctr[4] > 0) {
         exact_failed = true;
         break;
@@ -687,8 +785,20 @@ int flip_repair2d(sm_mesher *m, int max_rounds, bool *need_rebuild) {
         ok = m->h_ctr[3] == 0;
         break;
       }
+      if (m->h_ctr[3] > 0) {
+        if (m->h_ctr[3] < best - best / 20) {
+          best = m->h_ctr[3];
+          idle = 0;
+        } else if (++idle >= 3) {
+          break;
+        }
+      }
     }
   }
+  const double t_reloc = wall_ms();
+  if (trace_flips())
+    std::fprintf(stderr, "flip_repair2d: %.1f ms flips, %.1f ms relocation passes, ok %d\n",
+                 t_flips - t_start, t_reloc - t_flips, (int)ok);
   if (!ok && !exact_failed) {
     // which domains failed, and whether coincident vertices are the cause
     // (flips and sub-steps cannot separate them; the host rebuild drops
@@ -828,15 +938,12 @@ int snapshot_valid(sm_mesher *m) {
 // Batch handles: each domain's triangulated sphere is built on its own
 // vertex range (worker threads over the domains) and the pieces are
 // concatenated with their vertex and triangle ids offset, so no triangle
-// or neighbour link ever joins two domains.
-static int64_t batch_delaunay(sm_mesher *m, const std::vector<double> &hp,
-                              std::vector<int32_t> &tri, std::vector<int32_t> &adj,
-                              const std::vector<uint8_t> *only) {
-  const int nd = m->ndom;
-  std::vector<int> todo;
-  for (int d = 0; d < nd; ++d)
-    if (!only || (*only)[d]) todo.push_back(d);
-  std::vector<int64_t> cnt(nd, 0);
+// or neighbour link ever joins two domains.  Domain d of `todo` is written
+// at slot offset base[k] of tri / adj (its device slots start at
+// h_tbase[d]; ids are device ids).  hp holds the points (2 NA doubles).
+static int64_t batch_delaunay(sm_mesher *m, const double *hp, int32_t *tri, int32_t *adj,
+                              const std::vector<int> &todo, const std::vector<int64_t> &base) {
+  std::vector<int64_t> cnt(todo.size(), 0);
   std::atomic<int> next{0};
   const int64_t N = m->N;
   auto work = [&]() {
@@ -846,13 +953,13 @@ static int64_t batch_delaunay(sm_mesher *m, const std::vector<double> &hp,
       const int d = todo[k];
       const int64_t b = m->h_voff[d], n = m->h_voff[d + 1] - b;
       // the domain's points, then its four far vertices
-      loc.assign(hp.begin() + 2 * b, hp.begin() + 2 * (b + n));
-      loc.insert(loc.end(), hp.begin() + 2 * (N + 4 * d), hp.begin() + 2 * (N + 4 * d + 4));
+      loc.assign(hp + 2 * b, hp + 2 * (b + n));
+      loc.insert(loc.end(), hp + 2 * (N + 4 * d), hp + 2 * (N + 4 * d + 4));
       const int64_t na = n + 4, tb = m->h_tbase[d], cap = m->h_tbase[d + 1] - tb;
-      int32_t *td = tri.data() + 3 * tb, *ad = adj.data() + 3 * tb;
+      int32_t *td = tri + 3 * base[k], *ad = adj + 3 * base[k];
       ins.resize(na);
       int64_t c = sm_host_delaunay2d(loc.data(), na, td, ad, cap, ins.data());
-      cnt[d] = c;
+      cnt[k] = c;
       if (c < 0) continue;
       for (int64_t j = 0; j < 3 * c; ++j) {
         const int32_t v = td[j];
@@ -872,44 +979,74 @@ static int64_t batch_delaunay(sm_mesher *m, const std::vector<double> &hp,
   for (unsigned i = 1; i < nw; ++i) pool.emplace_back(work);
   work();
   for (auto &t : pool) t.join();
-  for (int d : todo)
-    if (cnt[d] < 0) return cnt[d];
-  return m->h_tbase[nd];
+  for (size_t k = 0; k < todo.size(); ++k)
+    if (cnt[k] < 0) return cnt[k];
+  return m->h_tbase[m->ndom];
 }
 
 // Batch handles: rebuild only the domains flagged by flag_failed_domains
-// and upload their slot ranges; the other domains keep their (valid)
-// triangulations.
-static int rebuild_flagged_domains(sm_mesher *m, const std::vector<double> &hp) {
+// from their own points (copied per domain) and upload their slot ranges;
+// the other domains keep their (valid) triangulations.
+static int rebuild_flagged_domains(sm_mesher *m) {
   m->rebuild_flagged = false;
-  std::vector<int32_t> tri((size_t)m->capT * 3), adj((size_t)m->capT * 3);
-  int64_t T = batch_delaunay(m, hp, tri, adj, &m->h_dfail);
-  if (T < 0) return sm_set_error((int)T, "host Delaunay rebuild failed (%lld)", (long long)T);
+  const double t0 = wall_ms();
+  hipStream_t s = m->stream;
+  std::vector<int> todo;
+  std::vector<int64_t> base;
+  int64_t slots = 0;
   for (int d = 0; d < m->ndom; ++d) {
     if (!m->h_dfail[d]) continue;
+    todo.push_back(d);
+    base.push_back(slots);
+    slots += m->h_tbase[d + 1] - m->h_tbase[d];
+  }
+  if (todo.empty()) return snapshot_valid(m);
+  const int64_t NA = m->NA;
+  if ((int64_t)m->h_prel.size() != 4 * NA) m->h_prel.assign(4 * NA, 0.0);
+  double *hp = m->h_prel.data();
+  for (int d : todo) {
+    const int64_t b = m->h_voff[d], n = m->h_voff[d + 1] - b, f = m->N + 4 * d;
+    SM_HIP(hipMemcpyAsync(hp + 2 * b, m->d_p + 2 * b, 2 * n * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    SM_HIP(hipMemcpyAsync(hp + 2 * f, m->d_p + 2 * f, 8 * sizeof(double), hipMemcpyDeviceToHost,
+                          s));
+  }
+  SM_HIP(hipStreamSynchronize(s));
+  std::unique_ptr<int32_t[]> tri(new int32_t[3 * slots]), adj(new int32_t[3 * slots]);
+  int64_t T = batch_delaunay(m, hp, tri.get(), adj.get(), todo, base);
+  if (T < 0) return sm_set_error((int)T, "host Delaunay rebuild failed (%lld)", (long long)T);
+  for (size_t k = 0; k < todo.size(); ++k) {
+    const int d = todo[k];
     const int64_t b = m->h_tbase[d], n = m->h_tbase[d + 1] - b;
-    SM_HIP(hipMemcpyAsync(m->d_tri + 3 * b, tri.data() + 3 * b, 3 * n * sizeof(int32_t),
-                          hipMemcpyHostToDevice, m->stream));
-    SM_HIP(hipMemcpyAsync(m->d_adj + 3 * b, adj.data() + 3 * b, 3 * n * sizeof(int32_t),
-                          hipMemcpyHostToDevice, m->stream));
+    SM_HIP(hipMemcpyAsync(m->d_tri + 3 * b, tri.get() + 3 * base[k], 3 * n * sizeof(int32_t),
+                          hipMemcpyHostToDevice, s));
+    SM_HIP(hipMemcpyAsync(m->d_adj + 3 * b, adj.get() + 3 * base[k], 3 * n * sizeof(int32_t),
+                          hipMemcpyHostToDevice, s));
     m->stats[3] += 1;
   }
-  SM_HIP(hipStreamSynchronize(m->stream));
-  return snapshot_valid(m);
+  SM_HIP(hipStreamSynchronize(s));
+  const double t1 = wall_ms();
+  int rc = snapshot_valid(m);
+  if (trace_flips())
+    std::fprintf(stderr, "rebuild_flagged_domains: %zu domains, %lld slots: %.1f ms rebuild, "
+                 "%.1f ms snapshot\n", todo.size(), (long long)slots, t1 - t0, wall_ms() - t1);
+  return rc;
 }
 
 int host_rebuild2d(sm_mesher *m) {
+  if (m->ndom > 0 && m->rebuild_flagged && m->have_tri) return rebuild_flagged_domains(m);
   const int64_t NA = m->NA;
-  std::vector<double> hp((size_t)NA * 2);
-  SM_HIP(hipMemcpyAsync(hp.data(), m->d_p, NA * 2 * sizeof(double), hipMemcpyDeviceToHost,
-                        m->stream));
+  std::unique_ptr<double[]> hpbuf(new double[2 * NA]);
+  double *hp = hpbuf.get();
+  SM_HIP(hipMemcpyAsync(hp, m->d_p, NA * 2 * sizeof(double), hipMemcpyDeviceToHost, m->stream));
   SM_HIP(hipStreamSynchronize(m->stream));
-  if (m->ndom > 0 && m->rebuild_flagged && m->have_tri) return rebuild_flagged_domains(m, hp);
-  std::vector<int32_t> tri((size_t)m->capT * 3), adj((size_t)m->capT * 3);
-  std::vector<uint8_t> ins((size_t)NA);
+  // uninitialised: every slot up to T is written by the triangulation
+  std::unique_ptr<int32_t[]> trib(new int32_t[3 * m->capT]), adjb(new int32_t[3 * m->capT]);
+  int32_t *tri = trib.get(), *adj = adjb.get();
   int64_t T = 0;
   if (m->ndom == 0) {
-    T = sm_host_delaunay2d(hp.data(), NA, tri.data(), adj.data(), m->capT, ins.data());
+    std::vector<uint8_t> ins((size_t)NA);
+    T = sm_host_delaunay2d(hp, NA, tri, adj, m->capT, ins.data());
   } else {
     // fixed slot ranges: domain d owns 2 (n_d + 4) slots from h_tbase[d]
     if ((int)m->h_tbase.size() != m->ndom + 1) {
@@ -925,13 +1062,19 @@ int host_rebuild2d(sm_mesher *m) {
       SM_HIP(hipMemcpy(m->d_tdom, tdom.data(), tdom.size() * sizeof(int32_t),
                        hipMemcpyHostToDevice));
     }
-    T = batch_delaunay(m, hp, tri, adj, nullptr);
+    std::vector<int> todo(m->ndom);
+    std::vector<int64_t> base(m->ndom);
+    for (int d = 0; d < m->ndom; ++d) {
+      todo[d] = d;
+      base[d] = m->h_tbase[d];
+    }
+    T = batch_delaunay(m, hp, tri, adj, todo, base);
   }
   if (T < 0) return sm_set_error((int)T, "host Delaunay rebuild failed (%lld)", (long long)T);
   m->T = T;
-  SM_HIP(hipMemcpyAsync(m->d_tri, tri.data(), T * 3 * sizeof(int32_t), hipMemcpyHostToDevice,
+  SM_HIP(hipMemcpyAsync(m->d_tri, tri, T * 3 * sizeof(int32_t), hipMemcpyHostToDevice,
                         m->stream));
-  SM_HIP(hipMemcpyAsync(m->d_adj, adj.data(), T * 3 * sizeof(int32_t), hipMemcpyHostToDevice,
+  SM_HIP(hipMemcpyAsync(m->d_adj, adj, T * 3 * sizeof(int32_t), hipMemcpyHostToDevice,
                         m->stream));
   int rc = fill<int32_t>(m->d_mark, m->capT, 0, m->stream);
   if (rc) return rc;

@@ -38,6 +38,8 @@ struct sm_mesher {
   // next host rebuild may be limited to them
   int32_t *d_dfail = nullptr;
   std::vector<uint8_t> h_dfail;
+  // (4 ndom) per-domain activity flags of the flip rounds (flip2d.hip)
+  int32_t *d_act = nullptr;
   bool rebuild_flagged = false;
   std::vector<double> h_prel;  // host points for relocations (p, then pvalid)
 

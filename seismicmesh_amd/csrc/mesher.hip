@@ -421,6 +421,7 @@ extern "C" int sm_mesher_create_batch(const sm_mesher_params *prm, int ndom,
   SM_HIP(hipMalloc(&m->d_dnfix, ndom * sizeof(int32_t)));
   SM_HIP(hipMalloc(&m->d_dh, ndom * sizeof(double)));
   SM_HIP(hipMalloc(&m->d_dscale, ndom * sizeof(double)));
+  SM_HIP(hipMalloc(&m->d_act, 4 * (size_t)ndom * sizeof(int32_t)));
   SM_HIP(hipMemcpy(m->d_voff, v32.data(), (ndom + 1) * sizeof(int32_t), hipMemcpyHostToDevice));
   SM_HIP(hipMemcpy(m->d_dnfix, nfix, ndom * sizeof(int32_t), hipMemcpyHostToDevice));
   SM_HIP(hipMemcpy(m->d_dh, h, ndom * sizeof(double), hipMemcpyHostToDevice));
@@ -439,6 +440,7 @@ extern "C" int sm_mesher_destroy(sm_mesher *m) {
   dfree(m->d_dnfix);
   dfree(m->d_dh);
   dfree(m->d_dscale);
+  dfree(m->d_act);
   dfree(m->d_dfail);
   for (int k = 0; k < 3; ++k) dfree(m->d_axis[k]);
   dfree(m->d_values);
