@@ -325,6 +325,13 @@ def main():
                                                       args.steps, dt))
         return dt, {k: s1[k] - s0[k] for k in s1}
 
+    # the autotuner picks each tunable kernel's workgroup size on this
+    # workload (two iterations in, so edges and forces exist)
+    m.set_points(p0)
+    m.run(2)
+    tuned = m.autotune(reps=20)
+    progress("autotune: %s" % {k: v[0] for k, v in tuned.items()})
+
     plain_elapsed, _ = timed(False)
     elapsed, stats = timed(True)
     n, C, E = m.sizes()
@@ -401,7 +408,8 @@ def main():
                        plain_value=round(N * args.steps / plain_elapsed, 1),
                        plain_path="same kernels launched one by one, host sync per stage",
                        host_rebuilds=stats["host_rebuilds"], flip_rounds=stats["flip_rounds"],
-                       rollbacks=stats["rollbacks"], kinetic_repairs=stats["kinetic_repairs"]),
+                       rollbacks=stats["rollbacks"], kinetic_repairs=stats["kinetic_repairs"],
+                       workgroup_sizes={k: v[0] for k, v in tuned.items()}),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mlp_roofline": mlp,

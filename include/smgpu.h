@@ -318,6 +318,16 @@ int sm_mesher_profile_read(sm_mesher *m, double *ms, int64_t *launches, int npha
  * a 2D handle) is supported; its launches leave nothing that the next flip
  * round does not recompute. */
 int sm_mesher_time_kernel(sm_mesher *m, int phase, int reps, double *ms);
+/* Autotuner (no counterpart in the reference): times the handle's tunable
+ * kernels -- [0] centroid filter, [1] bar forces, [2] gather / update /
+ * projection, [3] flip orientation, [4] flip marking, [5] flip apply (2D
+ * only) -- with workgroups of 64 ... 1024 threads on the current state
+ * (`reps` back-to-back launches each) and keeps the fastest size per
+ * kernel (the default 256 unless another is 3 % faster).  Needs one
+ * iteration run first; leaves the state as the next iteration expects it.
+ * chosen[k] / ms[k] (either may be NULL) receive the sizes and the best
+ * per-launch times for k < nk. */
+int sm_mesher_autotune(sm_mesher *m, int reps, int32_t *chosen, double *ms, int nk);
 /* Device pointers of the state, for zero-copy access from the host side. */
 int sm_mesher_device_points(sm_mesher *m, double **p);
 /* Stream used by the handle (hipStream_t as void*). */

@@ -603,7 +603,7 @@ int time_flip_mark(sm_mesher *m, int reps, double *ms) {
   int rc = fill<int32_t>(m->d_tctr, 8, 0, s);
   if (rc) return rc;
   const int64_t T = m->T;
-  const int B = 256;
+  const int B = m->blk[kBlkMark];
   const unsigned g = blocks_for(T, B);
   hipEvent_t e0, e1;
   SM_HIP(hipEventCreate(&e0));
@@ -625,6 +625,57 @@ int time_flip_mark(sm_mesher *m, int reps, double *ms) {
   return SM_OK;
 }
 
+int time_flip_tunable(sm_mesher *m, int which, int B, int reps, double *ms) {
+  SM_CHECK(which == kBlkOrient || which == kBlkMark || which == kBlkApply, SM_ERR_ARG,
+           "time_flip_tunable: kernel %d", which);
+  SM_CHECK(m->dim == 2 && m->have_tri && m->T > 0, SM_ERR_STATE,
+           "time_flip_tunable: needs a 2D handle with a triangulation");
+  SM_CHECK(B >= 64 && B <= 1024 && B % 64 == 0 && reps >= 1, SM_ERR_ARG,
+           "time_flip_tunable: bad block size or repetition count");
+  hipStream_t s = m->stream;
+  if (!m->d_tctr) SM_HIP(hipMalloc((void **)&m->d_tctr, 8 * sizeof(int32_t)));
+  // a private open round (uncertain predicates only counted): the
+  // orientation pass recomputes orientations and clears every claim, so
+  // the apply pass that follows finds no claimed candidate and flips
+  // nothing; the next flip round recomputes all of it
+  int rc = fill<int32_t>(m->d_tctr, 8, 0, s);
+  if (rc) return rc;
+  const int64_t T = m->T;
+  auto launch = [&](int k, int b) {
+    const unsigned g = blocks_for(T, b);
+    if (k == kBlkOrient)
+      hipLaunchKernelGGL(k_fix_orient, dim3(g), dim3(b), 0, s, T, m->d_tri, m->d_adj,
+                         m->d_partner, m->d_mark, m->d_p, m->d_ori, m->d_claim, m->d_queue,
+                         (const int32_t *)nullptr, m->d_tctr, 0, ActView(), 0);
+    else if (k == kBlkMark)
+      hipLaunchKernelGGL(k_mark, dim3(g), dim3(b), 0, s, T, m->d_tri, m->d_adj, m->d_ori,
+                         m->d_p, m->d_claim, m->d_cand, m->d_queue + T, m->d_tctr, 0, ActView(),
+                         0);
+    else
+      hipLaunchKernelGGL(k_apply, dim3(g), dim3(b), 0, s, T, m->d_tri, m->d_adj, m->d_claim,
+                         m->d_cand, m->d_partner, m->d_mark, m->d_tctr, (int32_t *)nullptr,
+                         ActView(), 0);
+  };
+  launch(kBlkOrient, 256);  // orientations of the current points, claims cleared
+  hipEvent_t e0, e1;
+  SM_HIP(hipEventCreate(&e0));
+  SM_HIP(hipEventCreate(&e1));
+  launch(which, B);
+  SM_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; ++i) launch(which, B);
+  SM_HIP(hipEventRecord(e1, s));
+  if (which == kBlkMark) launch(kBlkOrient, 256);  // clear the claims again
+  SM_LAUNCH_CHECK();
+  SM_HIP(hipEventSynchronize(e1));
+  SM_HIP(hipStreamSynchronize(s));
+  float el = 0.0f;
+  SM_HIP(hipEventElapsedTime(&el, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = (double)el / reps;
+  return SM_OK;
+}
+
 int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double *pts,
                         bool exact) {
   SM_CHECK(nrounds >= 1 && nrounds <= kMaxBatchRounds, SM_ERR_ARG,
@@ -633,6 +684,8 @@ int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double
   const int64_t T = m->T;
   const int B = 256;
   const unsigned g = blocks_for(T, B);
+  const int Bo = m->blk[kBlkOrient], Bm = m->blk[kBlkMark], Ba = m->blk[kBlkApply];
+  const unsigned go = blocks_for(T, Bo), gm = blocks_for(T, Bm), ga = blocks_for(T, Ba);
   int32_t *base = m->d_ctr + kCtrRounds;
   int32_t *rc = m->d_ctr + kCtrRoundId;
   const int qx = exact ? 1 : 0;
@@ -650,20 +703,20 @@ int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double
     const int32_t *prev = r == 0 ? nullptr : c - 8;
     int32_t *next = r + 1 < nrounds ? c + 8 : nullptr;
     const int par = r % 2;
-    hipLaunchKernelGGL(k_fix_orient, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj,
+    hipLaunchKernelGGL(k_fix_orient, dim3(go), dim3(Bo), 0, s, T, m->d_tri, m->d_adj,
                        m->d_partner, m->d_mark, pts, m->d_ori, m->d_claim, m->d_queue, prev, c,
                        qx, av, par);
     if (exact)
       hipLaunchKernelGGL(k_orient_exact, dim3(8), dim3(64), 0, s, m->d_tri, pts, m->d_ori,
                          m->d_queue, c);
     const int pm = prof_begin(m);
-    hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, pts,
+    hipLaunchKernelGGL(k_mark, dim3(gm), dim3(Bm), 0, s, T, m->d_tri, m->d_adj, m->d_ori, pts,
                        m->d_claim, m->d_cand, m->d_queue + T, c, qx, av, par);
     prof_end(m, 7, pm);
     if (exact)
       hipLaunchKernelGGL(k_mark_exact, dim3(8), dim3(64), 0, s, m->d_tri, m->d_adj, pts,
                          m->d_claim, m->d_cand, m->d_queue + T, c);
-    hipLaunchKernelGGL(k_apply, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_claim,
+    hipLaunchKernelGGL(k_apply, dim3(ga), dim3(Ba), 0, s, T, m->d_tri, m->d_adj, m->d_claim,
                        m->d_cand, m->d_partner, m->d_mark, c, next, av, par);
     SM_LAUNCH_CHECK();
   }

@@ -518,7 +518,7 @@ __global__ void k_project_surrogate(int64_t N, double *p, const float *dnn, doub
 
 template <int DIM, int MODE>
 static void launch_keep(sm_mesher *m) {
-  const int B = 256;
+  const int B = m->blk[kBlkKeep];
   hipLaunchKernelGGL((k_keep<DIM, MODE>), dim3(blocks_for(m->T, B)), dim3(B), 0, m->stream,
                      m->T, m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0], m->prm.geps,
                      m->d_keep);
@@ -526,7 +526,7 @@ static void launch_keep(sm_mesher *m) {
 
 template <int DIM, int MODE>
 static void launch_keep_surrogate(sm_mesher *m) {
-  const int B = 256;
+  const int B = m->blk[kBlkKeep];
   hipLaunchKernelGGL((k_keep_surrogate<DIM, MODE>), dim3(blocks_for(m->T, B)), dim3(B), 0,
                      m->stream, m->T, m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0],
                      m->prm.geps, m->d_dnn, m->band, m->d_keep, m->d_sg);
@@ -547,7 +547,7 @@ static int project_surrogate(sm_mesher *m) {
 int centroid_keep(sm_mesher *m) {
   if (m->T == 0) return SM_OK;
   if (m->ndom > 0) {
-    const int B = 256;
+    const int B = m->blk[kBlkKeep];
     if (m->deep_sdf)
       hipLaunchKernelGGL((k_keep_batch<kSdfDeep>), dim3(blocks_for(m->T, B)), dim3(B), 0,
                          m->stream, m->T, m->d_tri, m->d_p, batch_view(m), m->d_keep);
@@ -634,10 +634,36 @@ int build_edges(sm_mesher *m, bool sync_edges) {
 }
 
 template <int DIM, typename T>
+static void launch_edge_force(sm_mesher *m, int B) {
+  const unsigned g = blocks_for(edge_launch_bound(m), B);
+  hipLaunchKernelGGL((k_edge_force<DIM, T>), dim3(g), dim3(B), 0, m->stream,
+                     (const int32_t *)(m->d_ctr + kCtrE), m->d_edges, m->d_p, (const T *)m->d_L,
+                     (const T *)m->d_h, m->d_sums, batch_view(m), m->prm.L0mult, (T *)m->d_F);
+}
+
+template <int DIM, typename T>
+static void launch_vertex_update(sm_mesher *m, int B) {
+  const int64_t N = m->N;
+  const BatchView bv = batch_view(m);
+  if (m->deep_sdf)
+    hipLaunchKernelGGL((k_vertex_update<DIM, T, kSdfDeep>), dim3(blocks_for(N, B)), dim3(B), 0,
+                       m->stream, N, m->d_p, eptr_of(m), m->d_sptr, m->d_slist,
+                       (const T *)m->d_F, m->prm.nfix, m->prm.delta_t, m->d_tape_all,
+                       m->d_tape_off_dev, m->d_tape_len_dev, m->prm.nlevels, m->prm.deps,
+                       m->prm.h0, bv, m->d_maxF, m->surrogate ? 0 : 1);
+  else
+    hipLaunchKernelGGL((k_vertex_update<DIM, T, kSdfReg>), dim3(blocks_for(N, B)), dim3(B), 0,
+                       m->stream, N, m->d_p, eptr_of(m), m->d_sptr, m->d_slist,
+                       (const T *)m->d_F, m->prm.nfix, m->prm.delta_t, m->d_tape_all,
+                       m->d_tape_off_dev, m->d_tape_len_dev, m->prm.nlevels, m->prm.deps,
+                       m->prm.h0, bv, m->d_maxF, m->surrogate ? 0 : 1);
+}
+
+template <int DIM, typename T>
 static int forces_t(sm_mesher *m) {
   hipStream_t s = m->stream;
-  const int B = 256;
-  const int64_t E = m->E, N = m->N;
+  const int B = 256;  // k_edge_geom's block reduction assumes 256 threads
+  const int64_t E = m->E;
   const bool exact = m->prm.precision == 64;
   const unsigned ge = blocks_for(edge_launch_bound(m), B);
   const int32_t *Ep = m->d_ctr + kCtrE;
@@ -687,26 +713,13 @@ static int forces_t(sm_mesher *m) {
   }
   prof_end(m, 4, pb);
   pb = prof_begin(m);
-  hipLaunchKernelGGL((k_edge_force<DIM, T>), dim3(ge), dim3(B), 0, s, Ep, m->d_edges, m->d_p,
-                     (const T *)m->d_L, (const T *)m->d_h, m->d_sums, bv, m->prm.L0mult,
-                     (T *)m->d_F);
+  launch_edge_force<DIM, T>(m, m->blk[kBlkForce]);
   SM_LAUNCH_CHECK();
   prof_end(m, 5, pb);
   int rc = fill<uint64_t>(m->d_maxF, 1, 0ull, s);
   if (rc) return rc;
   pb = prof_begin(m);
-  if (m->deep_sdf)
-    hipLaunchKernelGGL((k_vertex_update<DIM, T, kSdfDeep>), dim3(blocks_for(N, B)), dim3(B), 0,
-                       s, N, m->d_p, eptr_of(m), m->d_sptr, m->d_slist, (const T *)m->d_F,
-                       m->prm.nfix, m->prm.delta_t, m->d_tape_all, m->d_tape_off_dev,
-                       m->d_tape_len_dev, m->prm.nlevels, m->prm.deps, m->prm.h0, bv,
-                       m->d_maxF, m->surrogate ? 0 : 1);
-  else
-    hipLaunchKernelGGL((k_vertex_update<DIM, T, kSdfReg>), dim3(blocks_for(N, B)), dim3(B), 0,
-                       s, N, m->d_p, eptr_of(m), m->d_sptr, m->d_slist, (const T *)m->d_F,
-                       m->prm.nfix, m->prm.delta_t, m->d_tape_all, m->d_tape_off_dev,
-                       m->d_tape_len_dev, m->prm.nlevels, m->prm.deps, m->prm.h0, bv,
-                       m->d_maxF, m->surrogate ? 0 : 1);
+  launch_vertex_update<DIM, T>(m, m->blk[kBlkUpdate]);
   SM_LAUNCH_CHECK();
   if (m->surrogate) {
     rc = m->deep_sdf ? project_surrogate<DIM, kSdfDeep>(m) : project_surrogate<DIM, kSdfReg>(m);
@@ -720,6 +733,72 @@ int forces_and_update(sm_mesher *m) {
   const bool f64 = m->prm.precision == 64;
   if (m->dim == 2) return f64 ? forces_t<2, double>(m) : forces_t<2, float>(m);
   return f64 ? forces_t<3, double>(m) : forces_t<3, float>(m);
+}
+
+template <int DIM, typename T>
+static void launch_tunable(sm_mesher *m, int which, int B) {
+  if (which == kBlkForce) {
+    launch_edge_force<DIM, T>(m, B);
+  } else if (which == kBlkUpdate) {
+    launch_vertex_update<DIM, T>(m, B);
+  } else if (m->ndom > 0) {
+    if (m->deep_sdf)
+      hipLaunchKernelGGL((k_keep_batch<kSdfDeep>), dim3(blocks_for(m->T, B)), dim3(B), 0,
+                         m->stream, m->T, m->d_tri, m->d_p, batch_view(m), m->d_keep);
+    else
+      hipLaunchKernelGGL((k_keep_batch<kSdfReg>), dim3(blocks_for(m->T, B)), dim3(B), 0,
+                         m->stream, m->T, m->d_tri, m->d_p, batch_view(m), m->d_keep);
+  } else {
+    if (m->deep_sdf)
+      hipLaunchKernelGGL((k_keep<DIM, kSdfDeep>), dim3(blocks_for(m->T, B)), dim3(B), 0,
+                         m->stream, m->T, m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0],
+                         m->prm.geps, m->d_keep);
+    else
+      hipLaunchKernelGGL((k_keep<DIM, kSdfReg>), dim3(blocks_for(m->T, B)), dim3(B), 0,
+                         m->stream, m->T, m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0],
+                         m->prm.geps, m->d_keep);
+  }
+}
+
+int time_force_tunable(sm_mesher *m, int which, int B, int reps, double *ms) {
+  SM_CHECK(which == kBlkKeep || which == kBlkForce || which == kBlkUpdate, SM_ERR_ARG,
+           "time_force_tunable: kernel %d", which);
+  SM_CHECK(B >= 64 && B <= 1024 && B % 64 == 0 && reps >= 1, SM_ERR_ARG,
+           "time_force_tunable: bad block size or repetition count");
+  hipStream_t s = m->stream;
+  const bool f64 = m->prm.precision == 64;
+  auto launch = [&]() {
+    if (m->dim == 2) {
+      if (f64) launch_tunable<2, double>(m, which, B);
+      else launch_tunable<2, float>(m, which, B);
+    } else {
+      if (f64) launch_tunable<3, double>(m, which, B);
+      else launch_tunable<3, float>(m, which, B);
+    }
+  };
+  // the update moves the points: time it on a copy of them and restore
+  const size_t pbytes = (size_t)m->NA * m->dim * sizeof(double);
+  if (which == kBlkUpdate)
+    SM_HIP(hipMemcpyAsync(m->d_pstage, m->d_p, pbytes, hipMemcpyDeviceToDevice, s));
+  hipEvent_t e0, e1;
+  SM_HIP(hipEventCreate(&e0));
+  SM_HIP(hipEventCreate(&e1));
+  launch();
+  SM_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; ++i) launch();
+  SM_HIP(hipEventRecord(e1, s));
+  SM_LAUNCH_CHECK();
+  SM_HIP(hipEventSynchronize(e1));
+  float el = 0.0f;
+  SM_HIP(hipEventElapsedTime(&el, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (which == kBlkUpdate) {
+    SM_HIP(hipMemcpyAsync(m->d_p, m->d_pstage, pbytes, hipMemcpyDeviceToDevice, s));
+    SM_HIP(hipStreamSynchronize(s));
+  }
+  *ms = (double)el / reps;
+  return SM_OK;
 }
 
 }  // namespace smk

@@ -127,6 +127,9 @@ struct sm_mesher {
   uint64_t *d_sg = nullptr;
 
   int32_t *d_tctr = nullptr;  // private counter block of sm_mesher_time_kernel
+  // workgroup sizes of the tunable kernels (smk::kBlk*), set by
+  // sm_mesher_autotune; 256 until then
+  int blk[6] = {256, 256, 256, 256, 256, 256};
 
   int64_t stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   int round = 0;
@@ -175,6 +178,11 @@ inline BatchView batch_view(const sm_mesher *m) {
   return b;
 }
 
+// tunable kernels: centroid filter, bar forces, gather/update/projection,
+// and the flip rounds' orientation, marking and apply kernels
+constexpr int kBlkKeep = 0, kBlkForce = 1, kBlkUpdate = 2, kBlkOrient = 3, kBlkMark = 4,
+              kBlkApply = 5, kNumBlk = 6;
+
 // d_ctr layout (int32): [4] unique edge count, [16] last round id,
 // [24..31] status of the last batch of flip rounds, [32..) one 8-int block
 // per flip round of a batch
@@ -207,6 +215,12 @@ int host_rebuild2d(sm_mesher *m);
 // kernel on the current triangulation (timing only; no state changes that
 // the next flip round does not overwrite)
 int time_flip_mark(sm_mesher *m, int reps, double *ms);
+// average ms of `reps` launches of tunable kernel `which` (kBlk*) with
+// workgroup size B on the current state; the state is left as the next
+// iteration expects it (forces.hip: keep / force / update, flip2d.hip:
+// orient / mark / apply)
+int time_force_tunable(sm_mesher *m, int which, int B, int reps, double *ms);
+int time_flip_tunable(sm_mesher *m, int which, int B, int reps, double *ms);
 // forces.hip: `sync_edges` reads the edge count back to the host (needed
 // by the exact-order sums); otherwise every launch is sized for the
 // largest possible edge count and reads the count on the device

@@ -790,6 +790,49 @@ extern "C" int sm_mesher_time_kernel(sm_mesher *m, int phase, int reps, double *
   return time_flip_mark(m, reps, ms);
 }
 
+extern "C" int sm_mesher_autotune(sm_mesher *m, int reps, int32_t *chosen, double *ms,
+                                  int nk) {
+  SM_CHECK(m && reps >= 1, SM_ERR_ARG, "mesher_autotune: bad argument");
+  SM_CHECK(m->have_tri && m->stats[0] > 0, SM_ERR_STATE,
+           "mesher_autotune: run at least one iteration first (edges and forces are timed "
+           "on the current state)");
+  SM_HIP(hipSetDevice(m->device));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  static const int cand[] = {64, 128, 256, 512, 1024};
+  for (int k = 0; k < kNumBlk; ++k) {
+    const bool flip = k == kBlkOrient || k == kBlkMark || k == kBlkApply;
+    if (flip && m->dim != 2) continue;
+    int best_b = 256;
+    double best = 0.0, t256 = 0.0;
+    int rc = flip ? time_flip_tunable(m, k, 256, reps, &t256)
+                  : time_force_tunable(m, k, 256, reps, &t256);
+    if (rc) return rc;
+    best = t256;
+    for (int b : cand) {
+      if (b == 256) continue;
+      double t = 0.0;
+      rc = flip ? time_flip_tunable(m, k, b, reps, &t) : time_force_tunable(m, k, b, reps, &t);
+      if (rc) return rc;
+      // another size must beat the default by 3 % (timing noise)
+      if (t < best && t < 0.97 * t256) {
+        best = t;
+        best_b = b;
+      }
+    }
+    m->blk[k] = best_b;
+    if (ms && k < nk) ms[k] = best;
+  }
+  if (chosen)
+    for (int k = 0; k < kNumBlk && k < nk; ++k) chosen[k] = m->blk[k];
+  // the captured iteration used the old sizes
+  if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->gexec = nullptr;
+  m->graph = nullptr;
+  m->graph_T = -1;
+  return SM_OK;
+}
+
 extern "C" int sm_mesher_surrogate_stats(sm_mesher *m, int64_t *skipped, int64_t *exact,
                                          double *max_err) {
   SM_CHECK(m && skipped && exact && max_err, SM_ERR_ARG, "mesher_surrogate_stats: null argument");

@@ -261,6 +261,20 @@ class Mesher:
                    "sm_mesher_surrogate_stats")
         return {"skipped": a.value, "exact": b.value, "max_err": e.value}
 
+    TUNABLE = ("centroid_filter", "edge_force", "vertex_update", "flip_orient", "flip_mark",
+               "flip_apply")
+
+    def autotune(self, reps=20):
+        """Pick the workgroup size of each tunable kernel by timing 64..1024
+        threads on the current state (run one iteration first).  Returns
+        {kernel: (threads, ms per launch)}."""
+        k = len(self.TUNABLE)
+        chosen = (ctypes.c_int32 * k)()
+        ms = (ctypes.c_double * k)()
+        _lib.check(_lib.lib.sm_mesher_autotune(self.h, int(reps), chosen, ms, k),
+                   "sm_mesher_autotune")
+        return {name: (int(chosen[i]), float(ms[i])) for i, name in enumerate(self.TUNABLE)}
+
     def time_kernel(self, phase="flip_mark", reps=200):
         """Average ms of `reps` back-to-back launches of a phase's kernel on
         the current state (HIP events on the handle's stream)."""

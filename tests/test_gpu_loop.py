@@ -254,3 +254,27 @@ def test_domain_extension_known_answer(style, answer):
                                           domain_pad=1e3)
     (p, t), _ = _counts(sm.generate_mesh, sm.Rectangle(bbox), ef, h0=100, perform_checks=True)
     assert abs(len(p) - answer[0]) <= 100 and abs(len(t) - answer[1]) <= 100
+
+
+def test_autotune_keeps_results_identical():
+    """The autotuner only changes workgroup sizes: after tuning, further
+    iterations give the same points as an untuned handle."""
+    from seismicmesh_amd import generation, workloads
+    domain, ef, _ = workloads.bp2004(scale=0.25)
+    st = generation.prepare(domain, ef, verbose=0, precision=32)
+    a, b = generation.make_mesher(st), generation.make_mesher(st)
+    a.set_points(st["p"])
+    b.set_points(st["p"])
+    a.run(2)
+    b.run(2)
+    tuned = b.autotune(reps=5)
+    print("autotune:", tuned)
+    assert set(tuned) == set(b.TUNABLE)
+    assert all(t in (64, 128, 256, 512, 1024) and ms > 0 for t, ms in tuned.values())
+    assert np.array_equal(a.points(), b.points())
+    for _ in range(5):
+        a.run(1)
+        b.run(1)
+        assert np.array_equal(a.points(), b.points())
+    a.close()
+    b.close()
