@@ -175,12 +175,15 @@ int sm_mlp_make_inputs(int64_t n, int dim, const double *x, const double *shift,
                        const double *scale, float eik_h, float *X, void *stream);
 /* Loss of the surrogate and its gradient with respect to the network
  * outputs y (same row blocks as sm_mlp_make_inputs):
- *   L = mean_i (y_i - t_i)^2 + lam * mean_i (|g_i| - 1)^2,
+ *   L = mean_i (y_i - t_i)^2 + lam * mean_i' (|g_i| - 1)^2,
  *   g_i,k = (y[(2k+1) n + i] - y[(2k+2) n + i]) / (2 eik_h)
- * (a central-difference eikonal term; lam = 0 drops it).  part[b] receives
- * the loss partial of block b of 256 samples (ceil(n / 256) floats). */
+ * (a central-difference eikonal term; lam = 0 drops it).  With trunc > 0
+ * the targets are a truncated SDF and samples with |t_i| >= trunc have no
+ * eikonal term (i' runs over the others; the mean still divides by n).
+ * part[b] receives the loss partial of block b of 256 samples
+ * (ceil(n / 256) floats). */
 int sm_mlp_sdf_loss(int64_t n, int dim, const float *y, const float *t, float eik_h, float lam,
-                    float *grad, float *part, void *stream);
+                    float trunc, float *grad, float *part, void *stream);
 /* Adam (Kingma & Ba): m1 = b1 m1 + (1-b1) g, m2 = b2 m2 + (1-b2) g^2,
  * w -= lr (m1 / (1 - b1^step)) / (sqrt(m2 / (1 - b2^step)) + eps). */
 int sm_mlp_adam(int64_t count, float *w, const float *g, float *m1, float *m2, float lr,

@@ -131,7 +131,7 @@ _PROTOS = {
     "sm_mlp_bias_silu": (_int, [_i64, _int, _vp, _vp, _vp, _vp]),
     "sm_mlp_silu_backward": (_int, [_i64, _vp, _vp, _vp, _vp]),
     "sm_mlp_make_inputs": (_int, [_i64, _int, _vp, _vp, _vp, _flt, _vp, _vp]),
-    "sm_mlp_sdf_loss": (_int, [_i64, _int, _vp, _vp, _flt, _flt, _vp, _vp, _vp]),
+    "sm_mlp_sdf_loss": (_int, [_i64, _int, _vp, _vp, _flt, _flt, _flt, _vp, _vp, _vp]),
     "sm_mlp_adam": (_int, [_i64, _vp, _vp, _vp, _vp, _flt, _flt, _flt, _flt, _int, _vp]),
     "sm_f32_to_bf16": (_int, [_i64, _vp, _vp, _vp]),
     "sm_mlp_pack_hidden": (_int, [_int, _int, _vp, _vp, _vp]),

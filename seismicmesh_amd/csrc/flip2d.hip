@@ -427,24 +427,26 @@ __global__ void k_domain_fail(int64_t T, const int32_t *tri, const double *p,
   const int32_t a = tri[3 * t], b = tri[3 * t + 1], c = tri[3 * t + 2];
   if (a == kDeadVertex) return;
   const int last = status[6];
-  bool bad = last > 0 && mark[t] == last;
+  // reason bits: 1 flipped by the last round, 2 inverted or degenerate,
+  // 4 two coincident vertices
+  int why = (last > 0 && mark[t] == last) ? 1 : 0;
   if (a >= 0 && b >= 0 && c >= 0) {
     const double *pa = PT(p, a), *pb = PT(p, b), *pc = PT(p, c);
     const bool same = (pa[0] == pb[0] && pa[1] == pb[1]) || (pb[0] == pc[0] && pb[1] == pc[1]) ||
                       (pa[0] == pc[0] && pa[1] == pc[1]);
     if (same) {
       atomicOr(dup, 1);
-      bad = true;
+      why |= 4;
     } else {
       int o = smpred::orient2d_filter(pa, pb, pc);
       if (o == SM_PRED_UNRESOLVED) o = smpred::orient2d_if_exact(pa, pb, pc);
       if (o == SM_PRED_UNRESOLVED) o = smpred::orient2d_exact(pa, pb, pc);
-      bad |= o <= 0;
+      if (o <= 0) why |= 2;
     }
   }
-  if (!bad) return;
+  if (!why) return;
   const int32_t v = a >= 0 ? a : (b >= 0 ? b : c);
-  dflag[vdom ? vdom[v] : 0] = 1;
+  atomicOr(&dflag[vdom ? vdom[v] : 0], why);
 }
 
 }  // namespace
@@ -966,9 +968,25 @@ int flag_failed_domains(sm_mesher *m, int64_t *bad_points, bool *dup) {
   SM_HIP(hipStreamSynchronize(s));
   *dup = fl[nd] != 0;
   *bad_points = 0;
+  int nwhy[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   for (int d = 0; d < nd; ++d) {
     m->h_dfail[d] = fl[d] != 0;
     if (fl[d]) *bad_points += m->ndom > 0 ? m->h_voff[d + 1] - m->h_voff[d] : m->N;
+    nwhy[fl[d] & 7] += 1;
+  }
+  if (trace_flips()) {
+    std::fprintf(stderr, "failed domains by reason (1 flipping, 2 inverted, 4 coincident):");
+    for (int k = 1; k < 8; ++k)
+      if (nwhy[k]) std::fprintf(stderr, " [%d]=%d", k, nwhy[k]);
+    std::fprintf(stderr, "; first:");
+    int shown = 0;
+    for (int d = 0; d < nd && shown < 12; ++d)
+      if (fl[d]) {
+        std::fprintf(stderr, " %d(n=%lld,why=%d)", d,
+                     (long long)(m->ndom > 0 ? m->h_voff[d + 1] - m->h_voff[d] : m->N), fl[d]);
+        ++shown;
+      }
+    std::fprintf(stderr, "\n");
   }
   return SM_OK;
 }

@@ -65,53 +65,68 @@ __device__ inline float silu_fast(float z) {
 
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
-template <int IN, bool CELLS>
-__global__ __launch_bounds__(256) void k_mlp_eval(sm_mlp net, const double *x,
-                                                  const int32_t *cells, int64_t n, float *out) {
-  __shared__ __align__(16) __bf16 hs[kWaves][32][kRow];
+// PT point tiles of 32 per wave (WAVES waves per workgroup): every weight
+// fragment loaded into registers feeds PT matrix products, so PT = 2 halves
+// the weight stream per FLOP at the price of one wave per SIMD (256
+// accumulator registers, 33.8 KB of LDS per wave); the launch uses PT = 1.
+template <int IN, bool CELLS, int PT, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_mlp_eval(sm_mlp net, const double *x,
+                                                         const int32_t *cells, int64_t n,
+                                                         float *out) {
+  __shared__ __align__(16) __bf16 hs[WAVES][32 * PT][kRow];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int r = lane % 32, half = lane / 32;
-  const int64_t pi = ((int64_t)blockIdx.x * kWaves + wave) * 32 + r;
-  bool valid = pi < n;
-  float xin[IN];
-  {
-    double q[3] = {0.0, 0.0, 0.0};
-    if (valid) valid = mlp_input<IN, CELLS>(x, cells, pi, q);
-    if (!valid && pi < n && half == 0) out[pi] = INFINITY;
-#pragma unroll
-    for (int c = 0; c < IN; ++c)
-      xin[c] = valid ? (float)((q[c] - net.shift[c]) * net.scale[c]) : 0.0f;
-  }
+  const int64_t p0 = ((int64_t)blockIdx.x * WAVES + wave) * (32 * PT);
+  bool valid[PT];
   // input layer: this lane computes features [128 half, 128 half + 128) of
-  // its point, stored 8 at a time
-  for (int j0 = 0; j0 < kW / 2; j0 += 8) {
-    bf16x8 v;
+  // its point r of every tile, stored 8 at a time
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int f = half * (kW / 2) + j0 + j;
-      float z = net.b_in[f];
+  for (int pt = 0; pt < PT; ++pt) {
+    const int64_t pi = p0 + 32 * pt + r;
+    float xin[IN];
+    {
+      double q[3] = {0.0, 0.0, 0.0};
+      valid[pt] = pi < n;
+      if (valid[pt]) valid[pt] = mlp_input<IN, CELLS>(x, cells, pi, q);
+      if (!valid[pt] && pi < n && half == 0) out[pi] = INFINITY;
 #pragma unroll
-      for (int c = 0; c < IN; ++c) z = fmaf(net.w_in[f * IN + c], xin[c], z);
-      v[j] = (__bf16)silu_fast(z);
+      for (int c = 0; c < IN; ++c)
+        xin[c] = valid[pt] ? (float)((q[c] - net.shift[c]) * net.scale[c]) : 0.0f;
     }
-    *reinterpret_cast<bf16x8 *>(&hs[wave][r][half * (kW / 2) + j0]) = v;
+    for (int j0 = 0; j0 < kW / 2; j0 += 8) {
+      bf16x8 v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int f = half * (kW / 2) + j0 + j;
+        float z = net.b_in[f];
+#pragma unroll
+        for (int c = 0; c < IN; ++c) z = fmaf(net.w_in[f * IN + c], xin[c], z);
+        v[j] = (__bf16)silu_fast(z);
+      }
+      *reinterpret_cast<bf16x8 *>(&hs[wave][32 * pt + r][half * (kW / 2) + j0]) = v;
+    }
   }
   __syncthreads();
   // Hidden layers as C^T = W A^T: the A operand is a weight fragment (32
   // features x 16 k: lane holds feature 32 nb + r, k = 8 half .. 8 half + 7,
   // packed by k_pack_hidden), the B operand the activations (16 k x 32
-  // points: lane holds point r, same k), and the accumulator gives lane r
-  // the features 32 nb + 8 (i / 4) + 4 half + i % 4 (i = 0..15) of point r,
-  // four consecutive features per group -- one 8-byte LDS store each.
+  // points: lane holds point r of a tile, same k), and the accumulator
+  // gives lane r the features 32 nb + 8 (i / 4) + 4 half + i % 4
+  // (i = 0..15) of point r -- four consecutive features per group, one
+  // 8-byte LDS store each.
   const bf16x8 *whid = reinterpret_cast<const bf16x8 *>(net.w_hid);
-  float s = 0.0f;
+  float s[PT];
+#pragma unroll
+  for (int pt = 0; pt < PT; ++pt) s[pt] = 0.0f;
   for (int l = 0; l < net.hidden; ++l) {
     const bf16x8 *W = whid + (size_t)l * (kW * kW / 8) + lane;
-    floatx16 acc[8];
+    floatx16 acc[PT][8];
 #pragma unroll
-    for (int nb = 0; nb < 8; ++nb)
+    for (int pt = 0; pt < PT; ++pt)
 #pragma unroll
-      for (int i = 0; i < 16; ++i) acc[nb][i] = 0.0f;
+      for (int nb = 0; nb < 8; ++nb)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[pt][nb][i] = 0.0f;
     // the next k-step's weight fragments are loaded before this step's products
     bf16x8 wc[8], wn[8];
 #pragma unroll
@@ -121,10 +136,14 @@ __global__ __launch_bounds__(256) void k_mlp_eval(sm_mlp net, const double *x,
 #pragma unroll
         for (int nb = 0; nb < 8; ++nb) wn[nb] = W[((ks + 1) * 8 + nb) * 64];
       }
-      const bf16x8 a = *reinterpret_cast<const bf16x8 *>(&hs[wave][r][16 * ks + 8 * half]);
 #pragma unroll
-      for (int nb = 0; nb < 8; ++nb)
-        acc[nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wc[nb], a, acc[nb], 0, 0, 0);
+      for (int pt = 0; pt < PT; ++pt) {
+        const bf16x8 a =
+            *reinterpret_cast<const bf16x8 *>(&hs[wave][32 * pt + r][16 * ks + 8 * half]);
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb)
+          acc[pt][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wc[nb], a, acc[pt][nb], 0, 0, 0);
+      }
 #pragma unroll
       for (int nb = 0; nb < 8; ++nb) wc[nb] = wn[nb];
     }
@@ -138,28 +157,35 @@ __global__ __launch_bounds__(256) void k_mlp_eval(sm_mlp net, const double *x,
         const int f0 = 32 * nb + 8 * g + 4 * half;
         const float4 b4 = *reinterpret_cast<const float4 *>(bias + f0);
         const float bb[4] = {b4.x, b4.y, b4.z, b4.w};
-        bf16x4 v;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) v[e] = (__bf16)silu_fast(acc[nb][4 * g + e] + bb[e]);
-        if (last) {
-          // output layer straight from the registers (FP32 weights over the
-          // BF16-rounded activations)
+        for (int pt = 0; pt < PT; ++pt) {
+          bf16x4 v;
 #pragma unroll
-          for (int e = 0; e < 4; ++e) s = fmaf((float)v[e], net.w_out[f0 + e], s);
-        } else {
-          *reinterpret_cast<bf16x4 *>(&hs[wave][r][f0]) = v;
+          for (int e = 0; e < 4; ++e) v[e] = (__bf16)silu_fast(acc[pt][nb][4 * g + e] + bb[e]);
+          if (last) {
+            // output layer straight from the registers (FP32 weights over
+            // the BF16-rounded activations)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) s[pt] = fmaf((float)v[e], net.w_out[f0 + e], s[pt]);
+          } else {
+            *reinterpret_cast<bf16x4 *>(&hs[wave][32 * pt + r][f0]) = v;
+          }
         }
       }
     if (!last) __syncthreads();
   }
-  if (net.hidden == 0) {
-    for (int j = 0; j < kW / 2; ++j) {
-      const int f = half * (kW / 2) + j;
-      s = fmaf((float)hs[wave][r][f], net.w_out[f], s);
+#pragma unroll
+  for (int pt = 0; pt < PT; ++pt) {
+    if (net.hidden == 0) {
+      for (int j = 0; j < kW / 2; ++j) {
+        const int f = half * (kW / 2) + j;
+        s[pt] = fmaf((float)hs[wave][32 * pt + r][f], net.w_out[f], s[pt]);
+      }
     }
+    float t = s[pt] + __shfl_xor(s[pt], 32);
+    const int64_t pi = p0 + 32 * pt + r;
+    if (half == 0 && valid[pt]) out[pi] = (float)(net.out_scale * (double)(t + net.b_out));
   }
-  s += __shfl_xor(s, 32);
-  if (half == 0 && valid) out[pi] = (float)(net.out_scale * (double)(s + net.b_out));
 }
 
 // 128 x 128 output tile per workgroup, k in slices of 8 staged in LDS,
@@ -283,7 +309,7 @@ __global__ void k_make_inputs(int64_t n, int dim, const double *x, double3 shift
 }
 
 __global__ void k_sdf_loss(int64_t n, int dim, const float *y, const float *t, float eik_h,
-                           float lam, float *grad, float *part) {
+                           float lam, float trunc, float *grad, float *part) {
   __shared__ float red[256];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float e = 0.0f;
@@ -292,7 +318,15 @@ __global__ void k_sdf_loss(int64_t n, int dim, const float *y, const float *t, f
     const float dd = y[i] - t[i];
     grad[i] = 2.0f * dd * inv_n;
     e = dd * dd * inv_n;
-    if (lam > 0.0f) {
+    // a truncated target (|t| = trunc) has no unit gradient to match
+    const bool eik = lam > 0.0f && !(trunc > 0.0f && fabsf(t[i]) >= trunc);
+    if (lam > 0.0f && !eik) {
+      for (int k = 0; k < dim; ++k) {
+        grad[(2 * k + 1) * n + i] = 0.0f;
+        grad[(2 * k + 2) * n + i] = 0.0f;
+      }
+    }
+    if (eik) {
       float g[3], g2 = 0.0f;
       for (int k = 0; k < dim; ++k) {
         g[k] = (y[(2 * k + 1) * n + i] - y[(2 * k + 2) * n + i]) / (2.0f * eik_h);
@@ -355,13 +389,16 @@ template <bool CELLS>
 int launch_eval(const sm_mlp *net, const double *x, const int32_t *cells, int64_t n, float *d,
                 hipStream_t s) {
   if (n == 0) return SM_OK;
+  // one point tile per wave, 4 waves per workgroup (2 tiles per wave
+  // measured slower: 476 against 514 TFLOP/s at 4M points, one wave per
+  // SIMD cannot hide the weight loads)
   const unsigned g = blocks_for(n, 32 * kWaves);
   if (net->in_dim == 2)
-    hipLaunchKernelGGL((k_mlp_eval<2, CELLS>), dim3(g), dim3(64 * kWaves), 0, s, *net, x, cells,
-                       n, d);
+    hipLaunchKernelGGL((k_mlp_eval<2, CELLS, 1, kWaves>), dim3(g), dim3(64 * kWaves), 0, s,
+                       *net, x, cells, n, d);
   else
-    hipLaunchKernelGGL((k_mlp_eval<3, CELLS>), dim3(g), dim3(64 * kWaves), 0, s, *net, x, cells,
-                       n, d);
+    hipLaunchKernelGGL((k_mlp_eval<3, CELLS, 1, kWaves>), dim3(g), dim3(64 * kWaves), 0, s,
+                       *net, x, cells, n, d);
   SM_LAUNCH_CHECK();
   return SM_OK;
 }
@@ -473,12 +510,13 @@ extern "C" int sm_mlp_make_inputs(int64_t n, int dim, const double *x, const dou
 }
 
 extern "C" int sm_mlp_sdf_loss(int64_t n, int dim, const float *y, const float *t, float eik_h,
-                               float lam, float *grad, float *part, void *stream) {
+                               float lam, float trunc, float *grad, float *part, void *stream) {
   SM_CHECK(n > 0 && (dim == 2 || dim == 3) && y && t && grad && part, SM_ERR_ARG,
            "mlp_sdf_loss: bad argument");
   SM_CHECK(lam == 0.0f || eik_h > 0.0f, SM_ERR_ARG, "mlp_sdf_loss: eikonal term needs eik_h > 0");
+  SM_CHECK(trunc >= 0.0f, SM_ERR_ARG, "mlp_sdf_loss: negative truncation");
   hipLaunchKernelGGL(k_sdf_loss, dim3(blocks_for(n, 256)), dim3(256), 0, stream_of(stream), n,
-                     dim, y, t, eik_h, lam, grad, part);
+                     dim, y, t, eik_h, lam, trunc, grad, part);
   SM_LAUNCH_CHECK();
   return SM_OK;
 }

@@ -110,6 +110,7 @@ class NeuralSDF:
         self.w_hid_bf16 = torch.empty(max(self.hidden * self.width * self.width, 8),
                                       dtype=torch.bfloat16, device=self.dev)
         self.step = 0
+        self.trunc = None
         self._bufs = {}
         self._dirty = True
         self._shift_d = torch.tensor(np.r_[self.shift, 0.0][:3], dtype=torch.float64,
@@ -217,10 +218,12 @@ class NeuralSDF:
                                         nsplit, work if work is not None else None,
                                         self._stream()), "sm_gemm_f32")
 
-    def loss_and_grad(self, xt, tt, eik=0.1, eik_h=5e-3):
+    def loss_and_grad(self, xt, tt, eik=0.1, eik_h=5e-3, trunc=0.0):
         """Forward and backward pass over the samples xt ((n, dim) float64
         device tensor) with targets tt (float32, distances in the network's
-        frame, i.e. d * s).  Fills self.grad; returns the loss."""
+        frame, i.e. d * s; truncated at +-trunc when trunc > 0, where the
+        eikonal term is dropped).  Fills self.grad; returns the loss
+        partials (device)."""
         n = xt.shape[0]
         W, L, dim = self.width, self.hidden, self.dim
         b = self._buffers(n, eik > 0)
@@ -247,7 +250,8 @@ class NeuralSDF:
         _lib.check(_lib.lib.sm_mlp_bias_silu(rows, 1, _ptr(b["y"]), self._p("b_out"), None, st),
                    "sm_mlp_bias_silu")
         _lib.check(_lib.lib.sm_mlp_sdf_loss(n, dim, _ptr(b["y"]), _ptr(tt), float(eik_h),
-                                            float(eik), _ptr(b["gy"]), _ptr(b["part"]), st),
+                                            float(eik), float(trunc), _ptr(b["gy"]),
+                                            _ptr(b["part"]), st),
                    "sm_mlp_sdf_loss")
         g = self.grad
         gy = _ptr(b["gy"])
@@ -306,9 +310,12 @@ class NeuralSDF:
         return x, domain.eval(x)
 
     def train(self, domain, nsamples=200000, steps=1500, batch=8192, lr=1e-3, lr_final=5e-5,
-              eik=0.1, eik_h=5e-3, near=None, seed=0, log_every=0, samples=None):
-        """Fit the exact SDF of `domain`; returns the loss every `log_every`
-        steps (all on the GPU; the loss is read back only when logged)."""
+              eik=0.1, eik_h=5e-3, near=None, seed=0, log_every=0, samples=None, trunc=None):
+        """Fit the exact SDF of `domain` (truncated at +-trunc when given:
+        the network then only has to be accurate within trunc of the
+        surface, which is all the exact-fallback band needs); returns the
+        loss every `log_every` steps (all on the GPU; the loss is read back
+        only when logged)."""
         torch = self.torch
         if near is None:
             near = 0.05 * float((self.bbox[:, 1] - self.bbox[:, 0]).max()) / 2.0
@@ -317,7 +324,12 @@ class NeuralSDF:
         else:
             x, d = samples
         x = np.ascontiguousarray(x, dtype=np.float64)
-        tn = (np.asarray(d) * self.scale[0]).astype(np.float32)
+        self.trunc = float(trunc) if trunc else None
+        d = np.asarray(d, dtype=np.float64)
+        if self.trunc:
+            d = np.clip(d, -self.trunc, self.trunc)
+        tn = (d * self.scale[0]).astype(np.float32)
+        trunc_n = self.trunc * self.scale[0] if self.trunc else 0.0
         n = len(x)
         batch = min(batch, n)
         rng = np.random.default_rng(seed + 1)
@@ -332,7 +344,7 @@ class NeuralSDF:
                 pos = 0
             xb, tb = xs[pos:pos + batch], ts[pos:pos + batch]
             pos += batch
-            part = self.loss_and_grad(xb, tb, eik=eik, eik_h=eik_h)
+            part = self.loss_and_grad(xb, tb, eik=eik, eik_h=eik_h, trunc=trunc_n)
             frac = it / max(steps - 1, 1)
             rate = lr * (lr_final / lr) ** frac
             self.adam_step(rate)
@@ -344,14 +356,24 @@ class NeuralSDF:
     def calibrate_band(self, domain, band, nval=100000, safety=2.0, seed=12345):
         """The exact-fallback band to use with this network: at least `band`
         (e.g. 2 h) and at least `safety` times the largest |d_nn - d| over
-        nval uniform points of the box (judged by the exact program).
-        Returns (band, max error)."""
+        nval uniform points of the box (judged by the exact program; both
+        truncated for a network trained on a truncated SDF, whose band must
+        stay below the truncation).  Returns (band, max error)."""
         rng = np.random.default_rng(seed)
         x = rng.uniform(self.bbox[:, 0], self.bbox[:, 1], (nval, self.dim))
         err = self.max_error(domain, x)
-        return max(float(band), safety * err), err
+        out = max(float(band), safety * err)
+        if self.trunc and out >= self.trunc:
+            raise ValueError("the band %.4g reaches the truncation %.4g of the training "
+                             "targets: train longer or truncate wider" % (out, self.trunc))
+        return out, err
 
     def max_error(self, domain, x):
-        """max |d_nn - d| over points x, judged by the exact program."""
+        """max |d_nn - d| over points x, judged by the exact program (both
+        clipped to +-trunc for a network trained on a truncated SDF)."""
         dn = self.eval(x).astype(np.float64)
-        return float(np.abs(dn - domain.eval(x)).max())
+        d = domain.eval(x)
+        if self.trunc:
+            dn = np.clip(dn, -self.trunc, self.trunc)
+            d = np.clip(d, -self.trunc, self.trunc)
+        return float(np.abs(dn - d).max())

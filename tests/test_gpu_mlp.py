@@ -57,17 +57,19 @@ def _small_net(dim=3, seed=0):
 
 def test_training_gradient_matches_oracle():
     from oracle import mlp as omlp
-    for dim, lam in ((3, 0.1), (2, 0.0)):
+    for dim, lam, trunc in ((3, 0.1, 0.0), (2, 0.0, 0.0), (3, 0.1, 0.25)):
         net = _small_net(dim, seed=1)
         rng = np.random.default_rng(2)
         n = 300
         x = rng.uniform(-1, 1, (n, dim))
         t = (np.linalg.norm(x, axis=1) - 0.5).astype(np.float32)
-        part = net.loss_and_grad(_dev(x), _dev(t), eik=lam, eik_h=5e-3)
+        if trunc:
+            t = np.clip(t, -trunc, trunc).astype(np.float32)
+        part = net.loss_and_grad(_dev(x), _dev(t), eik=lam, eik_h=5e-3, trunc=trunc)
         g = net.grad.cpu().numpy().astype(np.float64)
         p = net.numpy_params().astype(np.float64)
         L, gref = omlp.loss_and_grad(p, dim, 256, 3, x, t.astype(np.float64), net.shift,
-                                     net.scale, lam=lam, eik_h=5e-3)
+                                     net.scale, lam=lam, eik_h=5e-3, trunc=trunc)
         assert abs(float(part.cpu().numpy().astype(np.float64).sum()) - L) <= 1e-4 * abs(L)
         for name, (o, k) in net.layout.items():
             ref = gref[o:o + k]
@@ -187,5 +189,38 @@ def test_surrogate_loop_matches_exact_and_skips_work():
           % (err, band, 2 * h, s["skipped"], s["exact"], 100 * frac, s["max_err"]))
     assert frac > 0.3
     assert s["max_err"] < band
+    a.close()
+    b.close()
+
+
+def test_surrogate_loop_3d_matches_exact():
+    """3D: the surrogate-steered iterations (tetrahedra centroids and moved
+    points) give the exact path's points on a small CSG."""
+    import seismicmesh_amd as sm
+    from seismicmesh_amd import generation
+    from seismicmesh_amd.neural import NeuralSDF
+    dom = sm.Difference([sm.Union([sm.Ball([-0.3, 0.0, 0.0], 0.55),
+                                   sm.Cube((0.0, 0.8, -0.35, 0.35, -0.35, 0.35))]),
+                         sm.Ball([0.45, 0.1, 0.0], 0.2)])
+    net = NeuralSDF(3, (-1.0, 1.0, -1.0, 1.0, -1.0, 1.0), hidden=3, seed=0, omega=8.0)
+    # a truncated SDF: the band only needs the network near the surface
+    net.train(dom, nsamples=200000, steps=2000, batch=4096, lr=2e-3, lr_final=1e-4, eik=0.1,
+              near=0.05, seed=0, trunc=0.4)
+    h = 0.08
+    band, err = net.calibrate_band(dom, 2.0 * h, nval=100000)
+    st = generation.prepare(dom, h, verbose=0, precision=32)
+    a, b = generation.make_mesher(st), generation.make_mesher(st)
+    b.set_surrogate(net, band)
+    a.set_points(st["p"])
+    b.set_points(st["p"])
+    for _ in range(4):
+        a.run(1)
+        b.run(1)
+        assert np.array_equal(a.points(), b.points())
+    s = b.surrogate_stats()
+    frac = s["skipped"] / (s["skipped"] + s["exact"])
+    print("3D surrogate: max error %.4f, band %.4f: %.1f%% of exact evaluations skipped"
+          % (err, band, 100 * frac))
+    assert frac > 0.1
     a.close()
     b.close()

@@ -18,15 +18,17 @@ def _case(dim=3, width=8, hidden=2, n=5, seed=0):
 
 
 def test_gradient_against_central_differences():
-    for dim, lam in ((2, 0.0), (3, 0.1)):
+    for dim, lam, trunc in ((2, 0.0, 0.0), (3, 0.1, 0.0), (3, 0.1, 0.2)):
         p, x, t, sh, sc = _case(dim=dim)
-        L, g = mlp.loss_and_grad(p, dim, 8, 2, x, t, sh, sc, lam=lam, eik_h=1e-2)
+        if trunc:
+            t = np.clip(t, -trunc, trunc)  # some targets sit at the truncation
+        L, g = mlp.loss_and_grad(p, dim, 8, 2, x, t, sh, sc, lam=lam, eik_h=1e-2, trunc=trunc)
         fd = np.empty_like(p)
         for i in range(len(p)):
             e = np.zeros_like(p)
             e[i] = 1e-6
-            fd[i] = (mlp.loss(p + e, dim, 8, 2, x, t, sh, sc, lam, 1e-2)
-                     - mlp.loss(p - e, dim, 8, 2, x, t, sh, sc, lam, 1e-2)) / 2e-6
+            fd[i] = (mlp.loss(p + e, dim, 8, 2, x, t, sh, sc, lam, 1e-2, trunc)
+                     - mlp.loss(p - e, dim, 8, 2, x, t, sh, sc, lam, 1e-2, trunc)) / 2e-6
         assert np.abs(fd - g).max() <= 1e-7 * max(1.0, np.abs(g).max())
 
 
