@@ -62,26 +62,28 @@ def load_traffic(kernel, workload):
         return None
 
 
-def alg_bytes(phase, N, E, C, T):
-    """Algorithmic HBM bytes of one launch (2D, FP64 points, FP32 per-edge
-    arrays, int32 ids).  Each datum is counted once: gathers of point
-    coordinates count 16 B per referenced vertex."""
+def alg_bytes(phase, N, E, C, T, dim=2):
+    """Algorithmic HBM bytes of one launch (FP64 points, FP32 per-edge
+    arrays and sizing grid, int32 ids).  Each datum is counted once: gathers
+    of point coordinates count 8 dim B per referenced vertex."""
+    pt = 8 * dim
     if phase == "edge_geom":
-        # edge ids 8 + both endpoints 32 + 4 sizing taps 16 + L, h out 8
-        return E * (8 + 32 + 16 + 8)
+        # edge ids 8 + both endpoints + 2^dim sizing taps + L, h out 8
+        return E * (8 + 2 * pt + 4 * 2 ** dim + 8)
     if phase == "edge_force":
-        # edge ids 8 + L, h 8 + both endpoints 32 + force vector out 8
-        return E * (8 + 8 + 32 + 8)
+        # edge ids 8 + L, h 8 + both endpoints + force vector out
+        return E * (8 + 8 + 2 * pt + 4 * dim)
     if phase == "vertex_update":
-        # offsets 16 + point in/out 32 per vertex; per edge its force vector
-        # read from both endpoints 16 and the second-endpoint list entry 4
-        return N * (16 + 32) + E * (16 + 4)
+        # offsets 16 + point in/out per vertex; per edge its force vector
+        # read from both endpoints and the second-endpoint list entry 4
+        return N * (16 + 2 * pt) + E * (2 * 4 * dim + 4)
     if phase == "unique_edges":
-        # cells read twice 24 + 3 pairs written/read 24 per cell, edges out 8
-        return C * (24 + 24) + E * 8 + N * 16
+        # cells read twice + their pairs written/read, edges out 8
+        pairs = 3 if dim == 2 else 6
+        return C * (2 * 4 * (dim + 1) + 8 * pairs) + E * 8 + N * 16
     if phase == "centroid_filter":
-        # triangle ids 12 + keep flag 1 per triangle, points 16 per vertex
-        return T * (12 + 1) + N * 16
+        # cell ids + keep flag 1 per cell, points per vertex
+        return T * (4 * (dim + 1) + 1) + N * pt
     if phase == "flip_mark":
         # per triangle: ids 12, neighbours 12, orientation 4, three 64-bit
         # candidates 24, its 64-bit claim 8; points 16 per vertex
@@ -102,8 +104,10 @@ def cpu_baseline(st, p0, iters, reps=3):
     from oracle.sizing import build_interpolant
 
     dom = st["dom"]
-    if hasattr(dom, "bbox0") and type(dom).__name__ == "Rectangle":
+    if type(dom).__name__ == "Rectangle":
         odom = og.Rectangle(dom.bbox0)
+    elif type(dom).__name__ == "Cube":
+        odom = og.Cube(dom.bbox0)
     else:
         odom = og.Disk([dom.xc, dom.yc], dom.r)
     fh = st["fh"]
@@ -113,7 +117,9 @@ def cpu_baseline(st, p0, iters, reps=3):
     else:
         val = fh.value
         ofh = lambda x: np.array([val] * len(x))  # noqa: E731
-    s = dict(fd=odom.eval, fh=ofh, h0=st["h0"], dim=2, L0mult=1.2, geps=st["geps"],
+    dim = st["dim"]
+    s = dict(fd=odom.eval, fh=ofh, h0=st["h0"], dim=dim, L0mult=1 + 0.4 / 2 ** (dim - 1),
+             geps=st["geps"],
              deps=st["deps"], pfix=p0[:st["nfix"]], levels=[odom.eval], delta_t=st["delta_t"])
     old_aff = os.sched_getaffinity(0)
     core = min(old_aff)
@@ -244,7 +250,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=49)
     ap.add_argument("--warmup", type=int, default=49)
-    ap.add_argument("--workload", default="bp2004", choices=["bp2004", "disk", "many"])
+    ap.add_argument("--workload", default="bp2004", choices=["bp2004", "disk", "many", "eage"])
     ap.add_argument("--domains", type=int, default=512,
                     help="number of domains of the many-domains workload")
     ap.add_argument("--scale", type=float, default=1.0)
@@ -291,6 +297,11 @@ def main():
             domain, ef, info = workloads.bp2004(scale=args.scale)
             wl = {"workload": "bp2004_2d", "grid": list(info["grid"]),
                   "bbox_m": list(info["bbox"]), "hmin_m": 75.0, "grade": 0.15, "max_iter": 50}
+        elif args.workload == "eage":
+            domain, ef, info = workloads.eage3d(scale=args.scale)
+            wl = {"workload": "eage_3d", "grid": list(info["grid"]),
+                  "bbox_m": list(info["bbox"]), "hmin_m": 150.0, "grade": 0.15,
+                  "max_iter": 50, "triangulation": "host exact Bowyer-Watson per iteration"}
         else:
             domain, ef, info = workloads.disk()
             wl = {"workload": "unit_disk_h0.02", "max_iter": 50}
@@ -330,10 +341,16 @@ def main():
     m.set_points(p0)
     m.run(2)
     tuned = m.autotune(reps=20)
+    dim = m.dim
     progress("autotune: %s" % {k: v[0] for k, v in tuned.items()})
 
-    plain_elapsed, _ = timed(False)
-    elapsed, stats = timed(True)
+    if dim == 3:
+        # 3D iterations have no captured-graph path: one timed pass
+        elapsed, stats = timed(False)
+        plain_elapsed = elapsed
+    else:
+        plain_elapsed, _ = timed(False)
+        elapsed, stats = timed(True)
     n, C, E = m.sizes()
     value = N * args.steps / elapsed
 
@@ -344,7 +361,7 @@ def main():
     prof = m.profile_read()
     m.profile(False)
     progress("event-profiled loop done")
-    T = 2 * N
+    T = 2 * N if dim == 2 else C
     best = max(SINGLE_KERNEL, key=lambda k: prof[k][0])
     ms_tot, launches = prof[best]
     # the dominant kernel alone: back-to-back launches on the final state
@@ -355,7 +372,7 @@ def main():
         launches = reps
     else:
         avg_ms = ms_tot / max(launches, 1)
-    traffic_alg = alg_bytes(best, N, E, C, T)
+    traffic_alg = alg_bytes(best, N, E, C, T, dim)
     achieved = traffic_alg / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     kernel = SINGLE_KERNEL[best]
     others = sum(v[0] for k, v in prof.items() if k != "flip_mark")
@@ -378,13 +395,14 @@ def main():
                          "pinned to core %d of %d (%s), median of 3 runs of %.1f s"
                          % (args.cpu_iters, nsamp, core, ncpu, model, secs)}
     elif not args.no_cpu:
-        rate, secs, core = cpu_baseline(st, p0, args.cpu_iters)
+        iters, reps = (args.cpu_iters, 3) if dim == 2 else (1, 1)
+        rate, secs, core = cpu_baseline(st, p0, iters, reps=reps)
         model, ncpu = cpu_info()
         cpu = {"value": round(rate, 1), "unit": "vertex-updates/s", "cores": 1, "kind": "port",
-               "sample": "%d force iterations of the same workload (N=%d) with the numpy "
+               "sample": "%d force iteration(s) of the same workload (N=%d) with the numpy "
                          "oracle + scipy Qhull (stand-in for CGAL), one thread pinned to core "
-                         "%d of %d (%s), median of 3 runs of %.1f s"
-                         % (args.cpu_iters, N, core, ncpu, model, secs)}
+                         "%d of %d (%s), median of %d run(s) of %.1f s"
+                         % (iters, N, core, ncpu, model, reps, secs)}
 
     if cpu is not None:
         progress("cpu baseline done")
@@ -404,7 +422,9 @@ def main():
         "dtype": "f32" if args.precision == 32 else "f64",
         "data": "synthetic",
         "config": dict(wl, N=N, edges=E, cells=C,
-                       path="device flips; one captured graph launch per iteration",
+                       path=("device flips; one captured graph launch per iteration"
+                             if dim == 2 else "host tetrahedralisation, device filter, edges, "
+                             "forces, update and projection"),
                        plain_value=round(N * args.steps / plain_elapsed, 1),
                        plain_path="same kernels launched one by one, host sync per stage",
                        host_rebuilds=stats["host_rebuilds"], flip_rounds=stats["flip_rounds"],

@@ -8,8 +8,10 @@
 // lexicographic sort is a counting sort on min (one bucket per vertex: a
 // histogram, an exclusive scan and a scatter) followed by a sort of each
 // bucket's max ids.  Buckets hold a vertex's higher neighbours (a handful in
-// a mesh), sorted in registers; long buckets (arbitrary user input) are
-// sorted by a whole workgroup in LDS or, beyond that, in global memory.
+// a 2D mesh: sorted in one thread's registers; a few dozen in 3D, where an
+// edge comes once per incident cell: one wave, one id per lane); long
+// buckets (arbitrary user input) are sorted by a whole workgroup in LDS
+// or, beyond that, in global memory.
 // Duplicates are dropped inside each sorted bucket; a second scan turns the
 // per-bucket unique counts into output offsets.  The output is exactly the
 // reference's sorted unique edge list.
@@ -85,6 +87,7 @@ __global__ void k_scatter(PairSrc src, const int32_t *rowptr, int32_t *cur, int3
 }
 
 constexpr int kRegSort = 16;    // buckets up to this size: per-thread sort
+constexpr int kWaveSort = 64;   // up to this size: one wave, one id per lane
 constexpr int kLdsSort = 4096;  // up to this size: one workgroup, LDS bitonic
 
 // Sorts 16 register values with a bitonic network: every index is a
@@ -109,18 +112,25 @@ __device__ inline void sort16(int32_t (&v)[16]) {
   }
 }
 
-// Per-bucket sort + unique count for short buckets (one thread each); long
-// buckets are flagged for the workgroup pass.
+// Per-bucket sort + unique count for short buckets (one thread each);
+// medium buckets (a 3D mesh's: each edge comes once per incident cell) are
+// listed for the one-wave pass, long ones for the workgroup pass.
 __global__ void k_row_sort(int64_t nrows, const int32_t *rowptr, int32_t *col, int32_t *ucnt,
-                           int32_t *long_rows, int32_t *nlong) {
+                           int32_t *long_rows, int32_t *nlong, int32_t *mid_rows,
+                           int32_t *nmid) {
   int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   int32_t s = rowptr[r], e = rowptr[r + 1];
   int32_t len = e - s;
   if (len > kRegSort) {
-    int32_t slot = atomicAdd(nlong, 1);
-    long_rows[slot] = (int32_t)r;
-    ucnt[r] = 0;  // filled by the long-bucket pass
+    if (len <= kWaveSort) {
+      int32_t slot = atomicAdd(nmid, 1);
+      mid_rows[slot] = (int32_t)r;
+    } else {
+      int32_t slot = atomicAdd(nlong, 1);
+      long_rows[slot] = (int32_t)r;
+    }
+    ucnt[r] = 0;  // filled by the wave or workgroup pass
     return;
   }
   int32_t v[16];
@@ -136,6 +146,38 @@ __global__ void k_row_sort(int64_t nrows, const int32_t *rowptr, int32_t *col, i
     }
   }
   ucnt[r] = u;
+}
+
+// Medium buckets (kRegSort < len <= 64), one wave each: lane k holds id k
+// (INT32_MAX past the end), a bitonic network over the 64 lanes exchanges
+// partners with lane shuffles, and a ballot counts the distinct ids.
+__global__ void k_wave_sort(const int32_t *mid_rows, const int32_t *nmid, const int32_t *rowptr,
+                            int32_t *col, int32_t *ucnt) {
+  const int lane = threadIdx.x % 64;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / 64);
+  const int32_t n = *nmid;
+  for (int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; wv < n; wv += nw) {
+    const int32_t r = mid_rows[wv];
+    const int32_t s = rowptr[r], len = rowptr[r + 1] - s;
+    int32_t v = lane < len ? col[s + lane] : INT32_MAX;
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        const int32_t o = __shfl_xor(v, stride);
+        const bool up = (lane & size) == 0;
+        const bool low = (lane & stride) == 0;
+        // the lower lane of a pair keeps the smaller value in an ascending
+        // block, the larger in a descending one
+        v = (low == up) ? (v < o ? v : o) : (v < o ? o : v);
+      }
+    }
+    const int32_t prev = __shfl_up(v, 1);
+    const bool first = lane < len && (lane == 0 || v != prev);
+    const uint64_t m = __ballot(first);
+    if (lane < len) col[s + lane] = v;
+    if (lane == 0) ucnt[r] = (int32_t)__popcll(m);
+  }
 }
 
 // Long buckets, one workgroup each: a bitonic sort in LDS (padded with
@@ -221,7 +263,8 @@ size_t unique_edges_workspace(int64_t npairs, int64_t nvert) {
   b += al(npairs * 4);       // col
   b += al((nvert + 1) * 4);  // ucnt -> eptr
   b += al((nvert + 1) * 4);  // long rows
-  b += al(4 * 4);            // counters
+  b += al((nvert + 1) * 4);  // medium rows
+  b += al(8 * 4);            // counters
   b += al((size_t)npairs * 2 * 4 + 4096 * 4);  // staging for very long buckets
   b += 2 * scan_workspace(nvert + 1);
   return b;
@@ -239,7 +282,8 @@ int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *
   int32_t *col = reinterpret_cast<int32_t *>(w); w += al(npairs * 4);
   int32_t *eptr = reinterpret_cast<int32_t *>(w); w += al((nvert + 1) * 4);
   int32_t *longr = reinterpret_cast<int32_t *>(w); w += al((nvert + 1) * 4);
-  int32_t *ctr = reinterpret_cast<int32_t *>(w); w += al(4 * 4);
+  int32_t *midr = reinterpret_cast<int32_t *>(w); w += al((nvert + 1) * 4);
+  int32_t *ctr = reinterpret_cast<int32_t *>(w); w += al(8 * 4);
   int32_t *stage = reinterpret_cast<int32_t *>(w);
   w += al((size_t)npairs * 2 * 4 + 4096 * 4);
   void *sws = w;
@@ -247,7 +291,7 @@ int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *
   int rc;
   if ((rc = fill<int32_t>(rowptr, nvert + 1, 0, s))) return rc;
   if ((rc = fill<int32_t>(cur, nvert + 1, 0, s))) return rc;
-  if ((rc = fill<int32_t>(ctr, 4, 0, s))) return rc;
+  if ((rc = fill<int32_t>(ctr, 8, 0, s))) return rc;
   PairSrc src{pairs, cells, keep, dim, n};
   const int B = 256;
   if (n > 0) {
@@ -261,7 +305,12 @@ int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *
   }
   if (nvert > 0) {
     hipLaunchKernelGGL(k_row_sort, dim3(blocks_for(nvert, B)), dim3(B), 0, s, nvert, rowptr,
-                       col, eptr, longr, ctr + 2);
+                       col, eptr, longr, ctr + 2, midr, ctr + 4);
+    SM_LAUNCH_CHECK();
+    // one wave per medium bucket (grid-stride over at most nvert of them)
+    unsigned gw = blocks_for(nvert * 64, B);
+    if (gw > 8192) gw = 8192;
+    hipLaunchKernelGGL(k_wave_sort, dim3(gw), dim3(B), 0, s, midr, ctr + 4, rowptr, col, eptr);
     SM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_long_sort, dim3(64), dim3(256), 0, s, longr, ctr + 2, rowptr, col, eptr,
                        stage, 0);

@@ -149,3 +149,49 @@ def csg3d(seed=3, nballs=48, nboxes=16, ncyl=8, ntori=4):
     bbox = (-1.0, 1.0, -1.0, 1.0, -1.0, 1.0)
     body.bbox = bbox
     return body, bbox
+
+
+EAGE_BBOX = (-4200.0, 0.0, 0.0, 13520.0, 0.0, 13520.0)
+EAGE_SHAPE = (210, 676, 676)
+
+
+def eage3d_velocity(shape=EAGE_SHAPE, bbox=EAGE_BBOX, seed=4):
+    """EAGE-shaped 3D velocity model on a (z, x, y) grid: layered
+    1500 -> 4500 m/s with depth (linear over the box) plus one seeded
+    ellipsoidal salt dome at 4480 m/s.  FP32, z-major."""
+    nz, nx, ny = shape
+    depth = -np.linspace(bbox[0], bbox[1], nz)
+    x = np.linspace(bbox[2], bbox[3], nx)
+    y = np.linspace(bbox[4], bbox[5], ny)
+    zmax = bbox[1] - bbox[0]
+    v = (1500.0 + 3000.0 * depth / zmax).astype(np.float32)
+    vp = np.repeat(v[:, None], nx, axis=1)[:, :, None].repeat(ny, axis=2)
+    rng = np.random.default_rng(seed)
+    cx, cy = rng.uniform(0.3, 0.7, 2) * (bbox[3] - bbox[2])
+    cz = rng.uniform(0.4, 0.7) * zmax
+    ax, ay = rng.uniform(2000.0, 3500.0, 2)
+    az = rng.uniform(800.0, 1500.0)
+    dz = ((depth - cz) / az) ** 2
+    dx = ((x - cx) / ax) ** 2
+    dy = ((y - cy) / ay) ** 2
+    inside = dz[:, None, None] + dx[None, :, None] + dy[None, None, :] <= 1.0
+    vp[inside] = 4480.0
+    return vp
+
+
+def eage3d(scale=1.0, seed=4, quiet=True):
+    """Returns (domain, edge_length, info) for the EAGE-shaped 3D workload:
+    sizing v / (2 Hz * 8 cells per wavelength), hmin = 150 m, grade 0.15,
+    on the Cube domain of the box.  ``scale`` < 1 coarsens the grid (same
+    physical box and sizes)."""
+    shape = tuple(max(8, int(round(s * scale))) for s in EAGE_SHAPE)
+    vp = eage3d_velocity(shape, EAGE_BBOX, seed)
+    kw = dict(velocity_data=vp, hmin=150.0, hmax=1e4, wl=8, freq=2.0, grade=0.15)
+    if quiet:
+        import contextlib
+        import io
+        with contextlib.redirect_stdout(io.StringIO()):
+            ef = get_sizing_function_from_segy(None, EAGE_BBOX, **kw)
+    else:
+        ef = get_sizing_function_from_segy(None, EAGE_BBOX, **kw)
+    return geo.Cube(EAGE_BBOX), ef, {"grid": shape, "bbox": EAGE_BBOX, "hmin": 150.0}
