@@ -259,6 +259,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mlp", action="store_true",
                     help="skip the surrogate inference measurement")
+    ap.add_argument("--workgroups", default=None, metavar="N,N,N,N,N,N",
+                    help="fixed workgroup sizes (filter, force, update, orient, mark, apply) "
+                         "instead of the autotuner's choice, e.g. to profile a tuned run")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device-to-device once (a kernel of known "
                          "traffic for scaling hardware byte counters)")
@@ -334,22 +337,43 @@ def main():
         s1 = m.stats()
         progress("%s loop: %d iterations in %.3f s" % ("graph" if graph else "plain",
                                                       args.steps, dt))
-        return dt, {k: s1[k] - s0[k] for k in s1}
+        d = {k: s1[k] - s0[k] for k in s1}
+        d["min_quality"] = min_quality()
+        return dt, d
+
+    def min_quality():
+        """Smallest cell quality of the loop's mesh after the timed
+        iterations (outside the timed region): 2D radius ratio, 3D minimum
+        dihedral angle in degrees."""
+        from seismicmesh_amd import mesh_utils as mu
+        m.triangulate()
+        p, t = m.points(), m.cells()
+        if sts is not None:
+            return round(min(float(mu.simp_qual(q, c).min()) for q, c in m.split(p, t)
+                             if len(c)), 4)
+        if m.dim == 2:
+            return round(float(mu.simp_qual(p, t).min()), 4)
+        return round(float(np.degrees(mu.dihedral_angles(p, t)).min()), 3)
 
     # the autotuner picks each tunable kernel's workgroup size on this
     # workload (two iterations in, so edges and forces exist)
     m.set_points(p0)
     m.run(2)
-    tuned = m.autotune(reps=20)
+    if args.workgroups:
+        sizes = [int(v) for v in args.workgroups.split(",")]
+        m.set_workgroups(sizes)
+        tuned = {k: (v, None) for k, v in zip(m.TUNABLE, sizes)}
+    else:
+        tuned = m.autotune(reps=20)
     dim = m.dim
     progress("autotune: %s" % {k: v[0] for k, v in tuned.items()})
 
     if dim == 3:
         # 3D iterations have no captured-graph path: one timed pass
         elapsed, stats = timed(False)
-        plain_elapsed = elapsed
+        plain_elapsed, plain_stats = elapsed, stats
     else:
-        plain_elapsed, _ = timed(False)
+        plain_elapsed, plain_stats = timed(False)
         elapsed, stats = timed(True)
     n, C, E = m.sizes()
     value = N * args.steps / elapsed
@@ -429,7 +453,11 @@ def main():
                        plain_path="same kernels launched one by one, host sync per stage",
                        host_rebuilds=stats["host_rebuilds"], flip_rounds=stats["flip_rounds"],
                        rollbacks=stats["rollbacks"], kinetic_repairs=stats["kinetic_repairs"],
-                       workgroup_sizes={k: v[0] for k, v in tuned.items()}),
+                       workgroup_sizes={k: v[0] for k, v in tuned.items()},
+                       min_quality=stats["min_quality"],
+                       plain_min_quality=plain_stats["min_quality"],
+                       quality_measure=("radius ratio (2D simp_qual)" if dim == 2
+                                        else "minimum dihedral angle, degrees")),
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mlp_roofline": mlp,

@@ -331,6 +331,9 @@ int sm_mesher_time_kernel(sm_mesher *m, int phase, int reps, double *ms);
  * chosen[k] / ms[k] (either may be NULL) receive the sizes and the best
  * per-launch times for k < nk. */
 int sm_mesher_autotune(sm_mesher *m, int reps, int32_t *chosen, double *ms, int nk);
+/* Sets the six workgroup sizes (same order; multiples of 64 in [64, 1024])
+ * directly, e.g. to reproduce a tuned configuration. */
+int sm_mesher_set_workgroups(sm_mesher *m, const int32_t *sizes, int nk);
 /* Device pointers of the state, for zero-copy access from the host side. */
 int sm_mesher_device_points(sm_mesher *m, double **p);
 /* Stream used by the handle (hipStream_t as void*). */

@@ -117,6 +117,7 @@ _PROTOS = {
     "sm_mesher_profile_read": (_int, [_vp, _vp, _vp, _int]),
     "sm_mesher_time_kernel": (_int, [_vp, _int, _int, ctypes.POINTER(_dbl)]),
     "sm_mesher_autotune": (_int, [_vp, _int, _vp, _vp, _int]),
+    "sm_mesher_set_workgroups": (_int, [_vp, _vp, _int]),
     "sm_mesher_device_points": (_int, [_vp, ctypes.POINTER(_vp)]),
     "sm_mesher_stream": (_int, [_vp, ctypes.POINTER(_vp)]),
     "sm_mesher_find_slivers": (_int, [_vp, _dbl, _dbl, ctypes.POINTER(_i64)]),

@@ -833,6 +833,22 @@ extern "C" int sm_mesher_autotune(sm_mesher *m, int reps, int32_t *chosen, doubl
   return SM_OK;
 }
 
+extern "C" int sm_mesher_set_workgroups(sm_mesher *m, const int32_t *sizes, int nk) {
+  SM_CHECK(m && sizes && nk == kNumBlk, SM_ERR_ARG,
+           "mesher_set_workgroups: need %d sizes", kNumBlk);
+  for (int k = 0; k < kNumBlk; ++k)
+    SM_CHECK(sizes[k] >= 64 && sizes[k] <= 1024 && sizes[k] % 64 == 0, SM_ERR_ARG,
+             "mesher_set_workgroups: size %d of kernel %d is not a multiple of 64 in "
+             "[64, 1024]", sizes[k], k);
+  for (int k = 0; k < kNumBlk; ++k) m->blk[k] = sizes[k];
+  if (m->gexec) (void)hipGraphExecDestroy(m->gexec);
+  if (m->graph) (void)hipGraphDestroy(m->graph);
+  m->gexec = nullptr;
+  m->graph = nullptr;
+  m->graph_T = -1;
+  return SM_OK;
+}
+
 extern "C" int sm_mesher_surrogate_stats(sm_mesher *m, int64_t *skipped, int64_t *exact,
                                          double *max_err) {
   SM_CHECK(m && skipped && exact && max_err, SM_ERR_ARG, "mesher_surrogate_stats: null argument");

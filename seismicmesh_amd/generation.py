@@ -275,6 +275,15 @@ class Mesher:
                    "sm_mesher_autotune")
         return {name: (int(chosen[i]), float(ms[i])) for i, name in enumerate(self.TUNABLE)}
 
+    def set_workgroups(self, sizes):
+        """Set the tunable kernels' workgroup sizes ({kernel: threads} for
+        the names in TUNABLE, or a sequence in that order)."""
+        if isinstance(sizes, dict):
+            sizes = [int(sizes[k]) for k in self.TUNABLE]
+        arr = (ctypes.c_int32 * len(self.TUNABLE))(*[int(v) for v in sizes])
+        _lib.check(_lib.lib.sm_mesher_set_workgroups(self.h, arr, len(self.TUNABLE)),
+                   "sm_mesher_set_workgroups")
+
     def time_kernel(self, phase="flip_mark", reps=200):
         """Average ms of `reps` back-to-back launches of a phase's kernel on
         the current state (HIP events on the handle's stream)."""

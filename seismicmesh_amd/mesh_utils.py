@@ -14,6 +14,7 @@ __all__ = [
     "get_boundary_edges", "get_boundary_facets", "get_boundary_vertices",
     "vertex_to_entities", "get_boundary_entities", "delete_boundary_entities",
     "laplacian2_fixed_point", "get_centroids", "linter", "do_any_overlap", "is_manifold",
+    "dihedral_angles",
 ]
 
 
@@ -291,3 +292,25 @@ def linter(p, t, dim=2, min_qual=0.10):
           + " elements in the mesh", flush=True)
     print("The minimum element quality is " + str(np.amin(qual)), flush=True)
     return p, t
+
+
+def dihedral_angles(p, t):
+    """(ncells, 6) dihedral angles in radians of tetrahedra t of points p,
+    computed on the GPU (sm_dihedral_angles, the reference's
+    fast_geometry.cpp:351-452 edge table)."""
+    import torch
+
+    from . import _lib
+
+    _lib.require_gpu()
+    t = np.ascontiguousarray(t, dtype=np.int32)
+    n = len(t)
+    if n == 0:
+        return np.zeros((0, 6))
+    pt = torch.from_numpy(np.ascontiguousarray(p, dtype=np.float64)).cuda()
+    tt = torch.from_numpy(t).cuda()
+    ang = torch.empty(6 * n, dtype=torch.float64, device=pt.device)
+    stream = torch.cuda.current_stream().cuda_stream
+    _lib.check(_lib.lib.sm_dihedral_angles(pt.data_ptr(), tt.data_ptr(), n, ang.data_ptr(),
+                                           stream), "sm_dihedral_angles")
+    return ang.cpu().numpy().reshape(n, 6)
