@@ -100,16 +100,9 @@ def cpu_baseline(st, p0, iters, reps=3):
     except ImportError:  # pragma: no cover
         threadpool_limits = None
     from oracle import distmesh as odm
-    from oracle import geometry as og
     from oracle.sizing import build_interpolant
 
-    dom = st["dom"]
-    if type(dom).__name__ == "Rectangle":
-        odom = og.Rectangle(dom.bbox0)
-    elif type(dom).__name__ == "Cube":
-        odom = og.Cube(dom.bbox0)
-    else:
-        odom = og.Disk([dom.xc, dom.yc], dom.r)
+    odom = _oracle_domain(st["dom"])
     fh = st["fh"]
     if fh.kind == "grid":
         rgi = build_interpolant(fh.grid.values.astype(np.float64), fh.grid.bbox)
@@ -145,15 +138,26 @@ def cpu_baseline(st, p0, iters, reps=3):
 
 
 def _oracle_domain(dom):
-    """The oracle's copy of a product geometry object (Disk, Rectangle and
+    """The oracle's copy of a product geometry object (Disk, Rectangle, Ball,
+    Cube, Cylinder, Torus -- translated, not rotated or stretched -- and
     their unions / differences)."""
     from oracle import geometry as og
     name = type(dom).__name__
+    tr = getattr(dom, "translation_vec", None)
+    tr = None if tr is None else [float(v) for v in tr]
     if name == "Disk":
-        return og.Disk([dom.xc, dom.yc], dom.r)
+        return og.Disk([dom.xc, dom.yc], dom.r, translate=tr)
     if name == "Rectangle":
-        return og.Rectangle(dom.bbox0)
-    if name in ("Union", "Difference"):
+        return og.Rectangle(dom.bbox0, translate=tr)
+    if name == "Ball":
+        return og.Ball([dom.xc, dom.yc, dom.zc], dom.r, translate=tr)
+    if name == "Cube":
+        return og.Cube(dom.bbox0, translate=tr)
+    if name == "Cylinder":
+        return og.Cylinder(h=2.0 * dom.h[1], r=dom.h[0], translate=tr)
+    if name == "Torus":
+        return og.Torus(dom.t[0], dom.t[1], translate=tr)
+    if name in ("Union", "Difference", "Intersection"):
         return getattr(og, name)([_oracle_domain(d) for d in dom.domains])
     raise ValueError("no oracle copy of %s" % name)
 
@@ -250,7 +254,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=49)
     ap.add_argument("--warmup", type=int, default=49)
-    ap.add_argument("--workload", default="bp2004", choices=["bp2004", "disk", "many", "eage"])
+    ap.add_argument("--workload", default="bp2004",
+                    choices=["bp2004", "disk", "many", "eage", "csg3d"])
     ap.add_argument("--domains", type=int, default=512,
                     help="number of domains of the many-domains workload")
     ap.add_argument("--scale", type=float, default=1.0)
@@ -305,6 +310,12 @@ def main():
             wl = {"workload": "eage_3d", "grid": list(info["grid"]),
                   "bbox_m": list(info["bbox"]), "hmin_m": 150.0, "grade": 0.15,
                   "max_iter": 50, "triangulation": "host exact Bowyer-Watson per iteration"}
+        elif args.workload == "csg3d":
+            domain, bbox3 = workloads.csg3d()
+            ef = 0.03 / args.scale
+            wl = {"workload": "csg3d_neural", "primitives": "48 balls, 16 boxes, 8 cylinders, "
+                  "4 tori (seed 3)", "h": ef, "max_iter": 50,
+                  "triangulation": "host exact Bowyer-Watson per iteration"}
         else:
             domain, ef, info = workloads.disk()
             wl = {"workload": "unit_disk_h0.02", "max_iter": 50}
@@ -320,34 +331,35 @@ def main():
 
     progress("prepared %s: %d points" % (wl["workload"], N))
 
-    def timed(graph):
-        m.set_graph(graph)
+    def timed(graph, h=None):
+        h = m if h is None else h
+        h.set_graph(graph)
         # warm-up: a full loop from the initial points (allocations, graph)
-        m.set_points(p0)
+        h.set_points(p0)
         if args.warmup > 0:
-            m.run(args.warmup)
+            h.run(args.warmup)
         # timed: `steps` iterations from the initial points
-        m.set_points(p0)
-        s0 = m.stats()
+        h.set_points(p0)
+        s0 = h.stats()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        m.run(args.steps)
+        h.run(args.steps)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
-        s1 = m.stats()
+        s1 = h.stats()
         progress("%s loop: %d iterations in %.3f s" % ("graph" if graph else "plain",
                                                       args.steps, dt))
         d = {k: s1[k] - s0[k] for k in s1}
-        d["min_quality"] = min_quality()
+        d["min_quality"] = min_quality(h)
         return dt, d
 
-    def min_quality():
+    def min_quality(h):
         """Smallest cell quality of the loop's mesh after the timed
         iterations (outside the timed region): 2D radius ratio, 3D minimum
         dihedral angle in degrees."""
         from seismicmesh_amd import mesh_utils as mu
-        m.triangulate()
-        p, t = m.points(), m.cells()
+        h.triangulate()
+        p, t = h.points(), h.cells()
         if sts is not None:
             return round(min(float(mu.simp_qual(q, c).min()) for q, c in m.split(p, t)
                              if len(c)), 4)
@@ -368,7 +380,34 @@ def main():
     dim = m.dim
     progress("autotune: %s" % {k: v[0] for k, v in tuned.items()})
 
-    if dim == 3:
+    neural = None
+    if args.workload == "csg3d":
+        # the exact-SDF loop against the loop steered by a surrogate trained
+        # here with the library's kernels (same points, same sizes)
+        from seismicmesh_amd.neural import NeuralSDF
+        t0 = time.perf_counter()
+        net = NeuralSDF(3, bbox3, hidden=3, seed=0, omega=8.0)
+        net.train(domain, nsamples=1000000, steps=2000, batch=8192, lr=2e-3, lr_final=1e-4,
+                  eik=0.1, near=0.05, seed=0)
+        band, err = net.calibrate_band(domain, 2.0 * ef, nval=200000)
+        t_train = time.perf_counter() - t0
+        progress("surrogate trained in %.1f s: max error %.4f, band %.4f" % (t_train, err, band))
+        exact = generation.make_mesher(st)
+        exact.set_workgroups({k: v[0] for k, v in tuned.items()})
+        m.set_surrogate(net, band)
+        elapsed, stats = timed(False)
+        p_neural = m.points()
+        plain_elapsed, plain_stats = timed(False, exact)
+        sg = m.surrogate_stats()
+        neural = {"training_s": round(t_train, 1), "steps": 2000, "samples": 1000000,
+                  "max_error": round(err, 4), "band": round(band, 4),
+                  "exact_evaluations_skipped": round(sg["skipped"] / max(sg["skipped"] +
+                                                                          sg["exact"], 1), 4),
+                  "max_error_in_band": round(sg["max_err"], 4),
+                  "points_identical_to_exact_path": bool(np.array_equal(p_neural,
+                                                                        exact.points()))}
+        exact.close()
+    elif dim == 3:
         # 3D iterations have no captured-graph path: one timed pass
         elapsed, stats = timed(False)
         plain_elapsed, plain_stats = elapsed, stats
@@ -462,6 +501,10 @@ def main():
         "cpu_baseline": cpu,
         "mlp_roofline": mlp,
     }
+    if neural is not None:
+        out["config"]["surrogate"] = neural
+        out["config"]["path"] = "surrogate-steered filter and projection (exact SDF in the band)"
+        out["config"]["plain_path"] = "exact SDF everywhere"
     print(json.dumps(out))
 
 
