@@ -34,6 +34,18 @@ inline unsigned blocks_for(int64_t n, int block) {
   return (unsigned)b;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2.  xcd_block() renumbers them so that XCD x processes the x-th
+// contiguous share of the blocks: a kernel that gathers from neighbouring
+// elements then finds them in its own XCD's L2 instead of every L2 fetching
+// the whole array.  A bijection of [0, gridDim.x) for any grid size.
+constexpr unsigned kNumXcd = 8;
+__device__ inline unsigned xcd_block() {
+  const unsigned b = blockIdx.x, nb = gridDim.x;
+  const unsigned q = nb / kNumXcd, r = nb % kNumXcd, x = b % kNumXcd;
+  return x * q + (x < r ? x : r) + b / kNumXcd;
+}
+
 // Fill kernel used instead of hipMemsetAsync so the sequence stays safe to
 // capture in a graph.
 template <typename T>

@@ -64,7 +64,7 @@ struct PairSrc {
 };
 
 __global__ void k_count(PairSrc src, int32_t *cnt) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (i >= src.n) return;
   const int per = src.per();
   for (int j = 0; j < per; ++j) {
@@ -74,7 +74,7 @@ __global__ void k_count(PairSrc src, int32_t *cnt) {
 }
 
 __global__ void k_scatter(PairSrc src, const int32_t *rowptr, int32_t *cur, int32_t *col) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (i >= src.n) return;
   const int per = src.per();
   for (int j = 0; j < per; ++j) {
@@ -118,7 +118,7 @@ __device__ inline void sort16(int32_t (&v)[16]) {
 __global__ void k_row_sort(int64_t nrows, const int32_t *rowptr, int32_t *col, int32_t *ucnt,
                            int32_t *long_rows, int32_t *nlong, int32_t *mid_rows,
                            int32_t *nmid) {
-  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t r = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   int32_t s = rowptr[r], e = rowptr[r + 1];
   int32_t len = e - s;
@@ -237,7 +237,7 @@ __global__ void k_long_sort(const int32_t *long_rows, const int32_t *nlong,
 
 __global__ void k_emit(int64_t nrows, const int32_t *rowptr, const int32_t *col,
                        const int32_t *eptr, int32_t *edges) {
-  int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t r = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   int32_t s = rowptr[r], e = rowptr[r + 1];
   int32_t o = eptr[r];

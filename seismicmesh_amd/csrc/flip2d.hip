@@ -126,7 +126,7 @@ __global__ void k_fix_orient(int64_t T, const int32_t *tri, int32_t *adj,
     skip = prev[7] || prev[0] == 0 || prev[5] > 0;
     id = prev[6] + 1;
   }
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (prev && t == 0) {
     c[6] = id;
     c[7] = skip;
@@ -257,7 +257,7 @@ __global__ void k_mark(int64_t T, const int32_t *tri, const int32_t *adj, const 
                        int32_t *ctr, int queue_exact, ActView av, int par) {
   if (ctr[7]) return;
   const int round = ctr[6];
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= T) return;
   // an idle domain's claims stay 0 (reset by k_fix_orient), so its stale
   // candidates never match in k_apply
@@ -353,7 +353,7 @@ __global__ void k_apply(int64_t T, int32_t *tri, int32_t *adj, const uint64_t *c
   if (next && blockIdx.x == 0 && threadIdx.x < 6) next[threadIdx.x] = 0;
   if (ctr[7]) return;
   const int round = ctr[6];
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   // the next round's inverted flags start clear (nothing reads them before)
   if (av.act && t < av.ndom) av.act[(3 - par) * av.ndom + t] = 0;
   if (t >= T) return;

@@ -34,7 +34,7 @@ namespace {
 template <int DIM, int MODE>
 __global__ void k_keep(int64_t T, const int32_t *tri, const double *p, const sm_node *tape,
                        int ntape, double geps, uint8_t *keep) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= T) return;
   const int32_t *c = tri + (DIM + 1) * t;
   bool real = true;
@@ -61,7 +61,7 @@ __global__ void k_keep(int64_t T, const int32_t *tri, const double *p, const sm_
 template <int MODE>
 __global__ void k_keep_batch(int64_t T, const int32_t *tri, const double *p, BatchView bv,
                              uint8_t *keep) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= T) return;
   const int32_t *c = tri + 3 * t;
   if (c[0] < 0 || c[1] < 0 || c[2] < 0) {
@@ -84,19 +84,19 @@ __global__ void k_keep_batch(int64_t T, const int32_t *tri, const double *p, Bat
 // second-endpoint lists: count, fill, order
 __global__ void k_count_second(const int32_t *Ep, const int32_t *edges, int32_t *cnt) {
   const int64_t E = *Ep;
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (e < E) atomicAdd(&cnt[edges[2 * e + 1]], 1);
 }
 __global__ void k_fill_second(const int32_t *Ep, const int32_t *edges, const int32_t *ptr,
                               int32_t *cur, int32_t *list) {
   const int64_t E = *Ep;
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (e >= E) return;
   int b = edges[2 * e + 1];
   list[ptr[b] + atomicAdd(&cur[b], 1)] = (int32_t)e;
 }
 __global__ void k_sort_second(int64_t N, const int32_t *ptr, int32_t *list) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t v = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (v >= N) return;
   int32_t s = ptr[v], e = ptr[v + 1];
   for (int32_t k = s + 1; k < e; ++k) {
@@ -131,7 +131,8 @@ __global__ void k_edge_geom(const int32_t *Ep, const int32_t *edges, const doubl
                             GridView grid, const double *h_ext, BatchView bv, T *L_out, T *h_out,
                             double *Lp, double *hp, double *part) {
   const int64_t E = *Ep;
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned blk = xcd_block();  // the partial sums are stored by this index
+  int64_t e = (int64_t)blk * blockDim.x + threadIdx.x;
   double sl = 0.0, sh = 0.0;
   if (e < E) {
     const int a = edges[2 * e], b = edges[2 * e + 1];
@@ -180,8 +181,8 @@ __global__ void k_edge_geom(const int32_t *Ep, const int32_t *edges, const doubl
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      part[2 * blockIdx.x] = rl[0];
-      part[2 * blockIdx.x + 1] = rh[0];
+      part[2 * blk] = rl[0];
+      part[2 * blk + 1] = rh[0];
     }
   }
 }
@@ -347,7 +348,7 @@ __global__ void k_edge_force(const int32_t *Ep, const int32_t *edges, const doub
                              const T *L, const T *h, const double *scale_ptr, BatchView bv,
                              double L0mult, T *F) {
   const int64_t E = *Ep;
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (e >= E) return;
   const int a = edges[2 * e], b = edges[2 * e + 1];
   const T scale = bv.vdom ? (T)bv.scale[bv.vdom[a]] : (T)scale_ptr[2];
@@ -392,7 +393,7 @@ __global__ void k_vertex_update(int64_t N, double *p, const int32_t *first_ptr,
                                 int nfix, double dt, const sm_node *tapes, const int *tape_off,
                                 const int *tape_len, int nlevels, double deps, double hmin,
                                 BatchView bv, uint64_t *maxF, int project) {
-  int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t v = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (v >= N) return;
   T f[DIM];
 #pragma unroll
@@ -462,7 +463,7 @@ template <int DIM, int MODE>
 __global__ void k_keep_surrogate(int64_t T, const int32_t *tri, const double *p,
                                  const sm_node *tape, int ntape, double geps, const float *dnn,
                                  double band, uint8_t *keep, uint64_t *sg) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= T) return;
   const int32_t *c = tri + (DIM + 1) * t;
   bool real = true;
@@ -498,7 +499,7 @@ template <int DIM, int MODE>
 __global__ void k_project_surrogate(int64_t N, double *p, const float *dnn, double band,
                                     const sm_node *tape, int ntape, double deps, double hmin,
                                     uint64_t *sg) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t v = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (v >= N) return;
   const double dn = (double)dnn[v];
   const bool exact = !(dn <= -band);
