@@ -55,6 +55,30 @@ __global__ void k_fill(T *p, int64_t n, T v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
+// Several int32 arrays set to one value in a single launch (one graph node
+// instead of one per array).
+struct FillList {
+  int32_t *p[4];
+  int64_t n[4];
+};
+template <typename L>
+__global__ void k_fill_list(L f, int32_t v) {
+  const int k = blockIdx.y;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < f.n[k]; i += stride) f.p[k][i] = v;
+}
+inline int fill_list(const FillList &f, int count, int32_t v, hipStream_t s) {
+  int64_t nmax = 0;
+  for (int k = 0; k < count; ++k) nmax = f.n[k] > nmax ? f.n[k] : nmax;
+  if (count <= 0 || nmax <= 0) return SM_OK;
+  unsigned g = blocks_for(nmax, 256);
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(k_fill_list<FillList>, dim3(g, (unsigned)count), dim3(256), 0, s, f, v);
+  SM_LAUNCH_CHECK();
+  return SM_OK;
+}
+
 template <typename T>
 inline int fill(T *p, int64_t n, T v, hipStream_t s) {
   if (n <= 0) return SM_OK;

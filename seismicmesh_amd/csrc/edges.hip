@@ -119,6 +119,7 @@ __global__ void k_row_sort(int64_t nrows, const int32_t *rowptr, int32_t *col, i
                            int32_t *long_rows, int32_t *nlong, int32_t *mid_rows,
                            int32_t *nmid) {
   int64_t r = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  if (r == 0) ucnt[nrows] = 0;  // the scan's last input (its total slot)
   if (r >= nrows) return;
   int32_t s = rowptr[r], e = rowptr[r + 1];
   int32_t len = e - s;
@@ -289,9 +290,10 @@ int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *
   void *sws = w;
 
   int rc;
-  if ((rc = fill<int32_t>(rowptr, nvert + 1, 0, s))) return rc;
-  if ((rc = fill<int32_t>(cur, nvert + 1, 0, s))) return rc;
-  if ((rc = fill<int32_t>(ctr, 8, 0, s))) return rc;
+  {
+    FillList fl{{rowptr, cur, ctr, nullptr}, {nvert + 1, nvert + 1, 8, 0}};
+    if ((rc = fill_list(fl, 3, 0, s))) return rc;
+  }
   PairSrc src{pairs, cells, keep, dim, n};
   const int B = 256;
   if (n > 0) {
@@ -319,7 +321,7 @@ int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *
                        stage, 1);
     SM_LAUNCH_CHECK();
   }
-  if ((rc = fill<int32_t>(eptr + nvert, 1, 0, s))) return rc;
+  if (nvert == 0 && (rc = fill<int32_t>(eptr, 1, 0, s))) return rc;
   if ((rc = scan_exclusive(eptr, eptr, nvert + 1, count32 ? count32 : ctr + 3, sws, s))) return rc;
   if (nvert > 0) {
     hipLaunchKernelGGL(k_emit, dim3(blocks_for(nvert, B)), dim3(B), 0, s, nvert, rowptr, col, eptr,

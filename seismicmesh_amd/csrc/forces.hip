@@ -619,8 +619,10 @@ int build_edges(sm_mesher *m, bool sync_edges) {
   }
   const int B = 256;
   const int64_t N = m->N, Eb = edge_launch_bound(m);
-  if ((rc = fill<int32_t>(m->d_sptr, N + 1, 0, s))) return rc;
-  if ((rc = fill<int32_t>(m->d_scur, N + 1, 0, s))) return rc;
+  {
+    FillList fl{{m->d_sptr, m->d_scur, nullptr, nullptr}, {N + 1, N + 1, 0, 0}};
+    if ((rc = fill_list(fl, 2, 0, s))) return rc;
+  }
   hipLaunchKernelGGL(k_count_second, dim3(blocks_for(Eb, B)), dim3(B), 0, s, (const int32_t *)Ep,
                      m->d_edges, m->d_sptr);
   SM_LAUNCH_CHECK();
