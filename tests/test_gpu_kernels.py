@@ -132,7 +132,8 @@ def _unique_edges_dev(pairs, nvert, dev):
     return out[:e].cpu().numpy()
 
 
-@pytest.mark.parametrize("case", ["empty", "single", "random", "dense_rows", "huge_row", "mesh"])
+@pytest.mark.parametrize("case", ["empty", "single", "random", "dense_rows", "huge_row",
+                                  "many_vertices", "wave_rows", "mesh"])
 def test_unique_edges_bit_exact(dev, case):
     from oracle import distmesh as odm
     from oracle import geometry as og
@@ -150,6 +151,13 @@ def test_unique_edges_bit_exact(dev, case):
     elif case == "huge_row":  # one bucket beyond the LDS tile
         nv = 20000
         pairs = np.column_stack([np.zeros(12000, np.int32), rng.integers(1, nv, 12000)]).astype(np.int32)
+    elif case == "many_vertices":  # prefix sums past the one-workgroup scan
+        nv = 300000
+        pairs = rng.integers(0, nv, (400000, 2)).astype(np.int32)
+    elif case == "wave_rows":  # buckets of 17..64 ids (the one-wave sort)
+        nv = 3000
+        a = np.repeat(np.arange(nv, dtype=np.int32), 40)
+        pairs = np.column_stack([a, rng.integers(0, nv, len(a))]).astype(np.int32)
     else:
         p = rng.random((3000, 2))
         t = odm.delaunay(p)
