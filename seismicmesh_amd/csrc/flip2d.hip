@@ -168,9 +168,37 @@ __global__ void k_fix_orient(int64_t T, const int32_t *tri, int32_t *adj,
   }
 }
 
-// The link fix of a batch's last round.
+// Summary of a batch of rounds into status[0..7]: [0] rounds executed,
+// [1] flips, [2] converged (the last executed round flipped nothing),
+// [3] inverted triangles seen by that round, [4] unresolved predicates,
+// [5] exact-predicate evaluations.
+__device__ inline void flip_status(const int32_t *ctr, int nrounds, int32_t *status,
+                                   int32_t *round_counter) {
+  int last = -1, flips = 0, unres = 0, exact = 0;
+  for (int r = 0; r < nrounds; ++r) {
+    const int32_t *c = ctr + 8 * r;
+    if (c[7]) break;
+    last = r;
+    flips += c[0];
+    unres += c[5];
+    exact += c[2] + c[3];
+  }
+  status[0] = last + 1;
+  status[1] = flips;
+  status[2] = last >= 0 && ctr[8 * last] == 0;
+  status[3] = last >= 0 ? ctr[8 * last + 1] : 0;
+  status[4] = unres;
+  status[5] = exact;
+  status[6] = last >= 0 ? ctr[8 * last + 6] : 0;
+  if (last >= 0) *round_counter = ctr[8 * last + 6];
+}
+
+// The link fix of a batch's last round, and (one thread) the batch's
+// summary (every round's counters are final by now).
 __global__ void k_fix_last(int64_t T, const int32_t *tri, int32_t *adj, const int32_t *partner,
-                           const int32_t *mark, const int32_t *c) {
+                           const int32_t *mark, const int32_t *c, const int32_t *base,
+                           int nrounds, int32_t *status, int32_t *round_counter) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) flip_status(base, nrounds, status, round_counter);
   if (c[7]) return;
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t < T) fix_links(t, tri, adj, partner, mark, c[6]);
@@ -388,31 +416,6 @@ __global__ void k_apply(int64_t T, int32_t *tri, int32_t *adj, const uint64_t *c
     if (av.act) av.act[par * av.ndom + av.tdom[t]] = 1;
     break;
   }
-}
-
-// Summary of a batch of rounds into status[0..7]: [0] rounds executed,
-// [1] flips, [2] converged (the last executed round flipped nothing),
-// [3] inverted triangles seen by that round, [4] unresolved predicates,
-// [5] exact-predicate evaluations.
-__global__ void k_flip_status(const int32_t *ctr, int nrounds, int32_t *status,
-                              int32_t *round_counter) {
-  int last = -1, flips = 0, unres = 0, exact = 0;
-  for (int r = 0; r < nrounds; ++r) {
-    const int32_t *c = ctr + 8 * r;
-    if (c[7]) break;
-    last = r;
-    flips += c[0];
-    unres += c[5];
-    exact += c[2] + c[3];
-  }
-  status[0] = last + 1;
-  status[1] = flips;
-  status[2] = last >= 0 && ctr[8 * last] == 0;
-  status[3] = last >= 0 ? ctr[8 * last + 1] : 0;
-  status[4] = unres;
-  status[5] = exact;
-  status[6] = last >= 0 ? ctr[8 * last + 6] : 0;
-  if (last >= 0) *round_counter = ctr[8 * last + 6];
 }
 
 // After a flip batch that did not end valid: flags the domains (one flag
@@ -724,9 +727,7 @@ int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double
   }
   const int32_t *lastc = base + 8 * (nrounds - 1);
   hipLaunchKernelGGL(k_fix_last, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_partner,
-                     m->d_mark, lastc);
-  hipLaunchKernelGGL(k_flip_status, dim3(1), dim3(1), 0, s, (const int32_t *)base, nrounds,
-                     status, rc);
+                     m->d_mark, lastc, (const int32_t *)base, nrounds, status, rc);
   SM_LAUNCH_CHECK();
   return SM_OK;
 }
