@@ -210,8 +210,9 @@ __global__ void k_orient_exact(const int32_t *tri, const double *p, int32_t *ori
   const int n = ctr[2];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     int t = queue[i];
-    int o = smpred::orient2d_exact(PT(p, tri[3 * t]), PT(p, tri[3 * t + 1]),
-                                   PT(p, tri[3 * t + 2]));
+    const double *pa = PT(p, tri[3 * t]), *pb = PT(p, tri[3 * t + 1]), *pc = PT(p, tri[3 * t + 2]);
+    int o = smpred::orient2d_fix(pa, pb, pc);
+    if (o == SM_PRED_UNRESOLVED) o = smpred::orient2d_exact(pa, pb, pc);
     if (o == SM_PRED_UNRESOLVED) {
       o = 0;
       atomicAdd(&ctr[5], 1);
@@ -354,13 +355,16 @@ __global__ void k_mark_exact(const int32_t *tri, const int32_t *adj, const doubl
     // whose uncertain cases are also settled exactly below)
     int s;
     if (q.x >= 0 && q.a >= 0 && q.b >= 0) {
-      s = smpred::incircle_exact(PT(p, q.x), PT(p, q.a), PT(p, q.b), PT(p, q.y));
+      s = smpred::incircle_fix(PT(p, q.x), PT(p, q.a), PT(p, q.b), PT(p, q.y));
+      if (s == SM_PRED_UNRESOLVED)
+        s = smpred::incircle_exact(PT(p, q.x), PT(p, q.a), PT(p, q.b), PT(p, q.y));
     } else {
       int u, v;
       if (q.x < 0) { u = q.a; v = q.b; }
       else if (q.a < 0) { u = q.b; v = q.x; }
       else { u = q.x; v = q.a; }
-      s = smpred::orient2d_exact(PT(p, u), PT(p, v), PT(p, q.y));
+      s = smpred::orient2d_fix(PT(p, u), PT(p, v), PT(p, q.y));
+      if (s == SM_PRED_UNRESOLVED) s = smpred::orient2d_exact(PT(p, u), PT(p, v), PT(p, q.y));
     }
     if (s == SM_PRED_UNRESOLVED) {
       atomicAdd(&ctr[5], 1);

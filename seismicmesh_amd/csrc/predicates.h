@@ -12,10 +12,14 @@
 //
 // The exact stage keeps expansions in fixed-size arrays.  A predicate whose
 // exact evaluation would not fit returns SM_PRED_UNRESOLVED; callers treat
-// that as "cannot decide here" and fall back to the host rebuild.
+// that as "cannot decide here" and fall back to the host rebuild.  Before
+// it, the 2D predicates have a register-only exact stage in 128/256-bit
+// integer arithmetic that settles nearby points of similar magnitude (the
+// fixed-point stage below).
 #pragma once
 
 #include <math.h>
+#include <stdint.h>
 
 #if defined(__HIPCC__)
 #define SM_HD __host__ __device__
@@ -557,6 +561,172 @@ SM_HD inline int incircle_if_exact(const double *a, const double *b, const doubl
   return (det > 0.0) - (det < 0.0);
 }
 
+// ------------------------------------------------------ fixed-point stage
+// Exact signs in integer arithmetic, registers only (no expansion arrays).
+// A finite double is m * 2^e with an odd integer m (|m| < 2^53) or zero.
+// With E the smallest e over the nonzero coordinates involved, every
+// coordinate difference is an integer multiple of 2^E; when each quotient
+// is below 2^62 in magnitude (nearby points of similar magnitude: every
+// small quad of a mesh that does not straddle a coordinate axis at a tiny
+// offset) the determinants are integer polynomials in the quotients.
+// orient2d's is below 2^125 in magnitude and incircle's below 2^252, so
+// 128- and 256-bit two's-complement integers hold them exactly.  Other
+// inputs return SM_PRED_UNRESOLVED and the expansion stage decides.
+typedef __int128 fix_i128;
+typedef unsigned __int128 fix_u128;
+
+struct FixNum {
+  uint64_t mag;  // odd, or 0 for a zero
+  int e;
+  bool neg;
+};
+
+SM_HD inline bool fix_split(double x, FixNum &f) {
+  uint64_t u;
+  __builtin_memcpy(&u, &x, sizeof u);
+  const int be = (int)((u >> 52) & 0x7ff);
+  uint64_t m = u & ((1ull << 52) - 1);
+  if (be == 0x7ff) return false;  // inf or nan
+  int e;
+  if (be == 0) {
+    if (m == 0) {
+      f.mag = 0;
+      f.e = 0;
+      f.neg = false;
+      return true;
+    }
+    e = -1074;
+  } else {
+    m |= 1ull << 52;
+    e = be - 1075;
+  }
+  const int tz = __builtin_ctzll(m);
+  f.mag = m >> tz;
+  f.e = e + tz;
+  f.neg = (u >> 63) != 0;
+  return true;
+}
+
+// (a - b) / 2^E exactly, when that is below 2^62 in magnitude (E <= the
+// exponent of every nonzero operand)
+SM_HD inline bool fix_diff(const FixNum &a, const FixNum &b, int E, int64_t &d) {
+  fix_i128 A = 0, B = 0;
+  if (a.mag) {
+    const int s = a.e - E;
+    if (s > 72) return false;  // |m| 2^s < 2^125
+    A = (fix_i128)((fix_u128)a.mag << s);
+    if (a.neg) A = -A;
+  }
+  if (b.mag) {
+    const int s = b.e - E;
+    if (s > 72) return false;
+    B = (fix_i128)((fix_u128)b.mag << s);
+    if (b.neg) B = -B;
+  }
+  const fix_i128 D = A - B;
+  const fix_i128 lim = (fix_i128)1 << 62;
+  if (D >= lim || D <= -lim) return false;
+  d = (int64_t)D;
+  return true;
+}
+
+// the coordinates of np points (np * dim doubles) split; E = smallest
+// exponent of a nonzero one (0 when all are zero)
+template <int NP, int DIM>
+SM_HD inline bool fix_split_all(const double *const *P, FixNum (&f)[NP][DIM], int &E) {
+  E = 1 << 30;
+  for (int i = 0; i < NP; ++i)
+    for (int k = 0; k < DIM; ++k) {
+      if (!fix_split(P[i][k], f[i][k])) return false;
+      if (f[i][k].mag && f[i][k].e < E) E = f[i][k].e;
+    }
+  if (E == (1 << 30)) E = 0;
+  return true;
+}
+
+struct Fix256 {
+  uint64_t w[4];  // two's complement, least significant word first
+};
+
+// x * |y| for x, |y| < 2^126, negated when y < 0
+SM_HD inline Fix256 fix_mul(fix_u128 x, fix_i128 y) {
+  const bool neg = y < 0;
+  const fix_u128 ym = neg ? (fix_u128)(-y) : (fix_u128)y;
+  const uint64_t x0 = (uint64_t)x, x1 = (uint64_t)(x >> 64);
+  const uint64_t y0 = (uint64_t)ym, y1 = (uint64_t)(ym >> 64);
+  const fix_u128 p00 = (fix_u128)x0 * y0, p01 = (fix_u128)x0 * y1;
+  const fix_u128 p10 = (fix_u128)x1 * y0, p11 = (fix_u128)x1 * y1;
+  Fix256 r;
+  r.w[0] = (uint64_t)p00;
+  const fix_u128 mid = (p00 >> 64) + (uint64_t)p01 + (uint64_t)p10;
+  r.w[1] = (uint64_t)mid;
+  const fix_u128 hi = (mid >> 64) + (p01 >> 64) + (p10 >> 64) + (uint64_t)p11;
+  r.w[2] = (uint64_t)hi;
+  r.w[3] = (uint64_t)(hi >> 64) + (uint64_t)(p11 >> 64);
+  if (neg) {
+    uint64_t carry = 1;
+    for (int i = 0; i < 4; ++i) {
+      const uint64_t t = ~r.w[i] + carry;
+      carry = (carry && t == 0) ? 1 : 0;
+      r.w[i] = t;
+    }
+  }
+  return r;
+}
+
+SM_HD inline void fix_add(Fix256 &a, const Fix256 &b) {
+  uint64_t c = 0;
+  for (int i = 0; i < 4; ++i) {
+    uint64_t s = a.w[i] + c;
+    uint64_t c1 = s < c;
+    s += b.w[i];
+    c1 += s < b.w[i];
+    a.w[i] = s;
+    c = c1;
+  }
+}
+
+SM_HD inline int fix_sign(const Fix256 &a) {
+  if (a.w[3] >> 63) return -1;
+  return (a.w[0] | a.w[1] | a.w[2] | a.w[3]) != 0;
+}
+
+SM_HD inline int orient2d_fix(const double *a, const double *b, const double *c) {
+  const double *P[3] = {a, b, c};
+  FixNum f[3][2];
+  int E;
+  if (!fix_split_all<3, 2>(P, f, E)) return SM_PRED_UNRESOLVED;
+  int64_t acx, acy, bcx, bcy;
+  if (!fix_diff(f[0][0], f[2][0], E, acx) || !fix_diff(f[0][1], f[2][1], E, acy) ||
+      !fix_diff(f[1][0], f[2][0], E, bcx) || !fix_diff(f[1][1], f[2][1], E, bcy))
+    return SM_PRED_UNRESOLVED;
+  const fix_i128 det = (fix_i128)acx * bcy - (fix_i128)acy * bcx;
+  return (det > 0) - (det < 0);
+}
+
+SM_HD inline int incircle_fix(const double *a, const double *b, const double *c,
+                              const double *d) {
+  const double *P[4] = {a, b, c, d};
+  FixNum f[4][2];
+  int E;
+  if (!fix_split_all<4, 2>(P, f, E)) return SM_PRED_UNRESOLVED;
+  int64_t v[3][2];
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 2; ++k)
+      if (!fix_diff(f[i][k], f[3][k], E, v[i][k])) return SM_PRED_UNRESOLVED;
+  // 2x2 minors and lifts, each below 2^125 in magnitude
+  const fix_i128 bc = (fix_i128)v[1][0] * v[2][1] - (fix_i128)v[2][0] * v[1][1];
+  const fix_i128 ca = (fix_i128)v[2][0] * v[0][1] - (fix_i128)v[0][0] * v[2][1];
+  const fix_i128 ab = (fix_i128)v[0][0] * v[1][1] - (fix_i128)v[1][0] * v[0][1];
+  fix_u128 lift[3];
+  for (int i = 0; i < 3; ++i)
+    lift[i] = (fix_u128)((fix_i128)v[i][0] * v[i][0]) + (fix_u128)((fix_i128)v[i][1] * v[i][1]);
+  Fix256 s = fix_mul(lift[0], bc);
+  fix_add(s, fix_mul(lift[1], ca));
+  fix_add(s, fix_mul(lift[2], ab));
+  return fix_sign(s);
+}
+
 // Host entry points: same filters, exact stage with room for any input a
 // double-precision coordinate set can produce in practice.
 inline int orient2d_h(const double *a, const double *b, const double *c) {
@@ -566,6 +736,8 @@ inline int incircle_h(const double *a, const double *b, const double *c, const d
   int s = incircle_filter(a, b, c, d);
   if (s != SM_PRED_UNRESOLVED) return s;
   s = incircle_if_exact(a, b, c, d);
+  if (s != SM_PRED_UNRESOLVED) return s;
+  s = incircle_fix(a, b, c, d);
   if (s != SM_PRED_UNRESOLVED) return s;
   s = incircle_exact<96>(a, b, c, d);
   if (s != SM_PRED_UNRESOLVED) return s;

@@ -33,3 +33,12 @@ int h_incircle_if_exact(const double *a, const double *b, const double *c, const
   return smpred::incircle_if_exact(a, b, c, d);
 }
 }
+
+extern "C" {
+int h_orient2d_fix(const double *a, const double *b, const double *c) {
+  return smpred::orient2d_fix(a, b, c);
+}
+int h_incircle_fix(const double *a, const double *b, const double *c, const double *d) {
+  return smpred::incircle_fix(a, b, c, d);
+}
+}

@@ -29,7 +29,7 @@ def lib(tmp_path_factory):
     L = ctypes.CDLL(out)
     for name in ("h_orient2d_exact", "h_incircle_exact96", "h_incircle_h", "h_orient3d_h",
                  "h_insphere_h", "h_insphere_exact160", "h_orient2d_if_exact",
-                 "h_incircle_if_exact"):
+                 "h_incircle_if_exact", "h_orient2d_fix", "h_incircle_fix"):
         getattr(L, name).restype = ctypes.c_int
     return L
 
@@ -158,3 +158,60 @@ def test_register_stage_is_exact_when_it_answers(lib):
         a, b, c, d = sq
         assert lib.h_orient2d_if_exact(_p(a), _p(b), _p(c)) == P.orient2d(a, b, c) == 1
         assert lib.h_incircle_if_exact(_p(a), _p(b), _p(c), _p(d)) == 0
+
+
+def test_fixed_point_stage(lib):
+    """The 128/256-bit integer stage answers exactly or defers.  Cases that
+    a slip in it would get wrong: degenerate quads (sign 0) next to
+    one-ulp perturbations (sign decided by the last bit), points far from
+    the origin (differences of 50+ significant bits: the 256-bit sums and
+    carries), mixed signs around the axes, subnormals and zeros, and inputs
+    it must refuse (exponent spans too wide, inf/nan)."""
+    rng = np.random.default_rng(4)
+    cases = _cases2d(rng)
+    for _ in range(300):
+        base = rng.standard_normal(2) * 10.0 ** rng.integers(-3, 8)
+        span = abs(base).max() * 10.0 ** -rng.integers(1, 15)
+        cases.append(base + rng.standard_normal((4, 2)) * span)
+    for _ in range(200):  # around the axes, zeros and subnormals included
+        q = rng.standard_normal((4, 2)) * 1e-3
+        q[rng.integers(0, 4), rng.integers(0, 2)] = 0.0
+        if rng.random() < 0.3:
+            q[rng.integers(0, 4), rng.integers(0, 2)] = 5e-324 * rng.integers(1, 1000)
+        cases.append(q)
+    for _ in range(200):  # cocircular on a scaled, shifted lattice, then nudged
+        i, j = rng.integers(-1000, 1000, 2)
+        w, v = rng.integers(1, 50, 2)
+        sq = np.array([[i, j], [i + w, j], [i + w, j + v], [i, j + v]], float) * 0.75 + [
+            -12000.0, 3000.0]
+        if rng.random() < 0.5:
+            k, c = rng.integers(0, 4), rng.integers(0, 2)
+            sq[k, c] = np.nextafter(sq[k, c], np.inf if rng.random() < 0.5 else -np.inf)
+        cases.append(sq[rng.permutation(4)])
+    answered = {"o": 0, "i": 0, "i0": 0}
+    for q in cases:
+        a, b, c, d = q
+        o = P.orient2d(a, b, c)
+        got = lib.h_orient2d_fix(_p(a), _p(b), _p(c))
+        assert got in (o, UNRES), (q, o, got)
+        answered["o"] += got != UNRES
+        if o == 0:
+            continue
+        if o < 0:
+            a, b = b, a
+        want = P.incircle(a, b, c, d)
+        got = lib.h_incircle_fix(_p(a), _p(b), _p(c), _p(d))
+        assert got in (want, UNRES), (q, want, got)
+        answered["i"] += got != UNRES
+        answered["i0"] += got == 0
+        # the answer does not depend on which vertex of the triangle comes first
+        got2 = lib.h_incircle_fix(_p(b), _p(c), _p(a), _p(d))
+        assert got2 == got
+    assert answered["o"] > 0.8 * len(cases) and answered["i"] > 0.7 * len(cases), answered
+    assert answered["i0"] > 100, answered
+    # refusals: a difference of 2^62 units or more, and non-finite input
+    a, b, c, d = np.array([[1.0, 0.0], [0.0, 1.0], [-1.0, 0.0], [0.0, 1e-30]])
+    assert lib.h_incircle_fix(_p(a), _p(b), _p(c), _p(d)) == UNRES
+    assert lib.h_orient2d_fix(_p(a), _p(b), _p(d)) == UNRES
+    assert lib.h_orient2d_fix(_p(a), _p(b), _p(np.array([np.inf, 0.0]))) == UNRES
+    assert lib.h_incircle_fix(_p(a), _p(b), _p(c), _p(np.array([np.nan, 0.0]))) == UNRES
