@@ -13,8 +13,10 @@
 // buckets (arbitrary user input) are sorted by a whole workgroup in LDS
 // or, beyond that, in global memory.
 // Duplicates are dropped inside each sorted bucket; a second scan turns the
-// per-bucket unique counts into output offsets.  The output is exactly the
-// reference's sorted unique edge list.
+// per-bucket unique counts into output offsets.  For a 2D triangulation with
+// neighbour links each shared edge is produced once in the first place (by
+// the lower-numbered of its two cells) and the second scan is skipped.  The
+// output is exactly the reference's sorted unique edge list.
 #include "common.h"
 #include "edges.h"
 
@@ -49,6 +51,11 @@ struct PairSrc {
   const int32_t *pairs;
   const int32_t *cells;
   const uint8_t *keep;  // optional cell mask
+  // optional (2D cells only): adj[3 c + k] is the cell across the edge
+  // opposite local vertex k, or < 0.  An edge shared by two kept cells is
+  // then produced once, by the lower-numbered one, so every bucket is free
+  // of duplicates and its offsets are the output offsets.
+  const int32_t *adj;
   int dim;
   int64_t n;            // pairs, or cells
   __device__ inline int per() const { return pairs ? 1 : (dim == 2 ? 3 : 6); }
@@ -58,6 +65,11 @@ struct PairSrc {
       return true;
     }
     if (keep && !keep[i]) return false;
+    if (adj) {
+      // local edge j joins vertices j and (j + 1) % 3: opposite vertex (j + 2) % 3
+      const int32_t o = adj[3 * i + (j + 2) % 3];
+      if (o >= 0 && o < i && (!keep || keep[o])) return false;
+    }
     cell_edge(cells, i, dim, j, a, b);
     return true;
   }
@@ -271,9 +283,22 @@ size_t unique_edges_workspace(int64_t npairs, int64_t nvert) {
   return b;
 }
 
-int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *keep, int dim,
-                     int64_t n, int64_t npairs, int64_t nvert, int32_t *edges, int32_t *count32,
-                     int64_t *count64, void *ws, size_t ws_bytes, hipStream_t s) {
+int32_t *unique_edges_offsets(void *ws, int64_t npairs, int64_t nvert, bool adj) {
+  // workspace order: rowptr, cur, col, eptr (see unique_edges_run); with
+  // neighbour links the bucket offsets are the output offsets
+  char *w = reinterpret_cast<char *>(ws);
+  if (adj) return reinterpret_cast<int32_t *>(w);
+  w += al((nvert + 1) * 4);
+  w += al((nvert + 1) * 4);
+  w += al(npairs * 4);
+  return reinterpret_cast<int32_t *>(w);
+}
+
+int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *keep,
+                     const int32_t *adj, int dim, int64_t n, int64_t npairs, int64_t nvert,
+                     int32_t *edges, int32_t *count32, int64_t *count64, void *ws,
+                     size_t ws_bytes, hipStream_t s) {
+  SM_CHECK(!adj || (cells && dim == 2), SM_ERR_ARG, "unique edges: adjacency needs 2D cells");
   SM_CHECK(ws_bytes >= unique_edges_workspace(npairs, nvert), SM_ERR_CAPACITY,
            "unique edges: workspace %zu < %zu bytes", ws_bytes,
            unique_edges_workspace(npairs, nvert));
@@ -294,13 +319,16 @@ int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *
     FillList fl{{rowptr, cur, ctr, nullptr}, {nvert + 1, nvert + 1, 8, 0}};
     if ((rc = fill_list(fl, 3, 0, s))) return rc;
   }
-  PairSrc src{pairs, cells, keep, dim, n};
+  PairSrc src{pairs, cells, keep, adj, dim, n};
   const int B = 256;
+  int32_t *total = count32 ? count32 : ctr + 3;
   if (n > 0) {
     hipLaunchKernelGGL(k_count, dim3(blocks_for(n, B)), dim3(B), 0, s, src, rowptr);
     SM_LAUNCH_CHECK();
   }
-  if ((rc = scan_exclusive(rowptr, rowptr, nvert + 1, ctr + 1, sws, s))) return rc;
+  // with adjacency the buckets hold no duplicates: their offsets and total
+  // are the output's
+  if ((rc = scan_exclusive(rowptr, rowptr, nvert + 1, adj ? total : ctr + 1, sws, s))) return rc;
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, B)), dim3(B), 0, s, src, rowptr, cur, col);
     SM_LAUNCH_CHECK();
@@ -321,16 +349,17 @@ int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *
                        stage, 1);
     SM_LAUNCH_CHECK();
   }
-  if (nvert == 0 && (rc = fill<int32_t>(eptr, 1, 0, s))) return rc;
-  if ((rc = scan_exclusive(eptr, eptr, nvert + 1, count32 ? count32 : ctr + 3, sws, s))) return rc;
+  if (!adj) {
+    if (nvert == 0 && (rc = fill<int32_t>(eptr, 1, 0, s))) return rc;
+    if ((rc = scan_exclusive(eptr, eptr, nvert + 1, total, sws, s))) return rc;
+  }
   if (nvert > 0) {
-    hipLaunchKernelGGL(k_emit, dim3(blocks_for(nvert, B)), dim3(B), 0, s, nvert, rowptr, col, eptr,
-                       edges);
+    hipLaunchKernelGGL(k_emit, dim3(blocks_for(nvert, B)), dim3(B), 0, s, nvert, rowptr, col,
+                       adj ? (const int32_t *)rowptr : (const int32_t *)eptr, edges);
     SM_LAUNCH_CHECK();
   }
   if (count64) {
-    hipLaunchKernelGGL(k_count_to_i64, dim3(1), dim3(1), 0, s, count32 ? count32 : ctr + 3,
-                       count64);
+    hipLaunchKernelGGL(k_count_to_i64, dim3(1), dim3(1), 0, s, total, count64);
     SM_LAUNCH_CHECK();
   }
   return SM_OK;
@@ -348,8 +377,8 @@ extern "C" int sm_unique_edges(const int32_t *pairs, int64_t npairs, int64_t nve
   SM_CHECK(npairs >= 0 && nvert >= 0, SM_ERR_ARG, "unique_edges: negative size");
   SM_CHECK(npairs == 0 || (pairs && edges), SM_ERR_ARG, "unique_edges: null pointer");
   SM_CHECK(count && ws, SM_ERR_ARG, "unique_edges: null count/workspace");
-  return smk::unique_edges_run(pairs, nullptr, nullptr, 2, npairs, npairs, nvert, edges, nullptr,
-                               count, ws, ws_bytes, smk::stream_of(stream));
+  return smk::unique_edges_run(pairs, nullptr, nullptr, nullptr, 2, npairs, npairs, nvert, edges,
+                               nullptr, count, ws, ws_bytes, smk::stream_of(stream));
 }
 
 extern "C" size_t sm_cell_edges_workspace(int64_t ncells, int dim, int64_t nvert) {
@@ -364,6 +393,6 @@ extern "C" int sm_cell_edges(const int32_t *cells, int64_t ncells, int dim, int6
   SM_CHECK(ncells == 0 || (cells && edges), SM_ERR_ARG, "cell_edges: null pointer");
   SM_CHECK(count && ws, SM_ERR_ARG, "cell_edges: null count/workspace");
   int64_t np = ncells * (dim == 2 ? 3 : 6);
-  return smk::unique_edges_run(nullptr, cells, nullptr, dim, ncells, np, nvert, edges, nullptr,
-                               count, ws, ws_bytes, smk::stream_of(stream));
+  return smk::unique_edges_run(nullptr, cells, nullptr, nullptr, dim, ncells, np, nvert, edges,
+                               nullptr, count, ws, ws_bytes, smk::stream_of(stream));
 }

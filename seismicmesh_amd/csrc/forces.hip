@@ -582,16 +582,9 @@ int centroid_keep(sm_mesher *m) {
   return SM_OK;
 }
 
-// first-endpoint offsets live in the unique-edge workspace (eptr)
+// first-endpoint offsets live in the unique-edge workspace
 static int32_t *eptr_of(sm_mesher *m) {
-  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-  int64_t nv = m->N;
-  int64_t np = m->capT * (m->dim == 2 ? 3 : 6);
-  char *w = reinterpret_cast<char *>(m->d_ews);
-  w += al((nv + 1) * 4);
-  w += al((nv + 1) * 4);
-  w += al(np * 4);
-  return reinterpret_cast<int32_t *>(w);
+  return unique_edges_offsets(m->d_ews, m->capT * (m->dim == 2 ? 3 : 6), m->N, m->dim == 2);
 }
 
 // Launch bound for per-edge kernels: a triangulation of N points has at
@@ -609,8 +602,10 @@ int build_edges(sm_mesher *m, bool sync_edges) {
   const int dim = m->dim;
   int64_t np = m->capT * (dim == 2 ? 3 : 6);
   int32_t *Ep = m->d_ctr + kCtrE;
-  int rc = unique_edges_run(nullptr, m->d_tri, m->d_keep, dim, m->T, np, m->N, m->d_edges, Ep,
-                            nullptr, m->d_ews, m->ews_bytes, s);
+  // 2D triangulations carry consistent neighbour links here (device flips
+  // end with the link fix; every host rebuild writes both arrays)
+  int rc = unique_edges_run(nullptr, m->d_tri, m->d_keep, dim == 2 ? m->d_adj : nullptr, dim,
+                            m->T, np, m->N, m->d_edges, Ep, nullptr, m->d_ews, m->ews_bytes, s);
   if (rc) return rc;
   if (sync_edges) {
     SM_HIP(hipMemcpyAsync(m->h_ctr + kCtrE, Ep, sizeof(int32_t), hipMemcpyDeviceToHost, s));
