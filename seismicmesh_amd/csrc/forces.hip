@@ -387,43 +387,62 @@ __device__ inline double project_level(double *x, const sm_node *tape, int ntape
   return d;
 }
 
+// atomicMax of the 64-bit patterns over a whole wave with one atomic: the
+// wave's maximum is taken by shuffles first (every lane of the wave must
+// call this).  For non-negative doubles the bit patterns order as the
+// values; a NaN pattern exceeds every finite one, as with a per-lane atomic.
+__device__ inline void wave_max_bits(uint64_t *dst, uint64_t bits) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const uint64_t o = (uint64_t)__shfl_xor((unsigned long long)bits, off);
+    bits = o > bits ? o : bits;
+  }
+  if (__lane_id() == 0) atomicMax((unsigned long long *)dst, (unsigned long long)bits);
+}
+
 template <int DIM, typename T, int MODE>
 __global__ void k_vertex_update(int64_t N, double *p, const int32_t *first_ptr,
                                 const int32_t *second_ptr, const int32_t *second, const T *F,
                                 int nfix, double dt, const sm_node *tapes, const int *tape_off,
                                 const int *tape_len, int nlevels, double deps, double hmin,
                                 BatchView bv, uint64_t *maxF, int project) {
-  int64_t v = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
-  if (v >= N) return;
+  const int64_t v = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  const bool live = v < N;
   T f[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) f[d] = (T)0;
-  for (int32_t k = second_ptr[v], ke = second_ptr[v + 1]; k < ke; ++k) {
-    int64_t e = second[k];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) f[d] = f[d] + (-F[e * DIM + d]);
-  }
-  for (int32_t e = first_ptr[v], ee = first_ptr[v + 1]; e < ee; ++e) {
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) f[d] = f[d] + F[(int64_t)e * DIM + d];
-  }
   int dom = 0;
-  bool fixed = v < nfix;
-  if (bv.vdom) {
-    dom = bv.vdom[v];
-    fixed = v - bv.voff[dom] < bv.nfix[dom];
-  }
-  if (fixed) {
+  uint64_t sbits = 0;
+  if (live) {
+    for (int32_t k = second_ptr[v], ke = second_ptr[v + 1]; k < ke; ++k) {
+      int64_t e = second[k];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) f[d] = (T)0;
-  }
-  double s2 = (double)f[0] * (double)f[0];
+      for (int d = 0; d < DIM; ++d) f[d] = f[d] + (-F[e * DIM + d]);
+    }
+    for (int32_t e = first_ptr[v], ee = first_ptr[v + 1]; e < ee; ++e) {
 #pragma unroll
-  for (int d = 1; d < DIM; ++d) s2 = s2 + (double)f[d] * (double)f[d];
-  double s = sqrt(s2);
-  // batches track the movement relative to each domain's h0
-  if (bv.vdom) s = s / bv.h[dom];
-  atomicMax((unsigned long long *)maxF, (unsigned long long)__double_as_longlong(s));
+      for (int d = 0; d < DIM; ++d) f[d] = f[d] + F[(int64_t)e * DIM + d];
+    }
+    bool fixed = v < nfix;
+    if (bv.vdom) {
+      dom = bv.vdom[v];
+      fixed = v - bv.voff[dom] < bv.nfix[dom];
+    }
+    if (fixed) {
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) f[d] = (T)0;
+    }
+    double s2 = (double)f[0] * (double)f[0];
+#pragma unroll
+    for (int d = 1; d < DIM; ++d) s2 = s2 + (double)f[d] * (double)f[d];
+    double s = sqrt(s2);
+    // batches track the movement relative to each domain's h0
+    if (bv.vdom) s = s / bv.h[dom];
+    sbits = (uint64_t)__double_as_longlong(s);
+  }
+  // every lane of the wave reaches this point (lanes past the end add 0)
+  wave_max_bits(maxF, sbits);
+  if (!live) return;
   double x[3];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) x[d] = p[v * DIM + d] + dt * (double)f[d];
