@@ -58,8 +58,8 @@ __global__ void k_fill(T *p, int64_t n, T v) {
 // Several int32 arrays set to one value in a single launch (one graph node
 // instead of one per array).
 struct FillList {
-  int32_t *p[4];
-  int64_t n[4];
+  int32_t *p[6];
+  int64_t n[6];
 };
 template <typename L>
 __global__ void k_fill_list(L f, int32_t v) {

@@ -12,10 +12,18 @@ size_t unique_edges_workspace(int64_t npairs, int64_t nvert);
 // vertex k at 3 c + k (or < 0); the links must be mutual.  Writes the sorted
 // unique edges, the int32 count (device, optional) and the int64 count
 // (device, optional).
+// Optional output of a run with neighbour links: for each vertex v the ids
+// of the edges whose second endpoint is v, as offsets `ptr` (nvert + 1) into
+// `list` (one entry per edge), in no particular order within a vertex;
+// `cur` is nvert + 1 counters of scratch.
+struct SecondLists {
+  int32_t *ptr, *cur, *list;
+};
+
 int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *keep,
-                     const int32_t *adj, int dim, int64_t n, int64_t npairs, int64_t nvert,
-                     int32_t *edges, int32_t *count32, int64_t *count64, void *ws,
-                     size_t ws_bytes, hipStream_t s);
+                     const int32_t *adj, const SecondLists *second, int dim, int64_t n,
+                     int64_t npairs, int64_t nvert, int32_t *edges, int32_t *count32,
+                     int64_t *count64, void *ws, size_t ws_bytes, hipStream_t s);
 
 // After unique_edges_run with the same npairs and nvert: the device array
 // (nvert + 1 entries) of each first endpoint's offset in the edge list, in

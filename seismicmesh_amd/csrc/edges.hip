@@ -75,13 +75,17 @@ struct PairSrc {
   }
 };
 
-__global__ void k_count(PairSrc src, int32_t *cnt) {
+// cnt2 (optional): the same edges counted by their second endpoint
+__global__ void k_count(PairSrc src, int32_t *cnt, int32_t *cnt2) {
   int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (i >= src.n) return;
   const int per = src.per();
   for (int j = 0; j < per; ++j) {
     int32_t a, b;
-    if (src.get(i, j, a, b)) atomicAdd(&cnt[a], 1);
+    if (src.get(i, j, a, b)) {
+      atomicAdd(&cnt[a], 1);
+      if (cnt2) atomicAdd(&cnt2[b], 1);
+    }
   }
 }
 
@@ -249,15 +253,17 @@ __global__ void k_long_sort(const int32_t *long_rows, const int32_t *nlong,
 }
 
 __global__ void k_emit(int64_t nrows, const int32_t *rowptr, const int32_t *col,
-                       const int32_t *eptr, int32_t *edges) {
+                       const int32_t *eptr, int32_t *edges, SecondLists second) {
   int64_t r = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (r >= nrows) return;
   int32_t s = rowptr[r], e = rowptr[r + 1];
   int32_t o = eptr[r];
   for (int32_t k = s; k < e; ++k) {
     if (k == s || col[k] != col[k - 1]) {
+      const int32_t b = col[k];
       edges[2 * (int64_t)o] = (int32_t)r;
-      edges[2 * (int64_t)o + 1] = col[k];
+      edges[2 * (int64_t)o + 1] = b;
+      if (second.list) second.list[second.ptr[b] + atomicAdd(&second.cur[b], 1)] = o;
       ++o;
     }
   }
@@ -295,10 +301,13 @@ int32_t *unique_edges_offsets(void *ws, int64_t npairs, int64_t nvert, bool adj)
 }
 
 int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *keep,
-                     const int32_t *adj, int dim, int64_t n, int64_t npairs, int64_t nvert,
-                     int32_t *edges, int32_t *count32, int64_t *count64, void *ws,
-                     size_t ws_bytes, hipStream_t s) {
+                     const int32_t *adj, const SecondLists *second, int dim, int64_t n,
+                     int64_t npairs, int64_t nvert, int32_t *edges, int32_t *count32,
+                     int64_t *count64, void *ws, size_t ws_bytes, hipStream_t s) {
   SM_CHECK(!adj || (cells && dim == 2), SM_ERR_ARG, "unique edges: adjacency needs 2D cells");
+  SM_CHECK(!second || (adj && second->ptr && second->cur && second->list), SM_ERR_ARG,
+           "unique edges: second-endpoint lists need neighbour links");
+  const SecondLists sl = second ? *second : SecondLists{nullptr, nullptr, nullptr};
   SM_CHECK(ws_bytes >= unique_edges_workspace(npairs, nvert), SM_ERR_CAPACITY,
            "unique edges: workspace %zu < %zu bytes", ws_bytes,
            unique_edges_workspace(npairs, nvert));
@@ -316,19 +325,20 @@ int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *
 
   int rc;
   {
-    FillList fl{{rowptr, cur, ctr, nullptr}, {nvert + 1, nvert + 1, 8, 0}};
-    if ((rc = fill_list(fl, 3, 0, s))) return rc;
+    FillList fl{{rowptr, cur, ctr, sl.ptr, sl.cur}, {nvert + 1, nvert + 1, 8, nvert + 1, nvert + 1}};
+    if ((rc = fill_list(fl, second ? 5 : 3, 0, s))) return rc;
   }
   PairSrc src{pairs, cells, keep, adj, dim, n};
   const int B = 256;
   int32_t *total = count32 ? count32 : ctr + 3;
   if (n > 0) {
-    hipLaunchKernelGGL(k_count, dim3(blocks_for(n, B)), dim3(B), 0, s, src, rowptr);
+    hipLaunchKernelGGL(k_count, dim3(blocks_for(n, B)), dim3(B), 0, s, src, rowptr, sl.ptr);
     SM_LAUNCH_CHECK();
   }
   // with adjacency the buckets hold no duplicates: their offsets and total
   // are the output's
   if ((rc = scan_exclusive(rowptr, rowptr, nvert + 1, adj ? total : ctr + 1, sws, s))) return rc;
+  if (second && (rc = scan_exclusive(sl.ptr, sl.ptr, nvert + 1, nullptr, sws, s))) return rc;
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, B)), dim3(B), 0, s, src, rowptr, cur, col);
     SM_LAUNCH_CHECK();
@@ -355,7 +365,7 @@ int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *
   }
   if (nvert > 0) {
     hipLaunchKernelGGL(k_emit, dim3(blocks_for(nvert, B)), dim3(B), 0, s, nvert, rowptr, col,
-                       adj ? (const int32_t *)rowptr : (const int32_t *)eptr, edges);
+                       adj ? (const int32_t *)rowptr : (const int32_t *)eptr, edges, sl);
     SM_LAUNCH_CHECK();
   }
   if (count64) {
@@ -377,8 +387,8 @@ extern "C" int sm_unique_edges(const int32_t *pairs, int64_t npairs, int64_t nve
   SM_CHECK(npairs >= 0 && nvert >= 0, SM_ERR_ARG, "unique_edges: negative size");
   SM_CHECK(npairs == 0 || (pairs && edges), SM_ERR_ARG, "unique_edges: null pointer");
   SM_CHECK(count && ws, SM_ERR_ARG, "unique_edges: null count/workspace");
-  return smk::unique_edges_run(pairs, nullptr, nullptr, nullptr, 2, npairs, npairs, nvert, edges,
-                               nullptr, count, ws, ws_bytes, smk::stream_of(stream));
+  return smk::unique_edges_run(pairs, nullptr, nullptr, nullptr, nullptr, 2, npairs, npairs, nvert,
+                               edges, nullptr, count, ws, ws_bytes, smk::stream_of(stream));
 }
 
 extern "C" size_t sm_cell_edges_workspace(int64_t ncells, int dim, int64_t nvert) {
@@ -393,6 +403,6 @@ extern "C" int sm_cell_edges(const int32_t *cells, int64_t ncells, int dim, int6
   SM_CHECK(ncells == 0 || (cells && edges), SM_ERR_ARG, "cell_edges: null pointer");
   SM_CHECK(count && ws, SM_ERR_ARG, "cell_edges: null count/workspace");
   int64_t np = ncells * (dim == 2 ? 3 : 6);
-  return smk::unique_edges_run(nullptr, cells, nullptr, nullptr, dim, ncells, np, nvert, edges,
-                               nullptr, count, ws, ws_bytes, smk::stream_of(stream));
+  return smk::unique_edges_run(nullptr, cells, nullptr, nullptr, nullptr, dim, ncells, np, nvert,
+                               edges, nullptr, count, ws, ws_bytes, smk::stream_of(stream));
 }

@@ -622,9 +622,13 @@ int build_edges(sm_mesher *m, bool sync_edges) {
   int64_t np = m->capT * (dim == 2 ? 3 : 6);
   int32_t *Ep = m->d_ctr + kCtrE;
   // 2D triangulations carry consistent neighbour links here (device flips
-  // end with the link fix; every host rebuild writes both arrays)
-  int rc = unique_edges_run(nullptr, m->d_tri, m->d_keep, dim == 2 ? m->d_adj : nullptr, dim,
-                            m->T, np, m->N, m->d_edges, Ep, nullptr, m->d_ews, m->ews_bytes, s);
+  // end with the link fix; every host rebuild writes both arrays); the
+  // builder then also fills the second-endpoint lists
+  const bool links = dim == 2;
+  const SecondLists second{m->d_sptr, m->d_scur, m->d_slist};
+  int rc = unique_edges_run(nullptr, m->d_tri, m->d_keep, links ? m->d_adj : nullptr,
+                            links ? &second : nullptr, dim, m->T, np, m->N, m->d_edges, Ep,
+                            nullptr, m->d_ews, m->ews_bytes, s);
   if (rc) return rc;
   if (sync_edges) {
     SM_HIP(hipMemcpyAsync(m->h_ctr + kCtrE, Ep, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -633,17 +637,18 @@ int build_edges(sm_mesher *m, bool sync_edges) {
   }
   const int B = 256;
   const int64_t N = m->N, Eb = edge_launch_bound(m);
-  {
+  if (!links) {
     FillList fl{{m->d_sptr, m->d_scur, nullptr, nullptr}, {N + 1, N + 1, 0, 0}};
     if ((rc = fill_list(fl, 2, 0, s))) return rc;
+    hipLaunchKernelGGL(k_count_second, dim3(blocks_for(Eb, B)), dim3(B), 0, s,
+                       (const int32_t *)Ep, m->d_edges, m->d_sptr);
+    SM_LAUNCH_CHECK();
+    if ((rc = scan_exclusive(m->d_sptr, m->d_sptr, N + 1, nullptr, m->d_scan_ws, s))) return rc;
+    hipLaunchKernelGGL(k_fill_second, dim3(blocks_for(Eb, B)), dim3(B), 0, s,
+                       (const int32_t *)Ep, m->d_edges, m->d_sptr, m->d_scur, m->d_slist);
+    SM_LAUNCH_CHECK();
   }
-  hipLaunchKernelGGL(k_count_second, dim3(blocks_for(Eb, B)), dim3(B), 0, s, (const int32_t *)Ep,
-                     m->d_edges, m->d_sptr);
-  SM_LAUNCH_CHECK();
-  if ((rc = scan_exclusive(m->d_sptr, m->d_sptr, N + 1, nullptr, m->d_scan_ws, s))) return rc;
-  hipLaunchKernelGGL(k_fill_second, dim3(blocks_for(Eb, B)), dim3(B), 0, s, (const int32_t *)Ep,
-                     m->d_edges, m->d_sptr, m->d_scur, m->d_slist);
-  SM_LAUNCH_CHECK();
+  // either way each vertex's list is then put in increasing edge order
   hipLaunchKernelGGL(k_sort_second, dim3(blocks_for(N, B)), dim3(B), 0, s, N, m->d_sptr,
                      m->d_slist);
   SM_LAUNCH_CHECK();
