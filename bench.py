@@ -85,9 +85,12 @@ def alg_bytes(phase, N, E, C, T, dim=2):
         # cell ids + keep flag 1 per cell, points per vertex
         return T * (4 * (dim + 1) + 1) + N * pt
     if phase == "flip_mark":
-        # per triangle: ids 12, neighbours 12, orientation 4, three 64-bit
-        # candidates 24, its 64-bit claim 8; points 16 per vertex
-        return T * (12 + 12 + 4 + 24 + 8) + N * 16
+        # per triangle: ids 12, neighbours 12, three 64-bit candidates 24,
+        # its 64-bit claim 8; points 16 per vertex.  Orientations are not
+        # read when no triangle is inverted or uncertain (the timed launches
+        # start from clear counters; were incircle tests uncertain, later
+        # launches would also read them and this rate is a lower bound)
+        return T * (12 + 12 + 24 + 8) + N * 16
     return 0
 
 

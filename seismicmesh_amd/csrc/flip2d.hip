@@ -286,6 +286,12 @@ __global__ void k_mark(int64_t T, const int32_t *tri, const int32_t *adj, const 
                        int32_t *ctr, int queue_exact, ActView av, int par) {
   if (ctr[7]) return;
   const int round = ctr[6];
+  // no triangle with orientation <= 0 and none left uncertain (the
+  // orientation pass counted both; a domain skipped this round has none):
+  // the inverted-triangle tests below are all false and their loads are
+  // skipped.  An uncertain incircle counted by this kernel meanwhile only
+  // sends a thread down the full path.
+  const bool tangled = ctr[1] != 0 || ctr[5] != 0;
   int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= T) return;
   // an idle domain's claims stay 0 (reset by k_fix_orient), so its stale
@@ -300,8 +306,8 @@ __global__ void k_mark(int64_t T, const int32_t *tri, const int32_t *adj, const 
         cand[3 * t + k] = 0;
         continue;
       }
-      const bool inv_t = ori[t] <= 0;
-      const bool inv_o = ori[o] <= 0;
+      const bool inv_t = tangled && ori[t] <= 0;
+      const bool inv_o = tangled && ori[o] <= 0;
       if (inv_t || inv_o) {
         // untangling flip: both new triangles (x, a, y) and (y, b, x) valid
         bool ghost_new = q.x < 0 || q.y < 0;
@@ -314,7 +320,7 @@ __global__ void k_mark(int64_t T, const int32_t *tri, const int32_t *adj, const 
           ok = valid_new(p, q.x, q.a, q.y) && valid_new(p, q.y, q.b, q.x);
         }
         if (ok) prio = (2ull << 32) | scramble((uint32_t)(3 * t + k), round);
-      } else if (near_inverted(adj, ori, t) || near_inverted(adj, ori, o)) {
+      } else if (tangled && (near_inverted(adj, ori, t) || near_inverted(adj, ori, o))) {
         // no Lawson flips next to an inverted triangle: in a tangled
         // neighbourhood they can fold two triangles onto each other; the
         // untangling flips there go first
@@ -606,9 +612,11 @@ int time_flip_mark(sm_mesher *m, int reps, double *ms) {
   SM_CHECK(reps >= 1 && reps <= 100000, SM_ERR_ARG, "time_kernel: bad repetition count");
   hipStream_t s = m->stream;
   if (!m->d_tctr) SM_HIP(hipMalloc((void **)&m->d_tctr, 8 * sizeof(int32_t)));
-  // counters of an open round (not finished, round id 0); uncertain
-  // predicates are only counted, so the launches write claims and
-  // candidates, which every flip round recomputes before use
+  // counters of an open round (not finished, round id 0, no inverted or
+  // uncertain triangle: the loop's usual state, in which k_mark does not
+  // read orientations); uncertain predicates are only counted, so the
+  // launches write claims and candidates, which every flip round
+  // recomputes before use
   int rc = fill<int32_t>(m->d_tctr, 8, 0, s);
   if (rc) return rc;
   const int64_t T = m->T;
