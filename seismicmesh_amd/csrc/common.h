@@ -98,5 +98,9 @@ __device__ inline double np_max(double a, double b) { return (a > b || a != a) ?
 size_t scan_workspace(int64_t n);
 int scan_exclusive(const int32_t *in, int32_t *out, int64_t n, int32_t *total, void *ws,
                    hipStream_t s);
+// Two independent scans of the same length in the same launches (each as
+// scan_exclusive); workspace bytes 2 * scan_workspace(n).
+int scan_exclusive2(const int32_t *in0, int32_t *out0, int32_t *total0, const int32_t *in1,
+                    int32_t *out1, int32_t *total1, int64_t n, void *ws, hipStream_t s);
 
 }  // namespace smk

@@ -337,8 +337,11 @@ int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *
   }
   // with adjacency the buckets hold no duplicates: their offsets and total
   // are the output's
-  if ((rc = scan_exclusive(rowptr, rowptr, nvert + 1, adj ? total : ctr + 1, sws, s))) return rc;
-  if (second && (rc = scan_exclusive(sl.ptr, sl.ptr, nvert + 1, nullptr, sws, s))) return rc;
+  if (second)
+    rc = scan_exclusive2(rowptr, rowptr, total, sl.ptr, sl.ptr, nullptr, nvert + 1, sws, s);
+  else
+    rc = scan_exclusive(rowptr, rowptr, nvert + 1, adj ? total : ctr + 1, sws, s);
+  if (rc) return rc;
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, B)), dim3(B), 0, s, src, rowptr, cur, col);
     SM_LAUNCH_CHECK();
