@@ -495,12 +495,16 @@ __global__ void k_copy_tri_masked(int64_t T, const int32_t *tri_src, const int32
 // snapshot buffers, but only when the flip status says the repair
 // converged to a valid triangulation (so a speculative graph iteration
 // never overwrites the last good state with a tangled one).
+// p_any (optional) receives the points whatever the status: the rollback
+// copy of a graph iteration, whose points do not move before its update.
 __global__ void k_snapshot_if_valid(const int32_t *status, int64_t T, const int32_t *tri,
                                     const int32_t *adj, int32_t *tri_s, int32_t *adj_s,
-                                    int64_t n2, const double *p, double *p_s) {
-  if (!(status[2] && status[3] == 0 && status[4] == 0)) return;
+                                    int64_t n2, const double *p, double *p_s, double *p_any) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  if (p_any)
+    for (int64_t k = i; k < n2; k += stride) p_any[k] = p[k];
+  if (!(status[2] && status[3] == 0 && status[4] == 0)) return;
   for (int64_t k = i; k < 3 * T; k += stride) {
     tri_s[k] = tri[k];
     adj_s[k] = adj[k];
@@ -508,12 +512,13 @@ __global__ void k_snapshot_if_valid(const int32_t *status, int64_t T, const int3
   for (int64_t k = i; k < n2; k += stride) p_s[k] = p[k];
 }
 
-int enqueue_snapshot(sm_mesher *m, const int32_t *status) {
+int enqueue_snapshot(sm_mesher *m, const int32_t *status, double *p_any) {
   const int64_t work = std::max<int64_t>(3 * m->T, m->NA * 2);
   unsigned g = blocks_for(work, 256);
   if (g > 2048) g = 2048;
   hipLaunchKernelGGL(k_snapshot_if_valid, dim3(g), dim3(256), 0, m->stream, status, m->T,
-                     m->d_tri, m->d_adj, m->d_tri_s, m->d_adj_s, m->NA * 2, m->d_p, m->d_pvalid);
+                     m->d_tri, m->d_adj, m->d_tri_s, m->d_adj_s, m->NA * 2, m->d_p, m->d_pvalid,
+                     p_any);
   SM_LAUNCH_CHECK();
   return SM_OK;
 }

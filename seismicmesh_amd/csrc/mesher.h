@@ -189,6 +189,10 @@ constexpr int kBlkKeep = 0, kBlkForce = 1, kBlkUpdate = 2, kBlkOrient = 3, kBlkM
 constexpr int kCtrE = 4;
 constexpr int kCtrRoundId = 16;
 constexpr int kCtrStatus = 24;
+// the 64-bit movement maximum (d_maxF) sits right before the status words,
+// so the captured iteration reads both back with one copy
+constexpr int kCtrMaxF = kCtrStatus - 2;
+static_assert(kCtrMaxF > kCtrRoundId && kCtrMaxF % 2 == 0, "counter layout");
 constexpr int kCtrRounds = 32;
 constexpr int kMaxBatchRounds = 64;
 constexpr int kCtrTotal = kCtrRounds + 8 * kMaxBatchRounds;
@@ -199,7 +203,7 @@ constexpr int kCtrScratchHost = 64;  // pinned host staging inside h_ctr
 int enqueue_flip_rounds(sm_mesher *m, int nrounds, int32_t *status, const double *pts,
                         bool exact);
 void account_flips(sm_mesher *m, const int32_t *status);
-int enqueue_snapshot(sm_mesher *m, const int32_t *status);
+int enqueue_snapshot(sm_mesher *m, const int32_t *status, double *p_any = nullptr);
 int snapshot_valid(sm_mesher *m);
 bool kinetic_repair(sm_mesher *m);
 // event-timed phases (no-ops unless profiling is enabled)

@@ -346,7 +346,7 @@ extern "C" int sm_mesher_create(const sm_mesher_params *prm, const sm_node *cons
   }
   SM_HIP(hipHostMalloc((void **)&m->h_status, 16 * sizeof(int32_t), hipHostMallocDefault));
   SM_HIP(hipMalloc(&m->d_sums, 4 * sizeof(double)));
-  SM_HIP(hipMalloc(&m->d_maxF, sizeof(uint64_t)));
+  m->d_maxF = reinterpret_cast<uint64_t *>(m->d_ctr + kCtrMaxF);  // inside d_ctr
   SM_HIP(hipHostMalloc((void **)&m->h_ctr, kCtrTotal * sizeof(int32_t), hipHostMallocDefault));
   SM_HIP(hipHostMalloc((void **)&m->h_sums, 4 * sizeof(double), hipHostMallocDefault));
   SM_HIP(hipHostMalloc((void **)&m->h_maxF, sizeof(uint64_t), hipHostMallocDefault));
@@ -446,7 +446,6 @@ extern "C" int sm_mesher_destroy(sm_mesher *m) {
   dfree(m->d_values);
   dfree(m->d_ctr);
   dfree(m->d_sums);
-  dfree(m->d_maxF);
   dfree(m->d_sg);
   dfree(m->d_tctr);
   if (m->h_ctr) (void)hipHostFree(m->h_ctr);
@@ -587,21 +586,17 @@ int capture_iteration(sm_mesher *m) {
   SM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   int rc = SM_OK;
   do {
-    if (hipMemcpyAsync(m->d_psnap, m->d_p, m->NA * m->dim * sizeof(double),
-                       hipMemcpyDeviceToDevice, s) != hipSuccess) {
-      rc = sm_set_error(SM_ERR_HIP, "capture: snapshot copy failed");
-      break;
-    }
     if ((rc = enqueue_flip_rounds(m, kGraphRounds, m->d_ctr + kCtrStatus, m->d_p, false)))
       break;
-    if ((rc = enqueue_snapshot(m, m->d_ctr + kCtrStatus))) break;
+    // the points do not move before the update: the snapshot kernel also
+    // keeps them for a rollback (d_psnap, NA x 2 doubles: graphs are 2D)
+    if ((rc = enqueue_snapshot(m, m->d_ctr + kCtrStatus, m->d_psnap))) break;
     if ((rc = centroid_keep(m))) break;
     if ((rc = build_edges(m, false))) break;
     if ((rc = forces_and_update(m))) break;
-    if (hipMemcpyAsync(m->h_status, m->d_ctr + kCtrStatus, 8 * sizeof(int32_t),
-                       hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipMemcpyAsync(m->h_maxF, m->d_maxF, sizeof(uint64_t), hipMemcpyDeviceToHost, s) !=
-            hipSuccess) {
+    // h_status[0..1]: the movement maximum, h_status[2..9]: the status
+    if (hipMemcpyAsync(m->h_status, m->d_ctr + kCtrMaxF, 10 * sizeof(int32_t),
+                       hipMemcpyDeviceToHost, s) != hipSuccess) {
       rc = sm_set_error(SM_ERR_HIP, "capture: status copy failed");
       break;
     }
@@ -632,7 +627,8 @@ int iterate_graph(sm_mesher *m) {
   }
   SM_HIP(hipGraphLaunch(m->gexec, m->stream));
   SM_HIP(hipStreamSynchronize(m->stream));
-  const int32_t *st = m->h_status;
+  const int32_t *st = m->h_status + 2;
+  std::memcpy(m->h_maxF, m->h_status, sizeof(uint64_t));
   account_flips(m, st);
   const bool valid = st[2] && st[3] == 0 && st[4] == 0;
   if (valid) {
