@@ -168,12 +168,14 @@ __global__ void k_row_sort(int64_t nrows, const int32_t *rowptr, int32_t *col, i
 // Medium buckets (kRegSort < len <= 64), one wave each: lane k holds id k
 // (INT32_MAX past the end), a bitonic network over the 64 lanes exchanges
 // partners with lane shuffles, and a ballot counts the distinct ids.
-__global__ void k_wave_sort(const int32_t *mid_rows, const int32_t *nmid, const int32_t *rowptr,
-                            int32_t *col, int32_t *ucnt) {
+// (block b of nb blocks given to this pass)
+__device__ inline void wave_sort_rows(unsigned b, unsigned nb, const int32_t *mid_rows,
+                                      const int32_t *nmid, const int32_t *rowptr, int32_t *col,
+                                      int32_t *ucnt) {
   const int lane = threadIdx.x % 64;
-  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / 64);
+  const int64_t nw = (int64_t)nb * (blockDim.x / 64);
   const int32_t n = *nmid;
-  for (int64_t wv = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / 64; wv < n; wv += nw) {
+  for (int64_t wv = ((int64_t)b * blockDim.x + threadIdx.x) / 64; wv < n; wv += nw) {
     const int32_t r = mid_rows[wv];
     const int32_t s = rowptr[r], len = rowptr[r + 1] - s;
     int32_t v = lane < len ? col[s + lane] : INT32_MAX;
@@ -201,13 +203,12 @@ __global__ void k_wave_sort(const int32_t *mid_rows, const int32_t *nmid, const 
 // INT32_MAX) for buckets up to kLdsSort ids (pass `big` = false, many
 // workgroups), or in the global staging area for longer ones (pass `big` =
 // true, launched with a single workgroup so the staging area is private).
-__global__ void k_long_sort(const int32_t *long_rows, const int32_t *nlong,
-                            const int32_t *rowptr, int32_t *col, int32_t *ucnt, int32_t *stage,
-                            int big) {
-  __shared__ int32_t sh[kLdsSort];
-  __shared__ int32_t red[256];
+__device__ inline void long_sort_rows(unsigned b, unsigned nb, const int32_t *long_rows,
+                                      const int32_t *nlong, const int32_t *rowptr, int32_t *col,
+                                      int32_t *ucnt, int32_t *stage, int big, int32_t *sh,
+                                      int32_t *red) {
   const int nl = *nlong;
-  for (int li = blockIdx.x; li < nl; li += gridDim.x) {
+  for (int li = (int)b; li < nl; li += (int)nb) {
     const int32_t r = long_rows[li];
     const int32_t s = rowptr[r], e = rowptr[r + 1];
     const int32_t len = e - s;
@@ -250,6 +251,28 @@ __global__ void k_long_sort(const int32_t *long_rows, const int32_t *nlong,
     }
     __syncthreads();
   }
+}
+
+constexpr unsigned kLongBlocks = 64;
+
+// The bucket sorts after the per-thread pass, in one launch of 256-thread
+// workgroups (the passes touch disjoint buckets): blocks [0, nwave) take
+// the one-wave buckets, the next kLongBlocks the LDS-sized long buckets,
+// the last one the buckets sorted in the global staging area (one
+// workgroup, so the staging area is private to it).
+__global__ void k_bucket_sorts(const int32_t *mid_rows, const int32_t *nmid,
+                               const int32_t *long_rows, const int32_t *nlong,
+                               const int32_t *rowptr, int32_t *col, int32_t *ucnt, int32_t *stage,
+                               unsigned nwave) {
+  __shared__ int32_t sh[kLdsSort];
+  __shared__ int32_t red[256];
+  const unsigned b = blockIdx.x;
+  if (b < nwave)
+    wave_sort_rows(b, nwave, mid_rows, nmid, rowptr, col, ucnt);
+  else if (b < nwave + kLongBlocks)
+    long_sort_rows(b - nwave, kLongBlocks, long_rows, nlong, rowptr, col, ucnt, stage, 0, sh, red);
+  else
+    long_sort_rows(0, 1, long_rows, nlong, rowptr, col, ucnt, stage, 1, sh, red);
 }
 
 __global__ void k_emit(int64_t nrows, const int32_t *rowptr, const int32_t *col,
@@ -350,16 +373,12 @@ int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *
     hipLaunchKernelGGL(k_row_sort, dim3(blocks_for(nvert, B)), dim3(B), 0, s, nvert, rowptr,
                        col, eptr, longr, ctr + 2, midr, ctr + 4);
     SM_LAUNCH_CHECK();
-    // one wave per medium bucket (grid-stride over at most nvert of them)
-    unsigned gw = blocks_for(nvert * 64, B);
+    // one wave per medium bucket (grid-stride over at most nvert of them),
+    // then the long buckets, in one launch
+    unsigned gw = blocks_for(nvert * 64, 256);
     if (gw > 8192) gw = 8192;
-    hipLaunchKernelGGL(k_wave_sort, dim3(gw), dim3(B), 0, s, midr, ctr + 4, rowptr, col, eptr);
-    SM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_long_sort, dim3(64), dim3(256), 0, s, longr, ctr + 2, rowptr, col, eptr,
-                       stage, 0);
-    SM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_long_sort, dim3(1), dim3(256), 0, s, longr, ctr + 2, rowptr, col, eptr,
-                       stage, 1);
+    hipLaunchKernelGGL(k_bucket_sorts, dim3(gw + kLongBlocks + 1), dim3(256), 0, s, midr, ctr + 4,
+                       longr, ctr + 2, rowptr, col, eptr, stage, gw);
     SM_LAUNCH_CHECK();
   }
   if (!adj) {
