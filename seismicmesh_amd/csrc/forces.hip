@@ -344,9 +344,12 @@ __global__ void k_np_total(const double *chunk_sums, int64_t nchunks, double *ou
 }
 
 template <int DIM, typename T>
+// maxF_clear (optional): the vertex update's movement maximum, cleared here
+// (this launch always precedes the update and has at least one workgroup)
 __global__ void k_edge_force(const int32_t *Ep, const int32_t *edges, const double *p,
                              const T *L, const T *h, const double *scale_ptr, BatchView bv,
-                             double L0mult, T *F) {
+                             double L0mult, T *F, uint64_t *maxF_clear) {
+  if (maxF_clear && blockIdx.x == 0 && threadIdx.x == 0) *maxF_clear = 0ull;
   const int64_t E = *Ep;
   int64_t e = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (e >= E) return;
@@ -660,7 +663,8 @@ static void launch_edge_force(sm_mesher *m, int B) {
   const unsigned g = blocks_for(edge_launch_bound(m), B);
   hipLaunchKernelGGL((k_edge_force<DIM, T>), dim3(g), dim3(B), 0, m->stream,
                      (const int32_t *)(m->d_ctr + kCtrE), m->d_edges, m->d_p, (const T *)m->d_L,
-                     (const T *)m->d_h, m->d_sums, batch_view(m), m->prm.L0mult, (T *)m->d_F);
+                     (const T *)m->d_h, m->d_sums, batch_view(m), m->prm.L0mult, (T *)m->d_F,
+                     m->d_maxF);
 }
 
 template <int DIM, typename T>
@@ -738,8 +742,8 @@ static int forces_t(sm_mesher *m) {
   launch_edge_force<DIM, T>(m, m->blk[kBlkForce]);
   SM_LAUNCH_CHECK();
   prof_end(m, 5, pb);
-  int rc = fill<uint64_t>(m->d_maxF, 1, 0ull, s);
-  if (rc) return rc;
+  // k_edge_force cleared the movement maximum
+  int rc = SM_OK;
   pb = prof_begin(m);
   launch_vertex_update<DIM, T>(m, m->blk[kBlkUpdate]);
   SM_LAUNCH_CHECK();
