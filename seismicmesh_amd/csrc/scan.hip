@@ -88,6 +88,37 @@ __global__ void k_apply(ScanJob j, int64_t n, const int32_t *offs) {
   if (j.total[y] && threadIdx.x == 0 && gridDim.x == 1) *j.total[y] = off + tot;
 }
 
+// The tile scan with the block's offset taken as the sum of the per-block
+// sums of the blocks before it, for grids of at most kTile blocks (no
+// separate scan of the sums); the last block writes each array's total.
+__global__ void k_apply_sums(ScanJob j, int64_t n, const int32_t *bsum) {
+  const int y = blockIdx.y;
+  const int32_t *bs = bsum + (int64_t)y * gridDim.x;
+  int32_t part = 0;
+  for (int k = threadIdx.x; k < (int)blockIdx.x; k += blockDim.x) part += bs[k];
+  int32_t unused;
+  const int32_t off = block_excl_scan(part, unused);
+  const int32_t *in = j.in[y];
+  int32_t *out = j.out[y];
+  int64_t base = (int64_t)blockIdx.x * kTile + (int64_t)threadIdx.x * kItems;
+  int32_t v[kItems];
+  int32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    v[k] = (base + k < n) ? in[base + k] : 0;
+    s += v[k];
+  }
+  int32_t excl;
+  int32_t tot = block_excl_scan(s, excl);
+  int32_t run = off + excl;
+#pragma unroll
+  for (int k = 0; k < kItems; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (j.total[y] && threadIdx.x == 0 && blockIdx.x == gridDim.x - 1) *j.total[y] = off + tot;
+}
+
 inline size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 int scan_run(const ScanJob &j, int count, int64_t n, void *ws, hipStream_t s) {
@@ -111,6 +142,12 @@ int scan_run(const ScanJob &j, int count, int64_t n, void *ws, hipStream_t s) {
   hipLaunchKernelGGL(k_reduce, dim3((unsigned)nb, (unsigned)count), dim3(kThreads), 0, s, j, n,
                      bsum);
   SM_LAUNCH_CHECK();
+  if (nb <= kTile) {
+    hipLaunchKernelGGL(k_apply_sums, dim3((unsigned)nb, (unsigned)count), dim3(kThreads), 0, s, j,
+                       n, (const int32_t *)bsum);
+    SM_LAUNCH_CHECK();
+    return SM_OK;
+  }
   ScanJob jb{};
   for (int y = 0; y < count; ++y) {
     jb.in[y] = bsum + y * nb;
