@@ -273,6 +273,20 @@ int sm_mesher_create(const sm_mesher_params *prm, const sm_node *const *tapes,
  * sm_mesher_run reports is in units of each domain's h0.  No counterpart
  * in the reference, which meshes one domain per call; replaces looping
  * generate_mesh (mesh_generator.py:291-529) over many domains. */
+/* Host SDF callback, for domains given as plain functions (the reference
+ * accepts domain=callable with a bbox keyword, mesh_generator.py:614-618):
+ * fills d[i] = sdf_level(x[i]) for n points x (host arrays, (n, dim) FP64);
+ * level 0 is the domain, 1.. the subdomains.  Returns 0 on success. */
+typedef int (*sm_host_sdf_fn)(void *ctx, int level, const double *x, int64_t n, double *d);
+/* From the next triangulation on, the centroid filter
+ * (_remove_triangles_outside, mesh_generator.py:734-738) and the projection
+ * of every level (_project_points_back_newton, :762-784) call `fn` on the
+ * host instead of running the handle's SDF programs: the points and cells
+ * are copied to the host, the callback runs there, and the keep flags and
+ * projected points are copied back; the rest of the iteration stays on the
+ * device.  fn = NULL switches back to the programs.  Single-domain handles
+ * without a surrogate only. */
+int sm_mesher_set_host_sdf(sm_mesher *m, sm_host_sdf_fn fn, void *ctx);
 int sm_mesher_create_batch(const sm_mesher_params *prm, int ndom, const sm_node *const *tapes,
                            const int *tape_lens, const double *h, const int32_t *nfix,
                            const int64_t *voff, int device, sm_mesher **out);

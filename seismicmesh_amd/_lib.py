@@ -81,6 +81,10 @@ HOST_SIZING_FN = ctypes.CFUNCTYPE(
     ctypes.c_int, ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int64,
     ctypes.POINTER(ctypes.c_double))
 
+HOST_SDF_FN = ctypes.CFUNCTYPE(
+    ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int64,
+    ctypes.POINTER(ctypes.c_double))
+
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
 _int = ctypes.c_int
@@ -102,6 +106,7 @@ _PROTOS = {
                                 HOST_SIZING_FN, _vp, _int, ctypes.POINTER(_vp)]),
     "sm_mesher_create_batch": (_int, [ctypes.POINTER(MesherParams), _int, _vp, _vp, _vp, _vp,
                                       _vp, _int, ctypes.POINTER(_vp)]),
+    "sm_mesher_set_host_sdf": (_int, [_vp, HOST_SDF_FN, _vp]),
     "sm_mesher_destroy": (_int, [_vp]),
     "sm_mesher_set_points": (_int, [_vp, _vp, _i64]),
     "sm_mesher_run": (_int, [_vp, _int, ctypes.POINTER(_dbl)]),

@@ -643,6 +643,13 @@ int time_flip_mark(sm_mesher *m, int reps, double *ms) {
   SM_HIP(hipEventElapsedTime(&el, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  // an uncertain incircle test in any launch sets counter 5, after which
+  // every later launch also loads the orientations (the tangled path): the
+  // figure would then not be the usual state's, so refuse it
+  int32_t tc[8];
+  SM_HIP(hipMemcpy(tc, m->d_tctr, sizeof(tc), hipMemcpyDeviceToHost));
+  SM_CHECK(tc[5] == 0, SM_ERR_STATE,
+           "time_kernel: %d uncertain predicates switched k_mark to the tangled path", tc[5]);
   *ms = (double)el / reps;
   return SM_OK;
 }

@@ -20,6 +20,7 @@
 // arithmetic bit for bit; the one transcendental scalar, (sum L^d / sum h^d)
 // ** (1/d), is taken on the host with the C library's pow as numpy does.
 #include <cmath>
+#include <vector>
 
 #include "common.h"
 #include "edges.h"
@@ -31,15 +32,18 @@ namespace smk {
 
 namespace {
 
+// A cell is real when all its vertex ids lie in [0, N): ghost cells carry
+// -1 and the far vertices of a far square sit at ids N.. (their cells are
+// never kept, whatever the SDF or a surrogate says there).
 template <int DIM, int MODE>
 __global__ void k_keep(int64_t T, const int32_t *tri, const double *p, const sm_node *tape,
-                       int ntape, double geps, uint8_t *keep) {
+                       int ntape, double geps, int32_t N, uint8_t *keep) {
   int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= T) return;
   const int32_t *c = tri + (DIM + 1) * t;
   bool real = true;
 #pragma unroll
-  for (int k = 0; k <= DIM; ++k) real &= c[k] >= 0;
+  for (int k = 0; k <= DIM; ++k) real &= (uint32_t)c[k] < (uint32_t)N;
   if (!real) {
     keep[t] = 0;
     return;
@@ -60,11 +64,12 @@ __global__ void k_keep(int64_t T, const int32_t *tri, const double *p, const sm_
 // centroid is tested against that domain's program and geps.
 template <int MODE>
 __global__ void k_keep_batch(int64_t T, const int32_t *tri, const double *p, BatchView bv,
-                             uint8_t *keep) {
+                             int32_t N, uint8_t *keep) {
   int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= T) return;
   const int32_t *c = tri + 3 * t;
-  if (c[0] < 0 || c[1] < 0 || c[2] < 0) {
+  if ((uint32_t)c[0] >= (uint32_t)N || (uint32_t)c[1] >= (uint32_t)N ||
+      (uint32_t)c[2] >= (uint32_t)N) {
     keep[t] = 0;
     return;
   }
@@ -484,13 +489,13 @@ __device__ inline void record_error(uint64_t *ctr, double dn, double d) {
 template <int DIM, int MODE>
 __global__ void k_keep_surrogate(int64_t T, const int32_t *tri, const double *p,
                                  const sm_node *tape, int ntape, double geps, const float *dnn,
-                                 double band, uint8_t *keep, uint64_t *sg) {
+                                 double band, int32_t N, uint8_t *keep, uint64_t *sg) {
   const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= T) return;
   const int32_t *c = tri + (DIM + 1) * t;
   bool real = true;
 #pragma unroll
-  for (int k = 0; k <= DIM; ++k) real &= c[k] >= 0;
+  for (int k = 0; k <= DIM; ++k) real &= (uint32_t)c[k] < (uint32_t)N;
   const double dn = real ? (double)dnn[t] : 0.0;
   const bool exact = real && !(fabs(dn) >= band);
   wave_count(sg + 0, real && !exact);
@@ -544,7 +549,7 @@ static void launch_keep(sm_mesher *m) {
   const int B = m->blk[kBlkKeep];
   hipLaunchKernelGGL((k_keep<DIM, MODE>), dim3(blocks_for(m->T, B)), dim3(B), 0, m->stream,
                      m->T, m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0], m->prm.geps,
-                     m->d_keep);
+                     (int32_t)m->N, m->d_keep);
 }
 
 template <int DIM, int MODE>
@@ -552,7 +557,7 @@ static void launch_keep_surrogate(sm_mesher *m) {
   const int B = m->blk[kBlkKeep];
   hipLaunchKernelGGL((k_keep_surrogate<DIM, MODE>), dim3(blocks_for(m->T, B)), dim3(B), 0,
                      m->stream, m->T, m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0],
-                     m->prm.geps, m->d_dnn, m->band, m->d_keep, m->d_sg);
+                     m->prm.geps, m->d_dnn, m->band, (int32_t)m->N, m->d_keep, m->d_sg);
 }
 
 template <int DIM, int MODE>
@@ -567,16 +572,114 @@ static int project_surrogate(sm_mesher *m) {
   return SM_OK;
 }
 
+// Domains given as host functions (sm_mesher_set_host_sdf).  The centroid
+// filter (mesh_generator.py:734-738): centroids summed in the cell's vertex
+// order and divided by dim + 1, as k_keep does; keep d < -geps.
+static int host_keep(sm_mesher *m) {
+  const int dim = m->dim, nv = dim + 1;
+  const int64_t T = m->T, N = m->N;
+  std::vector<int32_t> tri((size_t)T * nv);
+  std::vector<double> p((size_t)N * dim);
+  SM_HIP(hipMemcpyAsync(tri.data(), m->d_tri, tri.size() * sizeof(int32_t),
+                        hipMemcpyDeviceToHost, m->stream));
+  SM_HIP(hipMemcpyAsync(p.data(), m->d_p, p.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        m->stream));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  std::vector<int64_t> ids;
+  std::vector<double> x;
+  ids.reserve((size_t)T);
+  x.reserve((size_t)T * dim);
+  for (int64_t t = 0; t < T; ++t) {
+    const int32_t *c = tri.data() + nv * t;
+    bool real = true;
+    for (int k = 0; k < nv; ++k) real &= (uint32_t)c[k] < (uint32_t)N;
+    if (!real) continue;
+    ids.push_back(t);
+    for (int d = 0; d < dim; ++d) {
+      double acc = p[(size_t)c[0] * dim + d];
+      for (int k = 1; k < nv; ++k) acc = acc + p[(size_t)c[k] * dim + d];
+      x.push_back(acc / (double)nv);
+    }
+  }
+  std::vector<double> dv(ids.size());
+  if (!ids.empty()) {
+    const int rc = m->host_sdf(m->host_sdf_ctx, 0, x.data(), (int64_t)ids.size(), dv.data());
+    SM_CHECK(rc == 0, SM_ERR_ARG, "host SDF callback failed (%d)", rc);
+  }
+  std::vector<uint8_t> keep((size_t)T, 0);
+  for (size_t k = 0; k < ids.size(); ++k) keep[(size_t)ids[k]] = dv[k] < -m->prm.geps;
+  SM_HIP(hipMemcpyAsync(m->d_keep, keep.data(), (size_t)T, hipMemcpyHostToDevice, m->stream));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  return SM_OK;
+}
+
+// _project_points_back_newton (mesh_generator.py:762-784) for each level in
+// turn, with the host function: level 0 projects d > 0, a subdomain level
+// 0 < d < h0 / 1.5; forward-difference gradient with step deps, |g|^2
+// clamped to deps, x -= (d g_i) / |g|^2 (the update kernel's arithmetic).
+static int host_project(sm_mesher *m) {
+  const int dim = m->dim;
+  const int64_t N = m->N;
+  const double deps = m->prm.deps, hmin = m->prm.h0;
+  std::vector<double> p((size_t)N * dim);
+  SM_HIP(hipMemcpyAsync(p.data(), m->d_p, p.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        m->stream));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  std::vector<double> d((size_t)N), y, dy, g;
+  std::vector<int64_t> ix;
+  for (int l = 0; l < m->prm.nlevels; ++l) {
+    int rc = m->host_sdf(m->host_sdf_ctx, l, p.data(), N, d.data());
+    SM_CHECK(rc == 0, SM_ERR_ARG, "host SDF callback failed (%d)", rc);
+    ix.clear();
+    for (int64_t v = 0; v < N; ++v) {
+      const double dv = d[(size_t)v];
+      if (l == 0 ? dv > 0.0 : (dv > 0.0 && dv < hmin / 1.5)) ix.push_back(v);
+    }
+    const int64_t n = (int64_t)ix.size();
+    if (n == 0) continue;
+    y.assign((size_t)n * dim, 0.0);
+    dy.assign((size_t)n, 0.0);
+    g.assign((size_t)n * dim, 0.0);
+    for (int i = 0; i < dim; ++i) {
+      for (int64_t k = 0; k < n; ++k)
+        for (int c = 0; c < dim; ++c)
+          y[(size_t)k * dim + c] = p[(size_t)ix[k] * dim + c] + (c == i ? deps : 0.0);
+      rc = m->host_sdf(m->host_sdf_ctx, l, y.data(), n, dy.data());
+      SM_CHECK(rc == 0, SM_ERR_ARG, "host SDF callback failed (%d)", rc);
+      for (int64_t k = 0; k < n; ++k)
+        g[(size_t)k * dim + i] = (dy[(size_t)k] - d[(size_t)ix[k]]) / deps;
+    }
+    for (int64_t k = 0; k < n; ++k) {
+      const double *gk = g.data() + (size_t)k * dim;
+      double g2 = gk[0] * gk[0];
+      for (int i = 1; i < dim; ++i) g2 = g2 + gk[i] * gk[i];
+      if (g2 < deps) g2 = deps;
+      const double dk = d[(size_t)ix[k]];
+      for (int i = 0; i < dim; ++i) {
+        double &pv = p[(size_t)ix[k] * dim + i];
+        pv = pv - (dk * gk[i]) / g2;
+      }
+    }
+  }
+  SM_HIP(hipMemcpyAsync(m->d_p, p.data(), p.size() * sizeof(double), hipMemcpyHostToDevice,
+                        m->stream));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  return SM_OK;
+}
+
 int centroid_keep(sm_mesher *m) {
   if (m->T == 0) return SM_OK;
+  if (m->host_sdf) return host_keep(m);
   if (m->ndom > 0) {
     const int B = m->blk[kBlkKeep];
     if (m->deep_sdf)
       hipLaunchKernelGGL((k_keep_batch<kSdfDeep>), dim3(blocks_for(m->T, B)), dim3(B), 0,
-                         m->stream, m->T, m->d_tri, m->d_p, batch_view(m), m->d_keep);
+                         m->stream, m->T, m->d_tri, m->d_p, batch_view(m), (int32_t)m->N,
+                         m->d_keep);
     else
       hipLaunchKernelGGL((k_keep_batch<kSdfReg>), dim3(blocks_for(m->T, B)), dim3(B), 0,
-                         m->stream, m->T, m->d_tri, m->d_p, batch_view(m), m->d_keep);
+                         m->stream, m->T, m->d_tri, m->d_p, batch_view(m), (int32_t)m->N,
+                         m->d_keep);
     SM_LAUNCH_CHECK();
     return SM_OK;
   }
@@ -676,13 +779,13 @@ static void launch_vertex_update(sm_mesher *m, int B) {
                        m->stream, N, m->d_p, eptr_of(m), m->d_sptr, m->d_slist,
                        (const T *)m->d_F, m->prm.nfix, m->prm.delta_t, m->d_tape_all,
                        m->d_tape_off_dev, m->d_tape_len_dev, m->prm.nlevels, m->prm.deps,
-                       m->prm.h0, bv, m->d_maxF, m->surrogate ? 0 : 1);
+                       m->prm.h0, bv, m->d_maxF, (m->surrogate || m->host_sdf) ? 0 : 1);
   else
     hipLaunchKernelGGL((k_vertex_update<DIM, T, kSdfReg>), dim3(blocks_for(N, B)), dim3(B), 0,
                        m->stream, N, m->d_p, eptr_of(m), m->d_sptr, m->d_slist,
                        (const T *)m->d_F, m->prm.nfix, m->prm.delta_t, m->d_tape_all,
                        m->d_tape_off_dev, m->d_tape_len_dev, m->prm.nlevels, m->prm.deps,
-                       m->prm.h0, bv, m->d_maxF, m->surrogate ? 0 : 1);
+                       m->prm.h0, bv, m->d_maxF, (m->surrogate || m->host_sdf) ? 0 : 1);
 }
 
 template <int DIM, typename T>
@@ -751,6 +854,7 @@ static int forces_t(sm_mesher *m) {
     rc = m->deep_sdf ? project_surrogate<DIM, kSdfDeep>(m) : project_surrogate<DIM, kSdfReg>(m);
     if (rc) return rc;
   }
+  if (m->host_sdf && (rc = host_project(m))) return rc;
   prof_end(m, 6, pb);
   return SM_OK;
 }
@@ -770,19 +874,21 @@ static void launch_tunable(sm_mesher *m, int which, int B) {
   } else if (m->ndom > 0) {
     if (m->deep_sdf)
       hipLaunchKernelGGL((k_keep_batch<kSdfDeep>), dim3(blocks_for(m->T, B)), dim3(B), 0,
-                         m->stream, m->T, m->d_tri, m->d_p, batch_view(m), m->d_keep);
+                         m->stream, m->T, m->d_tri, m->d_p, batch_view(m), (int32_t)m->N,
+                         m->d_keep);
     else
       hipLaunchKernelGGL((k_keep_batch<kSdfReg>), dim3(blocks_for(m->T, B)), dim3(B), 0,
-                         m->stream, m->T, m->d_tri, m->d_p, batch_view(m), m->d_keep);
+                         m->stream, m->T, m->d_tri, m->d_p, batch_view(m), (int32_t)m->N,
+                         m->d_keep);
   } else {
     if (m->deep_sdf)
       hipLaunchKernelGGL((k_keep<DIM, kSdfDeep>), dim3(blocks_for(m->T, B)), dim3(B), 0,
                          m->stream, m->T, m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0],
-                         m->prm.geps, m->d_keep);
+                         m->prm.geps, (int32_t)m->N, m->d_keep);
     else
       hipLaunchKernelGGL((k_keep<DIM, kSdfReg>), dim3(blocks_for(m->T, B)), dim3(B), 0,
                          m->stream, m->T, m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0],
-                         m->prm.geps, m->d_keep);
+                         m->prm.geps, (int32_t)m->N, m->d_keep);
   }
 }
 

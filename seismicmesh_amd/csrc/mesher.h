@@ -49,6 +49,11 @@ struct sm_mesher {
   void *host_sizing_ctx = nullptr;
   double *d_mid = nullptr, *d_hext = nullptr;
 
+  // host SDF callback (domains given as Python functions): the centroid
+  // filter and the projection run on the host through it (forces.hip)
+  sm_host_sdf_fn host_sdf = nullptr;
+  void *host_sdf_ctx = nullptr;
+
   // sizing grid (device copies of axes and values)
   smk::GridView grid{};
   double *d_axis[3] = {nullptr, nullptr, nullptr};

@@ -659,14 +659,24 @@ extern "C" int sm_mesher_run(sm_mesher *m, int iters, double *maxdp_out) {
   SM_HIP(hipSetDevice(m->device));
   // the graph path needs a device-only iteration: FP32 forces, sizing on
   // the device, and no per-kernel event timing
-  const bool graph = m->dim == 2 && m->prm.precision == 32 && !m->host_sizing && !m->prof &&
-                     m->have_tri && m->use_graph;
+  const bool graph = m->dim == 2 && m->prm.precision == 32 && !m->host_sizing &&
+                     !m->host_sdf && !m->prof && m->have_tri && m->use_graph;
   for (int it = 0; it < iters; ++it) {
     int rc = graph ? iterate_graph(m) : iterate_plain(m);
     if (rc) return rc;
   }
   if (maxdp_out) *maxdp_out = m->last_maxdp;
   return prof_drain(m);
+}
+
+extern "C" int sm_mesher_set_host_sdf(sm_mesher *m, sm_host_sdf_fn fn, void *ctx) {
+  SM_CHECK(m, SM_ERR_ARG, "mesher_set_host_sdf: null handle");
+  SM_CHECK(m->ndom == 0, SM_ERR_STATE, "mesher_set_host_sdf: not for batch handles");
+  SM_CHECK(!fn || !m->surrogate, SM_ERR_STATE,
+           "mesher_set_host_sdf: a surrogate is set (it needs the device programs)");
+  m->host_sdf = fn;
+  m->host_sdf_ctx = fn ? ctx : nullptr;
+  return SM_OK;
 }
 
 extern "C" int sm_mesher_set_graph(sm_mesher *m, int enable) {
@@ -750,6 +760,8 @@ extern "C" int sm_mesher_set_surrogate(sm_mesher *m, const sm_mlp *net, double b
     SM_CHECK(net->in_dim == m->dim, SM_ERR_ARG, "mesher_set_surrogate: network takes %dD points",
              net->in_dim);
     SM_CHECK(band > 0.0, SM_ERR_ARG, "mesher_set_surrogate: band must be positive");
+    SM_CHECK(!m->host_sdf, SM_ERR_STATE,
+             "mesher_set_surrogate: the handle's SDF is a host function");
     SM_CHECK(m->ndom == 0 && m->prm.nlevels == 1 && !m->host_sizing, SM_ERR_STATE,
              "mesher_set_surrogate: single-domain handles without subdomains only");
     m->net = *net;
@@ -792,6 +804,8 @@ extern "C" int sm_mesher_autotune(sm_mesher *m, int reps, int32_t *chosen, doubl
   SM_CHECK(m->have_tri && m->stats[0] > 0, SM_ERR_STATE,
            "mesher_autotune: run at least one iteration first (edges and forces are timed "
            "on the current state)");
+  SM_CHECK(!m->host_sdf, SM_ERR_STATE,
+           "mesher_autotune: the centroid filter of a host-function domain runs on the host");
   SM_HIP(hipSetDevice(m->device));
   SM_HIP(hipStreamSynchronize(m->stream));
   static const int cand[] = {64, 128, 256, 512, 1024};

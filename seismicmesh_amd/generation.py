@@ -54,13 +54,32 @@ _DOMAINS = (geo.Ball, geo.Rectangle, geo.Disk, geo.Cube, geo.Cylinder, geo.Torus
             geo.Union, geo.Intersection, geo.Difference, geo.Repeat)
 
 
+class HostDomain:
+    """A domain given as a plain function sdf(x: (n, dim)) -> (n,) with a
+    bbox keyword (mesh_generator.py:614-618).  The device cannot run Python:
+    the force loop's centroid filter and projection call it on the host
+    (sm_mesher_set_host_sdf); everything else stays on the device."""
+
+    def __init__(self, fn, bbox):
+        self.fn = fn
+        self.bbox = bbox
+        self.corners = None
+        self.dim = len(bbox) // 2 if bbox is not None else None
+
+    def eval(self, x):
+        return np.asarray(self.fn(np.asarray(x, dtype=float)), dtype=float).reshape(-1)
+
+
 def _unpack_domain(domain, opts):
     if isinstance(domain, _DOMAINS):
         bbox, corners = domain.bbox, domain.corners
+    elif isinstance(domain, HostDomain):
+        bbox, corners = domain.bbox, None
     elif callable(domain):
-        raise NotImplementedError(
-            "domains given as plain Python functions cannot run on the GPU; build the "
-            "domain from seismicmesh_amd geometry objects (Disk, Rectangle, Union, ...)")
+        # mesh_generator.py:614-618: the bbox comes from the keywords
+        bbox, corners = opts.get("bbox"), None
+        _check_bbox(bbox)
+        return HostDomain(domain, bbox), bbox, corners
     else:
         raise ValueError("`domain` must be a function or a :class:`geometry` object")
     _check_bbox(bbox)
@@ -131,7 +150,10 @@ class Mesher:
                  dim=2, far_square=False):
         _lib.require_gpu()
         self.dim = dim
-        self._tapes = [geo.tape_array(l.tape()) for l in levels]
+        host = any(isinstance(l, HostDomain) for l in levels)
+        # a host-function level has no program: the handle gets a placeholder
+        # (never run) and the callback below evaluates every level
+        self._tapes = [geo.tape_array(_placeholder(dim) if host else l.tape()) for l in levels]
         ptrs = (ctypes.c_void_p * len(levels))(*[ctypes.addressof(t) for t in self._tapes])
         lens = (ctypes.c_int * len(levels))(*[len(t) for t in self._tapes])
         prm = _lib.MesherParams()
@@ -169,6 +191,19 @@ class Mesher:
         _lib.check(_lib.lib.sm_mesher_create(ctypes.byref(prm), ptrs, lens, ctypes.byref(g), cb,
                                              None, device, ctypes.byref(h)), "sm_mesher_create")
         self.h = h
+        self._sdf_cb = None
+        if host:
+            def _sdf(ctx, level, x, n, d):
+                try:
+                    xs = np.ctypeslib.as_array(x, shape=(n, dim)).copy()
+                    out = np.ctypeslib.as_array(d, shape=(n,))
+                    out[:] = levels[level].eval(xs)
+                    return 0
+                except Exception:  # reported by the library as a failed callback
+                    return 1
+            self._sdf_cb = _lib.HOST_SDF_FN(_sdf)
+            _lib.check(_lib.lib.sm_mesher_set_host_sdf(self.h, self._sdf_cb, None),
+                       "sm_mesher_set_host_sdf")
 
     def close(self):
         h = getattr(self, "h", None)
@@ -298,6 +333,10 @@ class Mesher:
         keys = ("iterations", "flip_rounds", "flips", "host_rebuilds", "exact_predicates",
                 "rollbacks", "kinetic_repairs", "unresolved", "relocations")
         return {k: int(s[i]) for i, k in enumerate(keys)}
+
+
+def _placeholder(dim):
+    return (geo.Disk([0.0, 0.0], 1.0) if dim == 2 else geo.Ball([0.0, 0.0, 0.0], 1.0)).tape()
 
 
 def _fd(domain):

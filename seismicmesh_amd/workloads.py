@@ -12,7 +12,7 @@
 """
 import numpy as np
 
-from . import geometry as geo
+from . import geometry as _geo
 from .sizing import get_sizing_function_from_segy
 
 BP_BBOX = (-12000.0, 0.0, 0.0, 67000.0)
@@ -51,19 +51,22 @@ def bp2004(scale=1.0, seed=2004, quiet=True):
             ef = get_sizing_function_from_segy(None, BP_BBOX, **kw)
     else:
         ef = get_sizing_function_from_segy(None, BP_BBOX, **kw)
-    domain = geo.Rectangle(BP_BBOX)
+    domain = _geo.Rectangle(BP_BBOX)
     return domain, ef, {"grid": shape, "bbox": BP_BBOX, "hmin": 75.0, "vp": vp}
 
 
 def disk(h=0.02):
-    return geo.Disk([0.0, 0.0], 1.0), h, {"bbox": (-1.0, 1.0, -1.0, 1.0)}
+    return _geo.Disk([0.0, 0.0], 1.0), h, {"bbox": (-1.0, 1.0, -1.0, 1.0)}
 
 
-def batch_domain(i, seed0=1000):
+def batch_domain(i, seed0=1000, geom=None):
     """Domain i of the many-domains workload: a seeded CSG of 1-6 primitives
     in [-2, 2]^2 -- disks with r ~ U[0.2, 1] and axis-aligned rectangles with
     sides ~ U[0.3, 2] -- the first kept, each next one added (union) or cut
-    out (difference)."""
+    out (difference).  ``geom`` is the module whose classes build it (this
+    package's geometry by default; any module with the reference's
+    geometry API)."""
+    geo = geom or _geo
     rng = np.random.default_rng(seed0 + i)
     k = int(rng.integers(1, 7))
     prims = []
@@ -92,17 +95,18 @@ def domain_area(dom, n=400):
     return float((d < 0).mean() * 16.0)
 
 
-def many_domains(n=512, seed0=1000, nmin=500, nmax=2.0e5):
+def many_domains(n=512, seed0=1000, nmin=500, nmax=2.0e5, geom=None):
     """The many-domains workload: n seeded CSG domains with uniform sizes
     h = sqrt(area / N_i), N_i log-uniform in [nmin, nmax].  A domain whose
     cut-outs leave less than 0.1 of area keeps its first primitive only."""
+    geo = geom or _geo
     domains, hs = [], []
     for i in range(n):
-        dom, rng = batch_domain(i, seed0)
+        dom, rng = batch_domain(i, seed0, geom)
         target = float(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
         area = domain_area(dom)
         if area < 0.1:
-            dom, rng2 = batch_domain(i, seed0)
+            dom, rng2 = batch_domain(i, seed0, geom)
             while isinstance(dom, (geo.Union, geo.Difference)):
                 dom = dom.domains[0]
             area = domain_area(dom)
@@ -111,12 +115,14 @@ def many_domains(n=512, seed0=1000, nmin=500, nmax=2.0e5):
     return domains, hs
 
 
-def csg3d(seed=3, nballs=48, nboxes=16, ncyl=8, ntori=4):
+def csg3d(seed=3, nballs=48, nboxes=16, ncyl=8, ntori=4, geom=None):
     """The many-primitive 3D CSG: ((nballs balls U nboxes boxes) minus ncyl
     cylinders) U ntori tori, all inside [-1, 1]^3, seeded.  Balls have
     r ~ U[0.05, 0.3]; boxes sides ~ U[0.1, 0.5]; cylinders (axis z)
     r ~ U[0.05, 0.15], length ~ U[0.5, 1.5]; tori r1 ~ U[0.15, 0.35],
-    r2 ~ U[0.03, 0.08].  Returns (domain, bbox)."""
+    r2 ~ U[0.03, 0.08].  Returns (domain, bbox); ``geom`` as in
+    batch_domain."""
+    geo = geom or _geo
     rng = np.random.default_rng(seed)
     solids = []
     for _ in range(nballs):
@@ -194,4 +200,4 @@ def eage3d(scale=1.0, seed=4, quiet=True):
             ef = get_sizing_function_from_segy(None, EAGE_BBOX, **kw)
     else:
         ef = get_sizing_function_from_segy(None, EAGE_BBOX, **kw)
-    return geo.Cube(EAGE_BBOX), ef, {"grid": shape, "bbox": EAGE_BBOX, "hmin": 150.0}
+    return _geo.Cube(EAGE_BBOX), ef, {"grid": shape, "bbox": EAGE_BBOX, "hmin": 150.0}

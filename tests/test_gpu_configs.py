@@ -1,0 +1,241 @@
+"""Every benchmark workload on the GPU against the oracle.
+
+* BP2004-shaped 2D (1911 x 5395 grid) at full size, end to end: vertex count
+  within 1 % and min / mean cell quality within 0.01 of the oracle's mesh.
+* Many independent 2D CSG domains in one batch handle: per domain, the kept
+  cells and sorted unique edges equal the oracle's Delaunay + filter on the
+  same (jittered) points, and each FP32 update is within 1e-5 of the
+  oracle's FP64 update, over two iterations.
+* The 76-primitive 3D CSG at h = 0.08: the same per-iteration checks, and the
+  surrogate-steered loop gives the exact loop's points.
+* The EAGE-shaped 3D workload (grid coarsened 10x, same physical sizes):
+  one iteration's cells, edges and update.
+
+Inputs that are Delaunay-degenerate (the initial lattices) are jittered by
+1e-6 h so that the triangulation being compared is unique."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu():
+    from seismicmesh_amd import _lib
+    _lib.require_gpu()
+
+
+def _jitter(p, nfix, h, seed):
+    rng = np.random.default_rng(seed)
+    q = p.copy()
+    q[nfix:] += rng.uniform(-1e-6, 1e-6, q[nfix:].shape) * h
+    return q
+
+
+def _cellset(t):
+    return {tuple(sorted(c)) for c in np.asarray(t).tolist()}
+
+
+def _oracle_state(domain, h, p):
+    from oracle import distmesh as odm
+    opts = dict(odm.GEN_DEFAULTS)
+    opts.update(verbose=0, points=p)
+    return odm.setup(domain, h, opts)
+
+
+def _check_iterations(m, s, p, nfix, iters, what):
+    """Runs `iters` iterations of handle m from points p, checking kept cells,
+    edges and the update against the oracle state s each time."""
+    from oracle import distmesh as odm
+    for it in range(iters):
+        t_all = odm.delaunay(p)
+        m.triangulate()
+        t = odm.remove_triangles_outside(p, t_all, s["fd"], s["geps"])
+        assert _cellset(m.cells()) == _cellset(t), "%s: cells, iteration %d" % (what, it)
+        assert np.array_equal(m.edges(), odm.get_edges(t)), "%s: edges, iteration %d" % (what, it)
+        m.run(1)
+        q_ref, _, _ = odm.force_step(p, t_all, s, list(range(nfix)))
+        q = m.points()
+        dp, dp_ref = q - p, q_ref - p
+        scale = np.abs(dp_ref).max()
+        assert scale > 0
+        err = np.abs(dp - dp_ref).max() / scale
+        print("%s iteration %d: %d points, %d cells, update error %.2e of the largest move"
+              % (what, it, len(p), len(t), err))
+        assert err <= 1e-5, "%s: update, iteration %d" % (what, it)
+        p = q
+
+
+def test_bp2004_full_size_end_to_end_vs_oracle():
+    import seismicmesh_amd as sm
+    from oracle import distmesh as odm
+    from oracle import geometry as og
+    from oracle.sizing import SizeFunction, build_interpolant
+    from seismicmesh_amd import mesh_utils as mu, workloads
+    domain, ef, info = workloads.bp2004()
+    grid = ef.cell_size
+    p, t = sm.generate_mesh(domain, ef, verbose=0, precision=32)
+    oef = SizeFunction(info["bbox"], build_interpolant(grid.values.astype(np.float64), grid.bbox),
+                       ef.hmin)
+    q, s = odm.generate_mesh(og.Rectangle(info["bbox"]), oef, verbose=0)
+    qa, qb = mu.simp_qual(p, t), og.simp_qual(q, s)
+    print("BP2004 end to end: device %d vertices, %d cells, quality min %.4f mean %.4f; "
+          "oracle %d vertices, %d cells, min %.4f mean %.4f"
+          % (len(p), len(t), qa.min(), qa.mean(), len(q), len(s), qb.min(), qb.mean()))
+    assert abs(len(p) - len(q)) <= 0.01 * len(q)
+    assert abs(qa.min() - qb.min()) <= 0.01
+    assert abs(qa.mean() - qb.mean()) <= 0.01
+
+
+def test_many_domains_batch_vs_oracle():
+    from oracle import distmesh as odm
+    from oracle import geometry as og
+    from seismicmesh_amd import batch, workloads
+    doms, hs = workloads.many_domains(n=6, nmax=2.0e4)
+    odoms, ohs = workloads.many_domains(n=6, nmax=2.0e4, geom=og)
+    assert hs == ohs
+    sts = batch.prepare_batch(doms, hs)
+    for k, st in enumerate(sts):
+        st["p"] = _jitter(st["p"], st["nfix"], st["h0"], 100 + k)
+    bm = batch.make_batch_mesher(sts)
+    p0 = np.vstack([st["p"] for st in sts])
+    bm.set_points(p0)
+    ps = [st["p"] for st in sts]
+    for it in range(2):
+        bm.triangulate()
+        parts = bm.split(bm.points(), bm.cells())
+        be = bm.edges()
+        for d, (st, od) in enumerate(zip(sts, odoms)):
+            s = _oracle_state(od, hs[d], ps[d])
+            t = odm.remove_triangles_outside(ps[d], odm.delaunay(ps[d]), s["fd"], s["geps"])
+            assert _cellset(parts[d][1]) == _cellset(t), "domain %d, iteration %d" % (d, it)
+            b, e = bm.voff[d], bm.voff[d + 1]
+            mine = be[(be[:, 0] >= b) & (be[:, 0] < e)] - b
+            assert np.array_equal(mine, odm.get_edges(t)), "domain %d, iteration %d" % (d, it)
+        bm.run(1)
+        pb = bm.points()
+        for d, (st, od) in enumerate(zip(sts, odoms)):
+            s = _oracle_state(od, hs[d], ps[d])
+            q_ref, _, _ = odm.force_step(ps[d], odm.delaunay(ps[d]), s, list(range(st["nfix"])))
+            b, e = bm.voff[d], bm.voff[d + 1]
+            dp, dp_ref = pb[b:e] - ps[d], q_ref - ps[d]
+            scale = np.abs(dp_ref).max()
+            assert scale > 0
+            err = np.abs(dp - dp_ref).max() / scale
+            print("batch domain %d (%d points) iteration %d: update error %.2e"
+                  % (d, len(ps[d]), it, err))
+            assert err <= 1e-5, "domain %d, iteration %d" % (d, it)
+            ps[d] = pb[b:e].copy()
+    bm.close()
+
+
+def test_csg3d_reduced_vs_oracle():
+    from oracle import geometry as og
+    from seismicmesh_amd import generation, workloads
+    dom, _ = workloads.csg3d()
+    odom, _ = workloads.csg3d(geom=og)
+    h = 0.08
+    st = generation.prepare(dom, h, verbose=0, precision=32)
+    p0 = _jitter(st["p"], st["nfix"], h, 3)
+    st["p"] = p0
+    m = generation.make_mesher(st)
+    m.set_points(p0)
+    s = _oracle_state(odom, h, p0)
+    _check_iterations(m, s, p0, st["nfix"], 2, "csg3d h=0.08")
+    m.close()
+
+
+def test_csg3d_surrogate_matches_exact():
+    """The surrogate-steered 3D loop on the 76-primitive CSG gives the exact
+    loop's points while skipping exact SDF evaluations."""
+    from seismicmesh_amd import generation, workloads
+    from seismicmesh_amd.neural import NeuralSDF
+    dom, bbox = workloads.csg3d()
+    h = 0.08
+    net = NeuralSDF(3, bbox, hidden=3, seed=0, omega=8.0)
+    net.train(dom, nsamples=200000, steps=1500, batch=4096, lr=2e-3, lr_final=1e-4, eik=0.1,
+              near=0.05, seed=0, trunc=0.3)
+    band, err = net.calibrate_band(dom, 2.0 * h, nval=100000)
+    st = generation.prepare(dom, h, verbose=0, precision=32)
+    a, b = generation.make_mesher(st), generation.make_mesher(st)
+    b.set_surrogate(net, band)
+    a.set_points(st["p"])
+    b.set_points(st["p"])
+    for _ in range(3):
+        a.run(1)
+        b.run(1)
+        assert np.array_equal(a.points(), b.points())
+    sg = b.surrogate_stats()
+    frac = sg["skipped"] / (sg["skipped"] + sg["exact"])
+    print("csg3d surrogate: max error on the check %.4f, band %.4f: %.1f%% of exact "
+          "evaluations skipped, max |d_nn - d| in the band %.4f"
+          % (err, band, 100 * frac, sg["max_err"]))
+    assert sg["max_err"] < band
+    assert frac > 0.0
+    a.close()
+    b.close()
+
+
+def test_eage3d_reduced_vs_oracle():
+    from oracle import geometry as og
+    from oracle.sizing import build_interpolant
+    from seismicmesh_amd import generation, workloads
+    domain, ef, info = workloads.eage3d(scale=0.1)
+    st = generation.prepare(domain, ef, verbose=0, precision=32)
+    p0 = _jitter(st["p"], st["nfix"], st["h0"], 4)
+    st["p"] = p0
+    m = generation.make_mesher(st)
+    m.set_points(p0)
+    grid = ef.cell_size
+    fh = build_interpolant(grid.values.astype(np.float64), grid.bbox)
+    fd = og.Cube(info["bbox"]).eval
+    s = dict(fd=fd, fh=fh, h0=st["h0"], dim=3, L0mult=1 + 0.4 / 4, geps=st["geps"],
+             deps=st["deps"], pfix=p0[:st["nfix"]], levels=[fd], delta_t=0.3)
+    _check_iterations(m, s, p0, st["nfix"], 1, "eage3d scale=0.1")
+    m.close()
+
+
+def test_far_square_with_surrogate_keeps_no_far_cell():
+    """With a far square, the cells of the four far vertices are never kept,
+    even when a surrogate (here an untrained network, whose sign far outside
+    its box is arbitrary) decides the centroid test there."""
+    from seismicmesh_amd import generation, geometry as geo
+    from seismicmesh_amd.neural import NeuralSDF
+    dom = geo.Disk([0.0, 0.0], 1.0)
+    h = 0.1
+    net = NeuralSDF(2, (-1.0, 1.0, -1.0, 1.0), hidden=3, seed=5, omega=8.0)
+    st = generation.prepare(dom, h, verbose=0, precision=32)
+    m = generation.make_mesher(st, far_square=True)
+    m.set_surrogate(net, 1e-9)
+    m.set_points(st["p"])
+    for _ in range(3):
+        m.run(1)
+        m.triangulate()
+        c = m.cells()
+        assert len(c) > 0 and c.max() < len(st["p"]) and c.min() >= 0
+    m.close()
+
+
+def test_unique_edges_recursive_scan():
+    """More than 1,048,576 vertices: the offset scans take the recursive
+    path (over 1,024 tiles)."""
+    import torch
+    from oracle import geometry as og
+    from seismicmesh_amd import _lib
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(9)
+    nvert = 1_200_000
+    pairs = rng.integers(0, nvert, (3_000_000, 2)).astype(np.int32)
+    want = og.unique_edges(pairs)
+    pt = torch.from_numpy(pairs).to(dev)
+    out = torch.empty((len(pairs), 2), dtype=torch.int32, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    wsb = _lib.lib.sm_unique_edges_workspace(len(pairs), nvert)
+    ws = torch.empty(int(wsb), dtype=torch.uint8, device=dev)
+    _lib.check(_lib.lib.sm_unique_edges(pt.data_ptr(), len(pairs), nvert, out.data_ptr(),
+                                        cnt.data_ptr(), ws.data_ptr(), int(wsb), None),
+               "sm_unique_edges")
+    torch.cuda.synchronize()
+    n = int(cnt.item())
+    assert n == len(want)
+    assert np.array_equal(out[:n].cpu().numpy(), want)
