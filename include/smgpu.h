@@ -318,8 +318,20 @@ int sm_mesher_get_edges(sm_mesher *m, int32_t *edges, int64_t cap);
  * had not converged, [6] kinetic repairs (points moved in sub-steps from the
  * last valid triangulation), [7] predicates the device's exact stage could
  * not settle (each forces a host rebuild), [8] vertices taken out and put
- * back on the host to untangle what flips could not. */
+ * back on the host to untangle what flips could not, [9] 3D slab builds,
+ * [10] slab builds redone serially because the filter kept an uncertified
+ * cell, [11] uncertified cells, [12] microseconds spent in the host 3D
+ * tetrahedralisation (wall clock, copies and overlapped filters included). */
 int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats);
+/* 3D handles: build each iteration's tetrahedralisation as nslabs slabs
+ * along `axis` on nthreads host threads (sm_host_delaunay3d_slabs); each
+ * slab's cells are copied to the device and filtered there on the handle's
+ * stream while later slabs are still being built.  Cells equal those of the
+ * serial build: when the filter keeps an uncertified cell the iteration's
+ * tetrahedralisation is rebuilt serially and the halo doubled.  Replaces the
+ * per-rank slabs of mesh_generator.py:808-852, :867-877 and
+ * decomp/blocker.py:4-111 (axis as the reference's `axis` option). */
+int sm_mesher_set_slabs(sm_mesher *m, int axis, int nslabs, double halo, int nthreads);
 /* Per-phase kernel timing with HIP events recorded on the handle's stream.
  * Phases: 0 flip rounds, 1 centroid filter, 2 unique edges + edge lists,
  * 3 bar lengths and sizing, 4 L0 sums, 5 bar forces, 6 gather + update +
@@ -394,6 +406,22 @@ int64_t sm_host_delaunay2d(const double *p, int64_t n, int32_t *tri, int32_t *ad
  * 0 for exact duplicates. */
 int64_t sm_host_delaunay3d(const double *p, int64_t n, int32_t *tet, int64_t cap,
                            uint8_t *inserted);
+
+/* The same tetrahedralisation built as nslabs equal-width slabs along `axis`
+ * on up to nthreads threads (replaces the slab decomposition of
+ * generation/utils.py:28-45 (make_init_points), mesh_generator.py:867-877
+ * (_form_extents, +-halo along the axis) and decomp/blocker.py:4-111, whose
+ * slabs the reference meshes on separate MPI ranks).  Slab k triangulates
+ * the points whose axis coordinate lies within `halo` of it and reports the
+ * tetrahedra whose lowest vertex id lies in it.  The first *ncertified
+ * tetrahedra have circumspheres inside their slab's window and are exactly
+ * those of sm_host_delaunay3d; the rest could not be certified (circumsphere
+ * wider than the window) and may differ: a caller that keeps any of them
+ * must rebuild with sm_host_delaunay3d.  Tetrahedra are in canonical vertex
+ * order (lowest id first, orientation kept) in both functions. */
+int64_t sm_host_delaunay3d_slabs(const double *p, int64_t n, int axis, int nslabs, double halo,
+                                 int nthreads, int32_t *tet, int64_t cap, uint8_t *inserted,
+                                 int64_t *ncertified);
 
 /* ------------------------------------------------------- 3D quality kernels */
 /* angles: (6*ncells) dihedral angles in radians, FP64 (fast_geometry.cpp

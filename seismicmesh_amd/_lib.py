@@ -117,6 +117,7 @@ _PROTOS = {
     "sm_mesher_get_cells": (_int, [_vp, _vp, _i64]),
     "sm_mesher_get_edges": (_int, [_vp, _vp, _i64]),
     "sm_mesher_stats": (_int, [_vp, _vp, _int]),
+    "sm_mesher_set_slabs": (_int, [_vp, _int, _int, _dbl, _int]),
     "sm_mesher_set_graph": (_int, [_vp, _int]),
     "sm_mesher_profile": (_int, [_vp, _int]),
     "sm_mesher_profile_read": (_int, [_vp, _vp, _vp, _int]),
@@ -143,6 +144,8 @@ _PROTOS = {
     "sm_mlp_pack_hidden": (_int, [_int, _int, _vp, _vp, _vp]),
     "sm_host_delaunay2d": (_i64, [_vp, _i64, _vp, _vp, _i64, _vp]),
     "sm_host_delaunay3d": (_i64, [_vp, _i64, _vp, _i64, _vp]),
+    "sm_host_delaunay3d_slabs": (_i64, [_vp, _i64, _int, _int, _dbl, _int, _vp, _i64, _vp,
+                                        ctypes.POINTER(_i64)]),
     "sm_dihedral_angles": (_int, [_vp, _vp, _i64, _vp, _vp]),
     "sm_circumsphere_grad": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "sm_limgrad": (_int, [_vp, _int, _int, _int, _dbl, _dbl, _vp]),

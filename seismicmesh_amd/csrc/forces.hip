@@ -545,19 +545,20 @@ __global__ void k_project_surrogate(int64_t N, double *p, const float *dnn, doub
 }  // namespace
 
 template <int DIM, int MODE>
-static void launch_keep(sm_mesher *m) {
+static void launch_keep(sm_mesher *m, int64_t off, int64_t T) {
   const int B = m->blk[kBlkKeep];
-  hipLaunchKernelGGL((k_keep<DIM, MODE>), dim3(blocks_for(m->T, B)), dim3(B), 0, m->stream,
-                     m->T, m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0], m->prm.geps,
-                     (int32_t)m->N, m->d_keep);
+  hipLaunchKernelGGL((k_keep<DIM, MODE>), dim3(blocks_for(T, B)), dim3(B), 0, m->stream, T,
+                     m->d_tri + (DIM + 1) * off, m->d_p, m->d_tapes[0], m->tape_len[0],
+                     m->prm.geps, (int32_t)m->N, m->d_keep + off);
 }
 
 template <int DIM, int MODE>
-static void launch_keep_surrogate(sm_mesher *m) {
+static void launch_keep_surrogate(sm_mesher *m, int64_t off, int64_t T) {
   const int B = m->blk[kBlkKeep];
-  hipLaunchKernelGGL((k_keep_surrogate<DIM, MODE>), dim3(blocks_for(m->T, B)), dim3(B), 0,
-                     m->stream, m->T, m->d_tri, m->d_p, m->d_tapes[0], m->tape_len[0],
-                     m->prm.geps, m->d_dnn, m->band, (int32_t)m->N, m->d_keep, m->d_sg);
+  hipLaunchKernelGGL((k_keep_surrogate<DIM, MODE>), dim3(blocks_for(T, B)), dim3(B), 0,
+                     m->stream, T, m->d_tri + (DIM + 1) * off, m->d_p, m->d_tapes[0],
+                     m->tape_len[0], m->prm.geps, m->d_dnn + off, m->band, (int32_t)m->N,
+                     m->d_keep + off, m->d_sg);
 }
 
 template <int DIM, int MODE>
@@ -683,25 +684,36 @@ int centroid_keep(sm_mesher *m) {
     SM_LAUNCH_CHECK();
     return SM_OK;
   }
+  bool done = false;
+  int rc = centroid_keep_range(m, 0, m->T, &done);
+  return rc;
+}
+
+int centroid_keep_range(sm_mesher *m, int64_t off, int64_t T, bool *done) {
+  *done = false;
+  if (m->host_sdf || m->ndom > 0) return SM_OK;
+  *done = true;
+  if (T <= 0) return SM_OK;
   if (m->surrogate) {
-    int rc = mlp_eval_cells(&m->net, m->d_tri, m->T, m->d_p, m->d_dnn, m->stream);
+    int rc = mlp_eval_cells(&m->net, m->d_tri + (m->dim + 1) * off, T, m->d_p, m->d_dnn + off,
+                            m->stream);
     if (rc) return rc;
     if (m->dim == 2) {
-      if (m->deep_sdf) launch_keep_surrogate<2, kSdfDeep>(m);
-      else launch_keep_surrogate<2, kSdfReg>(m);
+      if (m->deep_sdf) launch_keep_surrogate<2, kSdfDeep>(m, off, T);
+      else launch_keep_surrogate<2, kSdfReg>(m, off, T);
     } else {
-      if (m->deep_sdf) launch_keep_surrogate<3, kSdfDeep>(m);
-      else launch_keep_surrogate<3, kSdfReg>(m);
+      if (m->deep_sdf) launch_keep_surrogate<3, kSdfDeep>(m, off, T);
+      else launch_keep_surrogate<3, kSdfReg>(m, off, T);
     }
     SM_LAUNCH_CHECK();
     return SM_OK;
   }
   if (m->dim == 2) {
-    if (m->deep_sdf) launch_keep<2, kSdfDeep>(m);
-    else launch_keep<2, kSdfReg>(m);
+    if (m->deep_sdf) launch_keep<2, kSdfDeep>(m, off, T);
+    else launch_keep<2, kSdfReg>(m, off, T);
   } else {
-    if (m->deep_sdf) launch_keep<3, kSdfDeep>(m);
-    else launch_keep<3, kSdfReg>(m);
+    if (m->deep_sdf) launch_keep<3, kSdfDeep>(m, off, T);
+    else launch_keep<3, kSdfReg>(m, off, T);
   }
   SM_LAUNCH_CHECK();
   return SM_OK;

@@ -5,7 +5,10 @@
 #include <vector>
 
 #include "common.h"
+#include "delaunay3d.h"
 #include "sizing_eval.h"
+
+constexpr int kNumStats = 13;
 
 struct sm_mesher {
   sm_mesher_params prm{};
@@ -131,12 +134,25 @@ struct sm_mesher {
   int64_t dnn_cap = 0;
   uint64_t *d_sg = nullptr;
 
+  // 3D: the host tetrahedralisation, serial or in slabs on worker threads
+  // (delaunay3d.cpp); slab k's tetrahedra are copied to the device and
+  // filtered there while later slabs are still being built.  The cells from
+  // ncert on could not be certified: if the filter keeps one, the
+  // iteration's triangulation is rebuilt serially and the halo doubled.
+  int sl_axis = 1, sl_n = 1, sl_threads = 1;
+  double sl_halo = 0.0;
+  smdt::Slabs3 dt3;
+  int64_t ncert = -1;
+  double *h_pts = nullptr;  // pinned staging of the points (N, 3)
+  int32_t *h_tet = nullptr;  // pinned staging of the tetrahedra (capT, 4)
+  int64_t h_pts_cap = 0, h_tet_cap = 0;
+
   int32_t *d_tctr = nullptr;  // private counter block of sm_mesher_time_kernel
   // workgroup sizes of the tunable kernels (smk::kBlk*), set by
   // sm_mesher_autotune; 256 until then
   int blk[6] = {256, 256, 256, 256, 256, 256};
 
-  int64_t stats[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int64_t stats[kNumStats] = {};
   int round = 0;
   bool have_tri = false;
   double last_maxdp = 0.0;
@@ -234,6 +250,9 @@ int time_flip_tunable(sm_mesher *m, int which, int B, int reps, double *ms);
 // by the exact-order sums); otherwise every launch is sized for the
 // largest possible edge count and reads the count on the device
 int centroid_keep(sm_mesher *m);
+// the filter on cells [off, off + count) only (device programs; false when
+// this handle filters on the host, which then runs centroid_keep instead)
+int centroid_keep_range(sm_mesher *m, int64_t off, int64_t count, bool *done);
 int build_edges(sm_mesher *m, bool sync_edges);
 int forces_and_update(sm_mesher *m);
 // mlp.hip: surrogate inference (points, or centroids of cells)

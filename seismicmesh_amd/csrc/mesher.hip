@@ -2,6 +2,7 @@
 // SeismicMesh/generation/mesh_generator.py:460-527 (re-triangulate, filter
 // cells, forces, update, project), for 2D and 3D domains.
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <vector>
 
@@ -173,35 +174,114 @@ int host_sizing_pass(sm_mesher *m) {
 }
 
 // 3D: the tetrahedralisation is rebuilt on the host from the current points
-// every iteration (as the reference does with CGAL), then uploaded.
-int host_rebuild3d(sm_mesher *m) {
+// every iteration (as the reference does with CGAL).  Each slab's
+// tetrahedra go to the device (pinned staging, asynchronous copy) and
+// through the centroid filter as soon as that slab is built, while the
+// worker threads build the next ones; *filtered tells whether the filter
+// ran that way (device programs) or is still to run (host callbacks).
+int host_rebuild3d(sm_mesher *m, bool serial, bool *filtered) {
+  const auto t0 = std::chrono::steady_clock::now();
   const int64_t N = m->N;
-  std::vector<double> hp((size_t)N * 3);
-  SM_HIP(hipMemcpyAsync(hp.data(), m->d_p, N * 3 * sizeof(double), hipMemcpyDeviceToHost,
+  if (m->h_pts_cap < N) {
+    if (m->h_pts) (void)hipHostFree(m->h_pts);
+    m->h_pts = nullptr;
+    m->h_pts_cap = 0;
+    SM_HIP(hipHostMalloc((void **)&m->h_pts, (size_t)N * 3 * sizeof(double),
+                         hipHostMallocDefault));
+    m->h_pts_cap = N;
+  }
+  if (m->h_tet_cap < m->capT) {
+    if (m->h_tet) (void)hipHostFree(m->h_tet);
+    m->h_tet = nullptr;
+    m->h_tet_cap = 0;
+    SM_HIP(hipHostMalloc((void **)&m->h_tet, (size_t)m->capT * 4 * sizeof(int32_t),
+                         hipHostMallocDefault));
+    m->h_tet_cap = m->capT;
+  }
+  SM_HIP(hipMemcpyAsync(m->h_pts, m->d_p, N * 3 * sizeof(double), hipMemcpyDeviceToHost,
                         m->stream));
   SM_HIP(hipStreamSynchronize(m->stream));
-  std::vector<int32_t> tet((size_t)m->capT * 4);
-  std::vector<uint8_t> ins((size_t)N);
-  int64_t T = sm_host_delaunay3d(hp.data(), N, tet.data(), m->capT, ins.data());
-  if (T < 0) return sm_set_error((int)T, "host 3D Delaunay failed (%lld)", (long long)T);
-  m->T = T;
-  SM_HIP(hipMemcpyAsync(m->d_tri, tet.data(), T * 4 * sizeof(int32_t), hipMemcpyHostToDevice,
-                        m->stream));
-  SM_HIP(hipStreamSynchronize(m->stream));
+  const int S = serial ? 1 : m->sl_n;
+  int64_t off = 0, ncert = 0;
+  int err = SM_OK;
+  bool over = false, all_filtered = true;
+  int rc = m->dt3.run(
+      m->h_pts, N, m->sl_axis, S, m->sl_halo, serial ? 1 : m->sl_threads,
+      [&](int slab, const int32_t *v, int64_t count) {
+        if (slab < 0) ncert = off;
+        if (err || over || count == 0) return;
+        if (off + count > m->capT) {
+          over = true;
+          return;
+        }
+        std::memcpy(m->h_tet + 4 * off, v, (size_t)count * 4 * sizeof(int32_t));
+        if (hipMemcpyAsync(m->d_tri + 4 * off, m->h_tet + 4 * off,
+                           (size_t)count * 4 * sizeof(int32_t), hipMemcpyHostToDevice,
+                           m->stream) != hipSuccess) {
+          err = sm_set_error(SM_ERR_HIP, "host_rebuild3d: copy of slab %d failed", slab);
+          return;
+        }
+        // the points do not move before the filter, so it may run now
+        bool done = false;
+        int b = prof_begin(m);
+        err = centroid_keep_range(m, off, count, &done);
+        prof_end(m, 1, b);
+        all_filtered &= done;
+        off += count;
+      });
+  if (rc) return rc;
+  if (err) return err;
+  SM_CHECK(!over, SM_ERR_CAPACITY, "host 3D Delaunay: more than %lld tetrahedra",
+           (long long)m->capT);
+  m->T = off;
+  m->ncert = ncert;
   m->stats[3] += 1;
+  if (S > 1) {
+    m->stats[9] += 1;
+    m->stats[11] += off - ncert;
+  }
   m->have_tri = true;
+  *filtered = all_filtered;
+  m->stats[12] += (int64_t)std::chrono::duration_cast<std::chrono::microseconds>(
+                      std::chrono::steady_clock::now() - t0)
+                      .count();
   return SM_OK;
 }
 
 int triangulate(sm_mesher *m) {
   m->nsl = -1;  // a sliver selection refers to the previous cells
   if (m->dim == 3) {
-    int rc = host_rebuild3d(m);
+    bool filtered = false;
+    int rc = host_rebuild3d(m, false, &filtered);
     if (rc) return rc;
-    int b = prof_begin(m);
-    rc = centroid_keep(m);
-    prof_end(m, 1, b);
-    return rc;
+    if (!filtered) {
+      int b = prof_begin(m);
+      rc = centroid_keep(m);
+      prof_end(m, 1, b);
+      if (rc) return rc;
+    }
+    if (m->ncert < m->T) {
+      // an uncertified tetrahedron that the filter keeps: rebuild serially
+      const int64_t nu = m->T - m->ncert;
+      std::vector<uint8_t> kept((size_t)nu);
+      SM_HIP(hipMemcpyAsync(kept.data(), m->d_keep + m->ncert, (size_t)nu,
+                            hipMemcpyDeviceToHost, m->stream));
+      SM_HIP(hipStreamSynchronize(m->stream));
+      bool any = false;
+      for (uint8_t k : kept) any |= k != 0;
+      if (any) {
+        m->stats[10] += 1;
+        if (m->sl_halo > 0) m->sl_halo *= 2.0;
+        if ((rc = host_rebuild3d(m, true, &filtered))) return rc;
+        if (!filtered) {
+          int b = prof_begin(m);
+          rc = centroid_keep(m);
+          prof_end(m, 1, b);
+          if (rc) return rc;
+        }
+      }
+    }
+    return SM_OK;
   }
   bool rebuild = !m->have_tri;
   if (!rebuild) {
@@ -452,6 +532,8 @@ extern "C" int sm_mesher_destroy(sm_mesher *m) {
   if (m->h_status) (void)hipHostFree(m->h_status);
   if (m->h_sums) (void)hipHostFree(m->h_sums);
   if (m->h_maxF) (void)hipHostFree(m->h_maxF);
+  if (m->h_pts) (void)hipHostFree(m->h_pts);
+  if (m->h_tet) (void)hipHostFree(m->h_tet);
   for (hipEvent_t e : m->ev_pool) (void)hipEventDestroy(e);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
@@ -745,7 +827,23 @@ extern "C" int sm_mesher_get_edges(sm_mesher *m, int32_t *edges, int64_t cap) {
 
 extern "C" int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats) {
   SM_CHECK(m && stats, SM_ERR_ARG, "mesher_stats: null argument");
-  for (int i = 0; i < nstats && i < 9; ++i) stats[i] = m->stats[i];
+  for (int i = 0; i < nstats && i < kNumStats; ++i) stats[i] = m->stats[i];
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_set_slabs(sm_mesher *m, int axis, int nslabs, double halo,
+                                   int nthreads) {
+  SM_CHECK(m, SM_ERR_ARG, "mesher_set_slabs: null handle");
+  SM_CHECK(m->dim == 3, SM_ERR_ARG, "mesher_set_slabs: slabs split the 3D tetrahedralisation");
+  SM_CHECK(axis >= 0 && axis < 3, SM_ERR_ARG, "mesher_set_slabs: axis must be 0, 1 or 2");
+  SM_CHECK(nslabs >= 1 && nslabs <= 4096, SM_ERR_ARG, "mesher_set_slabs: bad slab count %d",
+           nslabs);
+  SM_CHECK(halo >= 0.0 && nthreads >= 1 && nthreads <= 1024, SM_ERR_ARG,
+           "mesher_set_slabs: bad halo or thread count");
+  m->sl_axis = axis;
+  m->sl_n = nslabs;
+  m->sl_halo = halo;
+  m->sl_threads = nthreads;
   return SM_OK;
 }
 

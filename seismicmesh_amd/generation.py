@@ -33,7 +33,7 @@ _KNOWN = {
     "verbose", "max_iter", "seed", "perform_checks", "pfix", "axis", "points", "domain",
     "edge_length", "bbox", "min_dh_angle_bound", "max_dh_angle_bound", "delta_t", "h0",
     "geps_mult", "subdomains", "gamma", "preserve", "mesh_improvement", "r0m_is_h0",
-    "precision", "surrogate", "surrogate_band",
+    "precision", "surrogate", "surrogate_band", "slabs",
 }
 
 
@@ -327,12 +327,28 @@ class Mesher:
                                                   ctypes.byref(ms)), "sm_mesher_time_kernel")
         return ms.value
 
+    STATS = ("iterations", "flip_rounds", "flips", "host_rebuilds", "exact_predicates",
+             "rollbacks", "kinetic_repairs", "unresolved", "relocations", "slab_builds",
+             "slab_fallbacks", "uncertified_cells", "host_tet_us")
+
     def stats(self):
-        s = (ctypes.c_int64 * 9)()
-        _lib.check(_lib.lib.sm_mesher_stats(self.h, s, 9), "sm_mesher_stats")
-        keys = ("iterations", "flip_rounds", "flips", "host_rebuilds", "exact_predicates",
-                "rollbacks", "kinetic_repairs", "unresolved", "relocations")
-        return {k: int(s[i]) for i, k in enumerate(keys)}
+        k = len(self.STATS)
+        s = (ctypes.c_int64 * k)()
+        _lib.check(_lib.lib.sm_mesher_stats(self.h, s, k), "sm_mesher_stats")
+        return {name: int(s[i]) for i, name in enumerate(self.STATS)}
+
+    def set_slabs(self, nslabs, halo, axis=1, threads=None):
+        """3D: build each iteration's tetrahedralisation as `nslabs`
+        equal-width slabs along coordinate `axis` (the reference's `axis`
+        option), each with the points within `halo` of it, on `threads` host
+        threads (default: one per slab, at most the CPU count).  The cells
+        are those of the serial build (uncertified ones that the filter
+        keeps trigger a serial rebuild and a doubled halo)."""
+        import os
+        if threads is None:
+            threads = max(1, min(int(nslabs), os.cpu_count() or 1))
+        _lib.check(_lib.lib.sm_mesher_set_slabs(self.h, int(axis), int(nslabs), float(halo),
+                                                int(threads)), "sm_mesher_set_slabs")
 
 
 def _placeholder(dim):
@@ -461,11 +477,39 @@ def prepare(domain, edge_length, **kwargs):
                 nfix=nfix, p=p, levels=levels, max_iter=max_iter, dim=dim)
 
 
-def make_mesher(st, far_square=False):
-    """A force-loop handle for a prepared problem (initial points not set)."""
-    return Mesher(st["levels"], st["fh"], st["h0"], st["delta_t"], st["geps"], st["deps"],
-                  st["nfix"], precision=st["opts"]["precision"], dim=st["dim"],
-                  far_square=far_square)
+# 3D: below this many points one thread builds the tetrahedralisation
+SLAB_MIN_POINTS = 20000
+
+
+def default_slabs(npoints, dim, requested=None):
+    """Slab count for the 3D host tetrahedralisation: `requested` if given
+    (1 = serial), else one slab per CPU (at most 16) for 3D problems of at
+    least SLAB_MIN_POINTS points."""
+    import os
+    if dim != 3:
+        return 1
+    if requested is not None:
+        return max(1, int(requested))
+    if npoints < SLAB_MIN_POINTS:
+        return 1
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+def make_mesher(st, far_square=False, slabs=None):
+    """A force-loop handle for a prepared problem (initial points not set).
+    3D handles build their tetrahedralisation in slabs along the `axis`
+    option (default_slabs; `slabs` or the `slabs` option overrides), with a
+    halo of 5 h0 -- the reference's subdomain extents (mesh_generator.py:
+    867-877)."""
+    m = Mesher(st["levels"], st["fh"], st["h0"], st["delta_t"], st["geps"], st["deps"],
+               st["nfix"], precision=st["opts"]["precision"], dim=st["dim"],
+               far_square=far_square)
+    if st["dim"] == 3:
+        req = slabs if slabs is not None else st["opts"].get("slabs")
+        ns = default_slabs(len(st["p"]), 3, req)
+        if ns > 1:
+            m.set_slabs(ns, 5.0 * st["h0"], axis=int(st["opts"].get("axis", 1)))
+    return m
 
 
 Mesher2D = Mesher

@@ -117,3 +117,140 @@ def test_3d_matches_qhull_and_skips_duplicates():
     t2, ins = _tets(q)
     assert ins[:50].all() and not ins[50:].any()
     assert t2.max() < 50
+
+
+# ------------------------------------------------- 3D: uniqueness and slabs
+def _tets_slabs(p, nslabs, halo, axis=1, threads=4):
+    import ctypes
+    from seismicmesh_amd import _lib
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    n = len(p)
+    cap = 12 * n + 64
+    t = np.zeros((cap, 4), np.int32)
+    ins = np.zeros(n, np.uint8)
+    nc = ctypes.c_int64(-1)
+    m = _lib.lib.sm_host_delaunay3d_slabs(p.ctypes.data, n, axis, nslabs, float(halo), threads,
+                                          t.ctypes.data, cap, ins.ctypes.data, ctypes.byref(nc))
+    assert m >= 0, _lib.last_error()
+    return t[:m], nc.value, ins
+
+
+def _canon_ok(p, t):
+    # lowest id first, second-lowest second, positively oriented
+    from oracle.predicates import orient3d
+    assert (t[:, 0] < t[:, 1]).all() and (t[:, 1] < t[:, 2]).all() and (t[:, 1] < t[:, 3]).all()
+    for tt in t[:: max(1, len(t) // 300)]:
+        assert orient3d(*p[tt]) > 0
+
+
+def _circumradius(p, t):
+    a = p[t[:, 0]]
+    u, v, w = p[t[:, 1]] - a, p[t[:, 2]] - a, p[t[:, 3]] - a
+    num = ((u * u).sum(1)[:, None] * np.cross(v, w) + (v * v).sum(1)[:, None] * np.cross(w, u)
+           + (w * w).sum(1)[:, None] * np.cross(u, v))
+    den = 2 * np.einsum("ij,ij->i", u, np.cross(v, w))
+    return np.linalg.norm(num / den[:, None], axis=1)
+
+
+def test_3d_lattice_unique_whatever_the_order():
+    """Cospherical lattice points: the symbolic perturbation makes the
+    tetrahedralisation a function of the point set, so any insertion order
+    gives the same cells; they tile the box exactly."""
+    g = np.stack(np.meshgrid(np.arange(7.0), np.arange(6.0), np.arange(5.0), indexing="ij"),
+                 -1).reshape(-1, 3)
+    t0, ins = _tets(g)
+    assert ins.all()
+    _canon_ok(g, t0)
+    vol = np.einsum("ij,ij->i", np.cross(g[t0[:, 1]] - g[t0[:, 0]], g[t0[:, 2]] - g[t0[:, 0]]),
+                    g[t0[:, 3]] - g[t0[:, 0]]) / 6
+    assert vol.min() > 0 and abs(vol.sum() - 6 * 5 * 4) < 1e-9
+    want = {tuple(x) for x in t0.tolist()}
+    rng = np.random.default_rng(3)
+    for _ in range(3):
+        perm = rng.permutation(len(g))
+        t1, _ = _tets(g[perm])
+        t1 = perm[t1]
+        assert {tuple(sorted(x)) for x in t1.tolist()} == {tuple(sorted(x)) for x in t0.tolist()}
+    assert len(want) == len(t0)
+
+
+def test_3d_perturbed_empty_spheres_on_degenerate_points():
+    """On points with many cospherical subsets every cell is empty of the
+    other points in the exact sense (insphere <= 0), and the cells fill the
+    convex hull."""
+    from scipy.spatial import ConvexHull
+    from oracle.predicates import insphere
+    g = np.stack(np.meshgrid(np.arange(3.0), np.arange(3.0), np.arange(3.0), indexing="ij"),
+                 -1).reshape(-1, 3)
+    p = np.vstack([g, [[0.5, 0.5, 0.5], [1.5, 1.0, 0.5]]])
+    t, ins = _tets(p)
+    assert ins.all()
+    for tt in t:
+        for k in range(len(p)):
+            if k not in tt:
+                assert insphere(*p[tt], p[k]) <= 0
+    vol = np.einsum("ij,ij->i", np.cross(p[t[:, 1]] - p[t[:, 0]], p[t[:, 2]] - p[t[:, 0]]),
+                    p[t[:, 3]] - p[t[:, 0]]) / 6
+    assert abs(vol.sum() - ConvexHull(p).volume) < 1e-12
+
+
+@pytest.mark.parametrize("kind", ["lattice", "jittered", "random"])
+def test_3d_slabs_equal_serial_build(kind):
+    """Slab builds (threads, halos) report exactly the serial cells as
+    certified, except cells whose circumsphere is wider than the halo, which
+    come back uncertified; on the lattice nothing is left out."""
+    rng = np.random.default_rng(11)
+    if kind == "random":
+        p = rng.random((6000, 3)) * [3.0, 2.0, 1.0]
+        h = (6.0 / 6000) ** (1 / 3)
+    else:
+        h = 0.1
+        g = np.mgrid[0:2.0 + h / 2:h, 0:1.5 + h / 2:h, 0:0.8 + h / 2:h].reshape(3, -1).T
+        p = g if kind == "lattice" else g + rng.uniform(-0.2, 0.2, g.shape) * h
+    serial, _ = _tets(p)
+    S = {tuple(x) for x in serial.tolist()}
+    for axis, nslabs, halo in [(0, 4, 2 * h), (1, 7, 3 * h), (2, 3, 5 * h), (0, 1, 0.0)]:
+        t, nc, ins = _tets_slabs(p, nslabs, halo, axis=axis)
+        assert ins.all()
+        _canon_ok(p, t)
+        cert = [tuple(x) for x in t[:nc].tolist()]
+        assert len(set(cert)) == len(cert)
+        assert set(cert) <= S
+        missing = np.array(sorted(S - set(cert)), dtype=np.int64).reshape(-1, 4)
+        if nslabs == 1 or kind == "lattice":
+            assert len(missing) == 0 and nc == len(t)
+        if len(missing):
+            # only cells whose sphere does not fit a halo around their slab
+            assert (_circumradius(p, missing) > 0.5 * halo * (1 - 1e-9)).all()
+
+
+def test_3d_slabs_duplicates_keep_lowest_id():
+    rng = np.random.default_rng(5)
+    p = rng.random((2000, 3))
+    q = np.vstack([p[1500:], p[:1500], p[:40]])  # duplicates of ids 500.. at the end
+    t_s, ins_s = _tets(q)
+    t_p, nc, ins_p = _tets_slabs(q, 5, 0.3, axis=0)
+    assert np.array_equal(ins_s, ins_p)
+    assert ins_s[:2000].all() and not ins_s[2000:].any()
+    assert t_s.max() < 2000 and t_p.max() < 2000
+
+
+def test_3d_convex_hull_cells_match_qhull_on_a_jittered_lattice():
+    """Flat cells along the hull faces of a lattice (which the reference's
+    CGAL triangulation has, and whose centroids lie inside a box domain that
+    the lattice does not reach) are produced too: every cell equals Qhull's
+    on a lattice moved by 1e-3 of its spacing."""
+    from scipy.spatial import Delaunay
+    h = 1.0
+    g = np.mgrid[0:9.0:h, 0:7.0:h, 0:4.0:h].reshape(3, -1).T
+    p = g + np.random.default_rng(2).uniform(-1e-3, 1e-3, g.shape) * h
+    t, ins = _tets(p)
+    assert ins.all()
+    want = {tuple(sorted(x)) for x in Delaunay(p).simplices.tolist()}
+    got = {tuple(sorted(x)) for x in t.tolist()}
+    assert got == want
+    vol = np.einsum("ij,ij->i", np.cross(p[t[:, 1]] - p[t[:, 0]], p[t[:, 2]] - p[t[:, 0]]),
+                    p[t[:, 3]] - p[t[:, 0]]) / 6
+    assert vol.min() > 0
+    # the hull's flat cells are there: some have a circumradius far above h
+    assert _circumradius(p, t).max() > 50 * h
