@@ -1,22 +1,32 @@
 #!/usr/bin/env python3
 """Throughput of the DistMesh force loop on one MI355X.
 
-Workload (default): the BP2004-shaped 2D velocity-model mesh -- a
-1911 x 5395 FP32 velocity grid over z in [-12 km, 0], x in [0, 67 km],
-sizing v / (2 Hz * 10) clamped to 75 m and gradient-limited at grade 0.15,
-on the Rectangle domain with its 4 corners fixed (seismicmesh_amd.workloads).
+Workload (default): the 3D many-primitive CSG of seismicmesh_amd.workloads
+(48 balls and 16 boxes, minus 8 cylinders, plus 4 tori in [-1, 1]^3, seed
+3) at uniform h = 0.015 (~0.5 M vertices).  Two paths run the same
+iterations from the same initial points:
 
-A step is one force iteration over the whole point set: device Delaunay
-repair, centroid filter, sorted unique edges, bar forces, gather, update
-and Newton projection.  The timed region is `steps` iterations starting
-from the initial points (the reference's loop performs max_iter - 1 = 49 of
-them); initialisation (lattice, first triangulation) and termination are
-outside it, as in the metric's definition.
+* fused: the tetrahedralisation built in 16 slabs on host threads, each
+  slab's cells copied and filtered on the device while later slabs are
+  built, and a neural SDF surrogate (3 -> 256 x4 -> 1, BF16 matrix-core
+  inference) trained here by the library's own kernels on 4 M samples of
+  the exact SDF deciding the centroid filter and the projection test away
+  from the boundary (exact SDF inside the band);
+* plain: the serial tetrahedralisation and the exact SDF everywhere.
 
-Output: one JSON line with the metric, the roofline of the dominant single
-kernel (HIP events around its launches on the loop's own stream) and a CPU
-baseline (the numpy oracle on a bounded sample of the same workload, one
-pinned core, median of three runs).
+A step is one force iteration over the whole point set: re-triangulation,
+centroid filter, sorted unique edges, bar forces, gather, update and Newton
+projection.  The timed region is `steps` iterations starting from the
+initial points (the reference's loop performs max_iter - 1 = 49 of them);
+initialisation (lattice, surrogate training) and termination are outside
+it, as in the metric's definition.  `--workload` selects the other
+workloads (2D BP2004-shaped, unit disk, 512 independent 2D domains,
+EAGE-shaped 3D).
+
+Output: one JSON line with the metric, the roofline of the dominant
+streaming kernel and of every streaming kernel (HIP events on the loop's
+own stream) and a CPU baseline (the numpy oracle on a bounded sample of the
+same workload, one pinned core).
 """
 import argparse
 import json
@@ -35,7 +45,8 @@ HBM_PEAK_GBS = 8000.0
 # kernels timed alone by the handle's event profiler, by phase name
 SINGLE_KERNEL = {"centroid_filter": "k_keep", "edge_geom": "k_edge_geom",
                  "edge_force": "k_edge_force", "vertex_update": "k_vertex_update",
-                 "flip_mark": "k_mark"}
+                 "flip_mark": "k_mark", "unique_edges": "edge build (k_count, scan, "
+                 "k_scatter, k_bucket_sorts, k_emit)"}
 
 
 def cpu_info():
@@ -62,42 +73,66 @@ def load_traffic(kernel, workload):
         return None
 
 
-def alg_bytes(phase, N, E, C, T, dim=2):
-    """Algorithmic HBM bytes of one launch (FP64 points, FP32 per-edge
-    arrays and sizing grid, int32 ids).  Each datum is counted once: gathers
-    of point coordinates count 8 dim B per referenced vertex."""
+def alg_bytes(phase, N, E, C, T, dim=2, grid_taps=True):
+    """Algorithmic HBM bytes of one pass of a phase over the whole mesh
+    (FP64 points, FP32 per-edge arrays and sizing grid, int32 ids): every
+    datum that has to cross HBM is counted once -- a vertex's coordinates
+    once however many edges or cells gather them, an edge's force vector
+    once although both endpoints read it.  Sizing-grid taps count 2^dim
+    FP32 values per edge (an upper bound on the distinct taps; none for a
+    constant size)."""
     pt = 8 * dim
+    taps = 4 * 2 ** dim if grid_taps else 0
     if phase == "edge_geom":
-        # edge ids 8 + both endpoints + 2^dim sizing taps + L, h out 8
-        return E * (8 + 2 * pt + 4 * 2 ** dim + 8)
+        # edge ids in, L and h out; points and sizing taps gathered
+        return E * (8 + taps + 8) + N * pt
     if phase == "edge_force":
-        # edge ids 8 + L, h 8 + both endpoints + force vector out
-        return E * (8 + 8 + 2 * pt + 4 * dim)
+        # edge ids, L and h in, force vector out; points gathered
+        return E * (8 + 8 + 4 * dim) + N * pt
     if phase == "vertex_update":
-        # offsets 16 + point in/out per vertex; per edge its force vector
-        # read from both endpoints and the second-endpoint list entry 4
-        return N * (16 + 2 * pt) + E * (2 * 4 * dim + 4)
+        # per vertex: two list offsets, point in and out; per edge: its force
+        # vector and its entry in the second-endpoint list
+        return N * (8 + 2 * pt) + E * (4 * dim + 4)
     if phase == "unique_edges":
-        # cells read twice + their pairs written/read, edges out 8
-        pairs = 3 if dim == 2 else 6
-        return C * (2 * 4 * (dim + 1) + 8 * pairs) + E * 8 + N * 16
+        # kept cells in, sorted edges and first-endpoint offsets out
+        return C * 4 * (dim + 1) + E * 8 + N * 4
     if phase == "centroid_filter":
-        # cell ids + keep flag 1 per cell, points per vertex
+        # cell ids in, keep flag out, points gathered
         return T * (4 * (dim + 1) + 1) + N * pt
     if phase == "flip_mark":
         # per triangle: ids 12, neighbours 12, three 64-bit candidates 24,
         # its 64-bit claim 8; points 16 per vertex.  Orientations are not
-        # read when no triangle is inverted or uncertain (the timed launches
-        # start from clear counters; were incircle tests uncertain, later
-        # launches would also read them and this rate is a lower bound)
+        # read when no triangle is inverted or uncertain
         return T * (12 + 12 + 24 + 8) + N * 16
     return 0
 
 
+STREAMING = ("centroid_filter", "unique_edges", "edge_geom", "edge_force", "vertex_update")
+
+
+def streaming_rooflines(prof, iters, N, E, C, T, dim, grid_taps):
+    """Achieved GB/s of each streaming phase over the profiled iterations:
+    algorithmic bytes per iteration / event-timed ms per iteration."""
+    out = {}
+    for ph in STREAMING:
+        ms, launches = prof.get(ph, (0.0, 0))
+        if ms <= 0 or iters <= 0:
+            continue
+        b = alg_bytes(ph, N, E, C, T, dim, grid_taps)
+        per_it = ms / iters
+        gbs = b / (per_it * 1e-3) / 1e9
+        out[ph] = {"kernel": SINGLE_KERNEL.get(ph, ph), "alg_bytes_per_iteration": int(b),
+                   "ms_per_iteration": round(per_it, 4),
+                   "launches_per_iteration": round(launches / iters, 2),
+                   "achieved_GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
 def cpu_baseline(st, p0, iters, reps=3):
     """The oracle (numpy FP64 restatement + scipy Qhull) on `iters` force
-    iterations of the same workload: one thread pinned to one core, BLAS
-    pools limited to one thread, median of `reps` runs."""
+    iterations of the same workload (or of the given subset p0 of its
+    points): one thread pinned to one core, BLAS pools limited to one
+    thread, median of `reps` runs."""
     try:
         from threadpoolctl import threadpool_limits
     except ImportError:  # pragma: no cover
@@ -116,7 +151,8 @@ def cpu_baseline(st, p0, iters, reps=3):
     dim = st["dim"]
     s = dict(fd=odom.eval, fh=ofh, h0=st["h0"], dim=dim, L0mult=1 + 0.4 / 2 ** (dim - 1),
              geps=st["geps"],
-             deps=st["deps"], pfix=p0[:st["nfix"]], levels=[odom.eval], delta_t=st["delta_t"])
+             deps=st["deps"], pfix=p0[:min(st["nfix"], len(p0))], levels=[odom.eval],
+             delta_t=st["delta_t"])
     old_aff = os.sched_getaffinity(0)
     core = min(old_aff)
     os.sched_setaffinity(0, {core})
@@ -130,7 +166,7 @@ def cpu_baseline(st, p0, iters, reps=3):
             t0 = time.perf_counter()
             for _ in range(iters):
                 t_all = odm.delaunay(p)
-                p, _, _ = odm.force_step(p, t_all, s, list(range(st["nfix"])))
+                p, _, _ = odm.force_step(p, t_all, s, list(range(min(st["nfix"], len(p)))))
             times.append(time.perf_counter() - t0)
     finally:
         if ctx:
@@ -213,7 +249,7 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
 def mlp_measure(npts=1 << 22, reps=10):
-    """The neural SDF surrogate's inference kernel (k_mlp_eval, BF16 matrix
+    """The neural SDF surrogate's inference kernel (k_mlp_eval_lds, BF16 matrix
     cores, 3 -> 256 x4 -> 1) on npts points with seeded random weights:
     average launch time from HIP events on the launch stream, and its
     FLOP rate against the dense BF16 peak."""
@@ -246,19 +282,219 @@ def mlp_measure(npts=1 << 22, reps=10):
     ms = e0.elapsed_time(e1) / reps
     flop = 2.0 * (3 * 256 + 3 * 256 * 256 + 256) * npts
     tf = flop / (ms * 1e-3) / 1e12
-    return {"bound": "mfma", "kernel": "k_mlp_eval", "points": npts,
+    return {"bound": "mfma", "kernel": "k_mlp_eval_lds", "points": npts,
             "flop_per_point": int(flop / npts), "avg_launch_us": round(ms * 1e3, 2),
             "achieved": round(tf, 1), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4)}
 
 
+def bench_3d(args):
+    """The 3D workloads: the fused path (slab-parallel tetrahedralisation
+    overlapped with the device filter, and for the CSG a trained surrogate)
+    against the plain path (serial tetrahedralisation, exact SDF), same
+    points, same iterations."""
+    import torch
+    from seismicmesh_amd import generation, workloads
+    from seismicmesh_amd import mesh_utils as mu
+
+    t_start = time.perf_counter()
+
+    def progress(msg):
+        print("[bench %.1fs] %s" % (time.perf_counter() - t_start, msg), file=sys.stderr,
+              flush=True)
+
+    if args.workload == "csg3d":
+        domain, bbox3 = workloads.csg3d()
+        ef = float(args.h)
+        wl = {"workload": "csg3d_neural", "primitives": "48 balls U 16 boxes, minus 8 "
+              "cylinders, U 4 tori in [-1, 1]^3 (seed 3)", "h": ef, "max_iter": 50}
+        grid_taps = False
+    else:
+        domain, ef, info = workloads.eage3d(scale=args.scale)
+        wl = {"workload": "eage_3d", "grid": list(info["grid"]), "bbox_m": list(info["bbox"]),
+              "hmin_m": 150.0, "grade": 0.15, "max_iter": 50}
+        grid_taps = True
+    st = generation.prepare(domain, ef, verbose=0, precision=args.precision)
+    p0 = st["p"]
+    N = len(p0)
+    progress("prepared %s: %d points" % (wl["workload"], N))
+    threads = max(1, min(args.slabs, os.cpu_count() or 1))
+    fused = generation.make_mesher(st, slabs=1)
+    axis = int(st["opts"]["axis"])
+    fused.set_slabs(args.slabs, axis=axis, threads=threads)
+    plain = generation.make_mesher(st, slabs=1)
+    # workgroup sizes tuned on this workload (two iterations in), shared
+    fused.set_points(p0)
+    fused.run(2)
+    tuned = fused.autotune(reps=10)
+    sizes = {k: v[0] for k, v in tuned.items()}
+    plain.set_workgroups(sizes)
+    progress("autotune: %s" % sizes)
+
+    neural = None
+    if args.workload == "csg3d":
+        from seismicmesh_amd.neural import NeuralSDF
+        t0 = time.perf_counter()
+        net = NeuralSDF(3, bbox3, hidden=3, seed=0, omega=8.0)
+        net.train(domain, nsamples=args.train_samples, steps=args.train_steps, batch=8192,
+                  lr=2e-3, lr_final=1e-4, eik=0.1, near=0.05, seed=0)
+        t_train = time.perf_counter() - t0
+        # a band per region of a 16^3 grid: twice the largest error measured
+        # there (and in the neighbouring regions), at least 2 h
+        bands, blo, bhi, err = net.calibrate_bands(domain, 2.0 * ef, res=16, nval=2000000)
+        progress("surrogate trained in %.1f s: max error %.4f, bands %.4f .. %.4f (median %.4f)"
+                 % (t_train, err, bands.min(), bands.max(), float(np.median(bands))))
+        fused.set_surrogate(net, 2.0 * ef, bands=(bands, blo, bhi))
+        neural = {"training_s": round(t_train, 1), "steps": args.train_steps,
+                  "samples": args.train_samples, "near": 0.05, "max_error": round(err, 4),
+                  "band_min": 2.0 * ef, "bands": "16^3 regions, 2 x the largest error measured "
+                  "on 2 M validation points (half within 2 h of the surface) in each region "
+                  "and its neighbours",
+                  "band_median": round(float(np.median(bands)), 4),
+                  "band_max": round(float(bands.max()), 4)}
+
+    def min_dihedral(h):
+        """Smallest dihedral angle (degrees) of the loop's kept cells, and
+        the 1st percentile of the cells' smallest angles.  The angle formula
+        (the reference's acos) gives NaN for cells flat to rounding, counted
+        as 0 degrees."""
+        h.triangulate()
+        p, t = h.points(), h.cells()
+        a = np.degrees(np.nan_to_num(mu.dihedral_angles(p, t), nan=0.0)).min(1)
+        return round(float(a.min()), 4), round(float(np.percentile(a, 1)), 4), len(t)
+
+    def timed(h, name):
+        h.set_points(p0)
+        if args.warmup > 0:
+            h.run(args.warmup)
+        h.set_points(p0)
+        s0 = h.stats()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        h.run(args.steps)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        s1 = h.stats()
+        progress("%s loop: %d iterations in %.3f s" % (name, args.steps, dt))
+        d = {k: s1[k] - s0[k] for k in s1}
+        d["min_quality"], d["p1_quality"], d["cells"] = min_dihedral(h)
+        return dt, d
+
+    elapsed, stats = timed(fused, "fused")
+    p_fused = fused.points()
+    if neural is not None:
+        sg = fused.surrogate_stats()
+        neural["exact_evaluations_skipped"] = round(sg["skipped"] / max(sg["skipped"] +
+                                                                       sg["exact"], 1), 4)
+        neural["max_error_in_band"] = round(sg["max_err"], 4)
+    plain_elapsed, plain_stats = timed(plain, "plain")
+    identical = bool(np.array_equal(p_fused, plain.points()))
+    n, C, E = fused.sizes()
+    value = N * args.steps / elapsed
+    plain_value = N * args.steps / plain_elapsed
+
+    # the fused region again with per-phase HIP events on the loop's stream
+    fused.set_points(p0)
+    fused.profile(True)
+    s0 = fused.stats()
+    fused.run(args.steps)
+    prof = fused.profile_read()
+    s1 = fused.stats()
+    fused.profile(False)
+    progress("event-profiled loop done")
+    streaming = streaming_rooflines(prof, args.steps, N, E, C, C, 3, grid_taps)
+    best = max(streaming, key=lambda k: streaming[k]["ms_per_iteration"])
+    b = streaming[best]
+    roofline = {"bound": "hbm", "kernel": b["kernel"], "achieved": b["achieved_GBps"],
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b["frac"],
+                "traffic": load_traffic(SINGLE_KERNEL.get(best, best), wl["workload"]),
+                "alg_bytes_per_iteration": b["alg_bytes_per_iteration"],
+                "ms_per_iteration": b["ms_per_iteration"],
+                "launches_per_iteration": b["launches_per_iteration"],
+                "streaming": streaming,
+                "phases_ms": {k: round(v[0], 3) for k, v in prof.items() if v[0] > 0}}
+    dev_ms = sum(v[0] for v in prof.values()) / args.steps
+
+    cpu = None
+    if not args.no_cpu:
+        # a bounded sample: the fixed points and the points of the lowest
+        # quarter along the slab axis, one oracle iteration
+        nf = st["nfix"]
+        rest = p0[nf:]
+        cut = np.quantile(rest[:, axis], 0.25)
+        sample = np.vstack([p0[:nf], rest[rest[:, axis] <= cut]])
+        rate, secs, core = cpu_baseline(st, sample, 1, reps=1)
+        model, ncpu = cpu_info()
+        cpu = {"value": round(rate, 1), "unit": "vertex-updates/s", "cores": 1, "kind": "port",
+               "sample": "1 force iteration of the workload's first quarter along axis %d "
+                         "(%d of its %d points) with the numpy oracle + scipy Qhull (stand-in "
+                         "for CGAL), one thread pinned to core %d of %d (%s), %.1f s"
+                         % (axis, len(sample), N, core, ncpu, model, secs)}
+        progress("cpu baseline done")
+    mlp = None if args.no_mlp else mlp_measure()
+
+    out = {
+        "metric": "vertex-updates/s on 1 MI355X, fused+neural vs plain GPU at equal min quality",
+        "value": round(value, 1),
+        "unit": "vertex-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == 32 else "f64",
+        "data": "synthetic",
+        "config": dict(
+            wl, N=N, edges=E, cells=C,
+            path=("tetrahedralisation by %d host threads inserting the points of %d slabs "
+                  "(axis %d) into one shared tetrahedralisation, its cells copied and filtered "
+                  "on the device piece by piece; device filter, edges, forces, update, "
+                  "projection%s"
+                  % (threads, args.slabs, axis,
+                     "; surrogate-steered filter and projection (exact SDF in the band)"
+                     if neural else "")),
+            plain_value=round(plain_value, 1),
+            speedup_vs_plain=round(value / plain_value, 3),
+            plain_path="serial tetrahedralisation, exact SDF, same device kernels",
+            points_identical_to_plain=identical,
+            min_quality=stats["min_quality"], plain_min_quality=plain_stats["min_quality"],
+            p1_quality=stats["p1_quality"], plain_p1_quality=plain_stats["p1_quality"],
+            quality_measure="minimum dihedral angle (degrees) of the raw loop mesh after the "
+                            "timed iterations (cells flat to rounding count as 0), and the 1st "
+                            "percentile of the cells' minimum angles",
+            host_tet_ms_per_iteration=round(stats["host_tet_us"] / 1e3 / args.steps, 2),
+            plain_host_tet_ms_per_iteration=round(plain_stats["host_tet_us"] / 1e3 /
+                                                  args.steps, 2),
+            device_ms_per_iteration=round(dev_ms, 3),
+            parallel_builds=stats["parallel_builds"],
+            retried_points_per_iteration=round(stats["retried_points"] / args.steps, 1),
+            workgroup_sizes=sizes),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "mlp_roofline": mlp,
+    }
+    if neural is not None:
+        out["config"]["surrogate"] = neural
+    fused.close()
+    plain.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=49)
-    ap.add_argument("--warmup", type=int, default=49)
-    ap.add_argument("--workload", default="bp2004",
-                    choices=["bp2004", "disk", "many", "eage", "csg3d"])
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="csg3d",
+                    choices=["csg3d", "eage", "bp2004", "disk", "many"])
+    ap.add_argument("--h", type=float, default=0.015,
+                    help="uniform size of the 3D CSG workload")
+    ap.add_argument("--slabs", type=int, default=16,
+                    help="3D: slabs (and host threads) of the fused path's tetrahedralisation")
+    ap.add_argument("--train-samples", type=int, default=4 << 20)
+    ap.add_argument("--train-steps", type=int, default=4000)
     ap.add_argument("--domains", type=int, default=512,
                     help="number of domains of the many-domains workload")
     ap.add_argument("--scale", type=float, default=1.0)
@@ -291,6 +527,10 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
+    if args.workload in ("csg3d", "eage"):
+        print(json.dumps(bench_3d(args)))
+        return
+
     sts = None
     if args.workload == "many":
         from seismicmesh_amd import batch
@@ -308,17 +548,6 @@ def main():
             domain, ef, info = workloads.bp2004(scale=args.scale)
             wl = {"workload": "bp2004_2d", "grid": list(info["grid"]),
                   "bbox_m": list(info["bbox"]), "hmin_m": 75.0, "grade": 0.15, "max_iter": 50}
-        elif args.workload == "eage":
-            domain, ef, info = workloads.eage3d(scale=args.scale)
-            wl = {"workload": "eage_3d", "grid": list(info["grid"]),
-                  "bbox_m": list(info["bbox"]), "hmin_m": 150.0, "grade": 0.15,
-                  "max_iter": 50, "triangulation": "host exact Bowyer-Watson per iteration"}
-        elif args.workload == "csg3d":
-            domain, bbox3 = workloads.csg3d()
-            ef = 0.03 / args.scale
-            wl = {"workload": "csg3d_neural", "primitives": "48 balls, 16 boxes, 8 cylinders, "
-                  "4 tori (seed 3)", "h": ef, "max_iter": 50,
-                  "triangulation": "host exact Bowyer-Watson per iteration"}
         else:
             domain, ef, info = workloads.disk()
             wl = {"workload": "unit_disk_h0.02", "max_iter": 50}
@@ -383,40 +612,8 @@ def main():
     dim = m.dim
     progress("autotune: %s" % {k: v[0] for k, v in tuned.items()})
 
-    neural = None
-    if args.workload == "csg3d":
-        # the exact-SDF loop against the loop steered by a surrogate trained
-        # here with the library's kernels (same points, same sizes)
-        from seismicmesh_amd.neural import NeuralSDF
-        t0 = time.perf_counter()
-        net = NeuralSDF(3, bbox3, hidden=3, seed=0, omega=8.0)
-        net.train(domain, nsamples=1000000, steps=2000, batch=8192, lr=2e-3, lr_final=1e-4,
-                  eik=0.1, near=0.05, seed=0)
-        band, err = net.calibrate_band(domain, 2.0 * ef, nval=200000)
-        t_train = time.perf_counter() - t0
-        progress("surrogate trained in %.1f s: max error %.4f, band %.4f" % (t_train, err, band))
-        exact = generation.make_mesher(st)
-        exact.set_workgroups({k: v[0] for k, v in tuned.items()})
-        m.set_surrogate(net, band)
-        elapsed, stats = timed(False)
-        p_neural = m.points()
-        plain_elapsed, plain_stats = timed(False, exact)
-        sg = m.surrogate_stats()
-        neural = {"training_s": round(t_train, 1), "steps": 2000, "samples": 1000000,
-                  "max_error": round(err, 4), "band": round(band, 4),
-                  "exact_evaluations_skipped": round(sg["skipped"] / max(sg["skipped"] +
-                                                                          sg["exact"], 1), 4),
-                  "max_error_in_band": round(sg["max_err"], 4),
-                  "points_identical_to_exact_path": bool(np.array_equal(p_neural,
-                                                                        exact.points()))}
-        exact.close()
-    elif dim == 3:
-        # 3D iterations have no captured-graph path: one timed pass
-        elapsed, stats = timed(False)
-        plain_elapsed, plain_stats = elapsed, stats
-    else:
-        plain_elapsed, plain_stats = timed(False)
-        elapsed, stats = timed(True)
+    plain_elapsed, plain_stats = timed(False)
+    elapsed, stats = timed(True)
     n, C, E = m.sizes()
     value = N * args.steps / elapsed
 
@@ -504,10 +701,6 @@ def main():
         "cpu_baseline": cpu,
         "mlp_roofline": mlp,
     }
-    if neural is not None:
-        out["config"]["surrogate"] = neural
-        out["config"]["path"] = "surrogate-steered filter and projection (exact SDF in the band)"
-        out["config"]["plain_path"] = "exact SDF everywhere"
     print(json.dumps(out))
 
 

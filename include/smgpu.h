@@ -318,20 +318,20 @@ int sm_mesher_get_edges(sm_mesher *m, int32_t *edges, int64_t cap);
  * had not converged, [6] kinetic repairs (points moved in sub-steps from the
  * last valid triangulation), [7] predicates the device's exact stage could
  * not settle (each forces a host rebuild), [8] vertices taken out and put
- * back on the host to untangle what flips could not, [9] 3D slab builds,
- * [10] slab builds redone serially because the filter kept an uncertified
- * cell, [11] uncertified cells, [12] microseconds spent in the host 3D
- * tetrahedralisation (wall clock, copies and overlapped filters included). */
+ * back on the host to untangle what flips could not, [9] 3D builds on
+ * several threads, [10] points the threads left to the final single-thread
+ * pass (their cells were locked by another thread twice), [11] 0, [12]
+ * microseconds spent in the host 3D tetrahedralisation (wall clock, copies
+ * and overlapped filters included). */
 int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats);
-/* 3D handles: build each iteration's tetrahedralisation as nslabs slabs
- * along `axis` on nthreads host threads (sm_host_delaunay3d_slabs); each
- * slab's cells are copied to the device and filtered there on the handle's
- * stream while later slabs are still being built.  Cells equal those of the
- * serial build: when the filter keeps an uncertified cell the iteration's
- * tetrahedralisation is rebuilt serially and the halo doubled.  Replaces the
- * per-rank slabs of mesh_generator.py:808-852, :867-877 and
+/* 3D handles: build each iteration's tetrahedralisation on nthreads host
+ * threads inserting the points of nslabs slabs along `axis` into one shared
+ * tetrahedralisation (sm_host_delaunay3d_slabs: the serial build's cells);
+ * its cells are copied to the device and filtered there, piece by piece, on
+ * the handle's stream while the host still hands out the next pieces.
+ * Replaces the per-rank slabs of mesh_generator.py:808-852, :867-877 and
  * decomp/blocker.py:4-111 (axis as the reference's `axis` option). */
-int sm_mesher_set_slabs(sm_mesher *m, int axis, int nslabs, double halo, int nthreads);
+int sm_mesher_set_slabs(sm_mesher *m, int axis, int nslabs, int nthreads);
 /* Per-phase kernel timing with HIP events recorded on the handle's stream.
  * Phases: 0 flip rounds, 1 centroid filter, 2 unique edges + edge lists,
  * 3 bar lengths and sizing, 4 L0 sums, 5 bar forces, 6 gather + update +
@@ -386,6 +386,14 @@ int sm_mesher_perturb_slivers(sm_mesher *m, double scale);
  * is the exact path's whenever |d_nn - d| < band - geps at every skipped
  * evaluation.  net = NULL switches the surrogate off. */
 int sm_mesher_set_surrogate(sm_mesher *m, const sm_mlp *net, double band);
+/* Per-region bands (no counterpart in the reference): bands (res^dim FP32,
+ * row-major over axes 0..dim-1, a host array) give each cell of a regular
+ * grid over [lo, hi] its own exact-fallback band, e.g. from the network's
+ * largest error measured in that cell; the band at x is then
+ * max(band, bands[cell of x]), and the largest of them outside the grid.
+ * bands = NULL goes back to the single band. */
+int sm_mesher_set_surrogate_bands(sm_mesher *m, int res, const double *lo, const double *hi,
+                                  const float *bands);
 /* Exact evaluations skipped and made since the surrogate was set, and the
  * largest |d_nn - d| over the evaluations made (the network's error in
  * the band, judged by the exact program). */
@@ -400,28 +408,35 @@ int sm_mesher_surrogate_stats(sm_mesher *m, int64_t *skipped, int64_t *exact,
 int64_t sm_host_delaunay2d(const double *p, int64_t n, int32_t *tri, int32_t *adj,
                            int64_t cap, uint8_t *inserted);
 
-/* Delaunay tetrahedralisation of p (n, 3), exact predicates (replaces
+/* Delaunay tetrahedralisation of the convex hull of p (n, 3), exact
+ * predicates with CGAL's symbolic perturbation, so that cospherical points
+ * get one tetrahedralisation whatever the insertion order (replaces
  * delaunay_class3.cpp:116-129, :276-344).  Writes the finite tetrahedra,
- * positively oriented; returns their count or an error code.  inserted[i] is
- * 0 for exact duplicates. */
+ * positively oriented, canonical (lowest id first, orientation kept) and
+ * sorted; returns their count or an error code.  inserted[i] is 0 for exact
+ * duplicates (of equal points the lowest id is kept). */
 int64_t sm_host_delaunay3d(const double *p, int64_t n, int32_t *tet, int64_t cap,
                            uint8_t *inserted);
 
-/* The same tetrahedralisation built as nslabs equal-width slabs along `axis`
- * on up to nthreads threads (replaces the slab decomposition of
+/* The same tetrahedralisation built by nthreads threads at once: the points
+ * are split into nslabs equal-width slabs along `axis` (the decomposition of
  * generation/utils.py:28-45 (make_init_points), mesh_generator.py:867-877
- * (_form_extents, +-halo along the axis) and decomp/blocker.py:4-111, whose
- * slabs the reference meshes on separate MPI ranks).  Slab k triangulates
- * the points whose axis coordinate lies within `halo` of it and reports the
- * tetrahedra whose lowest vertex id lies in it.  The first *ncertified
- * tetrahedra have circumspheres inside their slab's window and are exactly
- * those of sm_host_delaunay3d; the rest could not be certified (circumsphere
- * wider than the window) and may differ: a caller that keeps any of them
- * must rebuild with sm_host_delaunay3d.  Tetrahedra are in canonical vertex
- * order (lowest id first, orientation kept) in both functions. */
-int64_t sm_host_delaunay3d_slabs(const double *p, int64_t n, int axis, int nslabs, double halo,
-                                 int nthreads, int32_t *tet, int64_t cap, uint8_t *inserted,
-                                 int64_t *ncertified);
+ * and decomp/blocker.py:4-111, whose slabs the reference meshes on separate
+ * MPI ranks), and the threads insert the slabs' points into one shared
+ * tetrahedralisation, locking the cells they touch.  The cells are exactly
+ * those of sm_host_delaunay3d, in the same order: the perturbed predicates
+ * make the tetrahedralisation a function of the point set, and both
+ * functions return it canonical (lowest id first, orientation kept) and
+ * sorted. */
+int64_t sm_host_delaunay3d_slabs(const double *p, int64_t n, int axis, int nslabs, int nthreads,
+                                 int32_t *tet, int64_t cap, uint8_t *inserted);
+/* 1 when no point of p (n, 3) lies inside the circumsphere of any of the
+ * `count` positively oriented tetrahedra (the perturbed test of
+ * sm_host_delaunay3d, so a tetrahedron of it passes), 0 otherwise; an error
+ * code for bad arguments.  The check a slab build applies to uncertified
+ * cells that the force loop keeps. */
+int sm_host_delaunay3d_check(const double *p, int64_t n, const int32_t *tet, int64_t count,
+                             int nthreads);
 
 /* ------------------------------------------------------- 3D quality kernels */
 /* angles: (6*ncells) dihedral angles in radians, FP64 (fast_geometry.cpp

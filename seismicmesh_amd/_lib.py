@@ -85,6 +85,7 @@ HOST_SDF_FN = ctypes.CFUNCTYPE(
     ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int64,
     ctypes.POINTER(ctypes.c_double))
 
+
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
 _int = ctypes.c_int
@@ -117,7 +118,7 @@ _PROTOS = {
     "sm_mesher_get_cells": (_int, [_vp, _vp, _i64]),
     "sm_mesher_get_edges": (_int, [_vp, _vp, _i64]),
     "sm_mesher_stats": (_int, [_vp, _vp, _int]),
-    "sm_mesher_set_slabs": (_int, [_vp, _int, _int, _dbl, _int]),
+    "sm_mesher_set_slabs": (_int, [_vp, _int, _int, _int]),
     "sm_mesher_set_graph": (_int, [_vp, _int]),
     "sm_mesher_profile": (_int, [_vp, _int]),
     "sm_mesher_profile_read": (_int, [_vp, _vp, _vp, _int]),
@@ -129,6 +130,7 @@ _PROTOS = {
     "sm_mesher_find_slivers": (_int, [_vp, _dbl, _dbl, ctypes.POINTER(_i64)]),
     "sm_mesher_perturb_slivers": (_int, [_vp, _dbl]),
     "sm_mesher_set_surrogate": (_int, [_vp, _vp, _dbl]),
+    "sm_mesher_set_surrogate_bands": (_int, [_vp, _int, _vp, _vp, _vp]),
     "sm_mesher_surrogate_stats": (_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64),
                                          ctypes.POINTER(_dbl)]),
     "sm_mlp_eval": (_int, [_vp, _vp, _i64, _vp, _vp]),
@@ -144,8 +146,8 @@ _PROTOS = {
     "sm_mlp_pack_hidden": (_int, [_int, _int, _vp, _vp, _vp]),
     "sm_host_delaunay2d": (_i64, [_vp, _i64, _vp, _vp, _i64, _vp]),
     "sm_host_delaunay3d": (_i64, [_vp, _i64, _vp, _i64, _vp]),
-    "sm_host_delaunay3d_slabs": (_i64, [_vp, _i64, _int, _int, _dbl, _int, _vp, _i64, _vp,
-                                        ctypes.POINTER(_i64)]),
+    "sm_host_delaunay3d_slabs": (_i64, [_vp, _i64, _int, _int, _int, _vp, _i64, _vp]),
+    "sm_host_delaunay3d_check": (_int, [_vp, _i64, _vp, _i64, _int]),
     "sm_dihedral_angles": (_int, [_vp, _vp, _i64, _vp, _vp]),
     "sm_circumsphere_grad": (_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "sm_limgrad": (_int, [_vp, _int, _int, _int, _dbl, _dbl, _vp]),

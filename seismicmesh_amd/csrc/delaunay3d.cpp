@@ -31,23 +31,30 @@
 //
 // Slabs (the decomposition of mesh_generator.py:808-852 / decomp/blocker.py:
 // equal-width slabs along one axis).  Slab k builds the triangulation of its
-// own points plus every point whose axis coordinate lies in the window
-// [lo_k - halo, hi_k + halo].  It reports the
-// tetrahedra it owns -- those whose lowest-numbered vertex lies in slab k --
-// and certifies each one whose circumsphere spans, along the axis, less than
-// the window: every point inside or on that sphere is then in the window,
-// so the tetrahedron is Delaunay for the whole point set, and (by the
-// uniqueness above) every tetrahedron of the whole set owned by slab k whose
-// sphere fits the window is found this way.  Owned tetrahedra that cannot be
-// certified are reported after all certified ones; the caller checks that
-// none of them survives its centroid filter, and otherwise rebuilds
-// serially.
+// own points, every point whose axis coordinate lies in the window
+// [lo_k - halo, hi_k + halo], and the `nglobal` first points (the fixed
+// points, which every slab takes).  It reports the tetrahedra it owns --
+// those whose other vertices have their midpoint along the axis in slab k
+// (of all four when all are fixed) -- and certifies each one whose
+// circumsphere spans, along the axis, less
+// than the window: every point inside or on that sphere is then in the
+// window, so the tetrahedron is Delaunay for the whole point set.  By the
+// uniqueness above, every tetrahedron of the whole set whose non-fixed
+// vertices span at most 2 halo along the axis lies in its owner's window
+// and so is found there.  Owned tetrahedra that cannot be certified are
+// reported after all certified ones; the caller checks those that survive
+// its centroid filter against every point (Slabs3::verify) and rebuilds
+// serially if one of them is not Delaunay.  What this leaves open: a
+// tetrahedron whose non-fixed vertices span more than 2 halo along the axis
+// is seen by no slab; the force loop's kept cells span about one or two
+// edge lengths, and the default halo is five.
 //
 // Output tetrahedra are positively oriented and canonical: the lowest vertex
 // id first, the second-lowest next, the last two in the order that keeps
 // the orientation.  Exact duplicates are skipped: of several equal points
 // only the lowest id is used.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <condition_variable>
@@ -339,6 +346,7 @@ struct Builder3 {
       if (bnd[k].out >= 0) T[bnd[k].out].n[bnd[k].back] = ids[k];
     link_faces_through(qi, nb, [&](size_t k) { return bnd[k].i; });
     last = ids[0];
+    if ((size_t)qi < vtet.size()) vtet[qi] = ids[0];
     return 1;
   }
 
@@ -435,8 +443,9 @@ struct Builder3 {
     return k;
   }
 
-  // Triangulates the nloc points of P in Hilbert order.
-  int build(const double lo[3], double ext) {
+  // Triangulates the nloc points of P in Hilbert order (on nthreads
+  // threads, in nslabs slabs along `axis`, when that is more than one).
+  int build(const double lo[3], double ext, int nthreads = 1, int axis = 1, int nslabs = 1) {
     T.clear();
     free_slots.clear();
     dropped.clear();
@@ -456,6 +465,10 @@ struct Builder3 {
       T.clear();
       return SM_OK;  // fewer than four points off one plane: no tetrahedra
     }
+    retried = 0;
+    slab_sizes.clear();
+    if (nthreads > 1 && n >= kParallelMin)
+      return build_parallel(key, k0, nthreads, axis, nslabs);
     for (size_t k = k0; k < key.size(); ++k) {
       int rc = insert(key[k].second);
       if (rc < 0) return rc;
@@ -466,276 +479,702 @@ struct Builder3 {
     }
     return SM_OK;
   }
+
+  // ------------------------------------------------------------------
+  // Concurrent insertion into one shared triangulation.  Every thread
+  // inserts its own points; a tetrahedron (its vertices, links and cavity
+  // mark) is read or written only by the thread holding its lock, taken
+  // with a try-lock while walking to the point and growing its cavity.  A
+  // thread that finds a lock taken lets go of all its locks and tries that
+  // point again later, so no thread ever waits on another.  Since the
+  // tetrahedralisation is a function of the point set (the perturbation
+  // above), the result is the serial one whatever the interleaving.  The
+  // array of tetrahedra has a fixed capacity during the parallel phase (new
+  // slots from an atomic counter; freed slots, marked dead, reused by the
+  // thread that freed them when it can lock them), and a walk that starts
+  // from a dead slot (a stale hint) starts again from a live one.
+  static constexpr int kDead = -2;
+  static constexpr int kParallelMin = 4096;
+  static constexpr int kBlock = 64;
+  std::vector<int64_t> slab_sizes;  // points per slab of the last parallel build
+  std::vector<int> vtet;            // a tetrahedron made when each point went in
+  int64_t retried = 0;              // points left to the single-thread pass
+  struct TCtx {
+    uint32_t me = 0;
+    std::vector<int> held, cav, stack, ids, freel, retry, recent;
+    std::vector<BF> bnd;
+    std::vector<Tet> nt;
+    std::vector<Slot> table;
+    std::vector<int> used;
+    int last = 0;
+    int64_t blk = 0, blk_end = 0;  // this thread's block of fresh slots
+  };
+  std::vector<std::atomic<uint32_t>> *locks = nullptr;
+  std::atomic<int64_t> *ntet = nullptr;
+  int64_t cap_par = 0;
+  std::mutex *dup_mu = nullptr;
+
+  bool trylock(TCtx &c, int t) {
+    std::atomic<uint32_t> &l = (*locks)[t];
+    uint32_t cur = l.load(std::memory_order_relaxed);
+    if (cur == c.me) return true;
+    if (cur != 0) return false;
+    if (!l.compare_exchange_strong(cur, c.me, std::memory_order_acquire)) return false;
+    c.held.push_back(t);
+    return true;
+  }
+  // lets go of every lock of c but the one on t
+  void keep_only(TCtx &c, int t) {
+    for (int h : c.held)
+      if (h != t) {
+        mark[h] = 0;
+        (*locks)[h].store(0, std::memory_order_release);
+      }
+    c.held.clear();
+    c.held.push_back(t);
+  }
+  void unlock_all(TCtx &c) {
+    for (int t : c.held) {
+      mark[t] = 0;
+      (*locks)[t].store(0, std::memory_order_release);
+    }
+    c.held.clear();
+  }
+  int par_new_slot(TCtx &c) {
+    while (!c.freel.empty()) {
+      const int t = c.freel.back();
+      c.freel.pop_back();
+      // a stale hint of another thread may hold it: then it stays dead
+      if (trylock(c, t) && T[t].v[0] == kDead) return t;
+    }
+    // fresh slots come in blocks, so that threads do not share the
+    // cache lines of what they write; nobody knows of a fresh slot
+    // before it is handed out, so its lock is free
+    for (;;) {
+      if (c.blk >= c.blk_end) {
+        c.blk = ntet->fetch_add(kBlock);
+        if (c.blk >= cap_par) return -1;
+        c.blk_end = std::min<int64_t>(c.blk + kBlock, cap_par);
+      }
+      const int t = (int)c.blk++;
+      if (trylock(c, t)) return t;
+    }
+  }
+  bool live_locked(TCtx &c, int t) {
+    return t >= 0 && t < ntet->load(std::memory_order_relaxed) && t < cap_par &&
+           trylock(c, t) && T[t].v[0] != kDead;
+  }
+
+  // 1 inserted, 0 duplicate, 2 retry later (a lock was taken), < 0 failure
+  int par_insert(TCtx &c, int qi) {
+    const double *q = pt(qi);
+    int t = c.last;
+    if (!live_locked(c, t)) {
+      t = -1;
+      while (!c.recent.empty() && t < 0) {
+        if (live_locked(c, c.recent.back())) t = c.recent.back();
+        else c.recent.pop_back();
+      }
+      if (t < 0) {
+        unlock_all(c);
+        return 2;
+      }
+    }
+    // walk, hand over hand: the next tetrahedron is locked before the
+    // current one is let go
+    keep_only(c, t);
+    int t0 = -1;
+    for (size_t steps = 0; steps < 1000000; ++steps) {
+      const Tet &r = T[t];
+      const int is = inf_slot(r);
+      if (is >= 0) {
+        const int nb = r.n[is];
+        if (!trylock(c, nb)) {
+          unlock_all(c);
+          return 2;
+        }
+        keep_only(c, nb);
+        t = nb;
+        continue;
+      }
+      int moved = -1;
+      for (int i = 0; i < 4 && moved < 0; ++i)
+        if (orient_with(r, i, q) < 0) moved = i;
+      if (moved < 0) {
+        for (int i = 0; i < 4; ++i) {
+          const double *V = pt(r.v[i]);
+          if (V[0] == q[0] && V[1] == q[1] && V[2] == q[2]) {
+            const int dup = r.v[i];
+            {
+              std::lock_guard<std::mutex> lk(*dup_mu);
+              const int64_t keep = std::min(gid[dup], gid[qi]);
+              const int64_t drop = std::max(gid[dup], gid[qi]);
+              gid[dup] = keep;
+              dropped.push_back(drop);
+            }
+            c.last = t;
+            unlock_all(c);
+            return 0;
+          }
+        }
+        t0 = t;
+        break;
+      }
+      const int nb = r.n[moved];
+      if (nb < 0) {
+        unlock_all(c);
+        return SM_ERR_NUMERIC;
+      }
+      if (!trylock(c, nb)) {
+        unlock_all(c);
+        return 2;
+      }
+      if (inf_slot(T[nb]) >= 0) {
+        t0 = nb;  // q lies beyond this hull facet
+        break;
+      }
+      keep_only(c, nb);
+      t = nb;
+    }
+    if (t0 < 0) {
+      unlock_all(c);
+      return SM_ERR_NUMERIC;
+    }
+    // cavity: every neighbour looked at is locked first (its mark, links
+    // and vertices are then this thread's), and so is the finite
+    // tetrahedron behind a tested infinite one
+    c.cav.clear();
+    c.bnd.clear();
+    c.stack.assign(1, t0);
+    mark[t0] = 1;
+    while (!c.stack.empty()) {
+      const int tt = c.stack.back();
+      c.stack.pop_back();
+      c.cav.push_back(tt);
+      for (int i = 0; i < 4; ++i) {
+        const int nb = T[tt].n[i];
+        if (nb >= 0 && !trylock(c, nb)) {
+          unlock_all(c);
+          return 2;
+        }
+        if (nb >= 0 && mark[nb] == 1) continue;
+        bool in = false;
+        if (nb >= 0 && mark[nb] == 0) {
+          const int is = inf_slot(T[nb]);
+          if (is >= 0 && !trylock(c, T[nb].n[is])) {
+            unlock_all(c);
+            return 2;
+          }
+          in = conflict(nb, q);
+          mark[nb] = in ? 1 : 2;
+          if (in) c.stack.push_back(nb);
+        }
+        if (!in) {
+          int back = -1;
+          if (nb >= 0)
+            for (int j = 0; j < 4; ++j)
+              if (T[nb].n[j] == tt) back = j;
+          c.bnd.push_back({tt, i, nb, back});
+        }
+      }
+    }
+    const size_t nbd = c.bnd.size(), nc = c.cav.size();
+    c.nt.resize(nbd);
+    for (size_t k = 0; k < nbd; ++k) {
+      Tet r = T[c.bnd[k].t];
+      r.v[c.bnd[k].i] = qi;
+      for (int j = 0; j < 4; ++j) r.n[j] = -1;
+      r.n[c.bnd[k].i] = c.bnd[k].out;
+      c.nt[k] = r;
+    }
+    c.ids.resize(nbd);
+    for (size_t k = 0; k < nbd && k < nc; ++k) c.ids[k] = c.cav[k];
+    for (size_t k = nc; k < nbd; ++k) {
+      const int s2 = par_new_slot(c);
+      if (s2 < 0) {
+        unlock_all(c);
+        return SM_ERR_CAPACITY;
+      }
+      c.ids[k] = s2;
+    }
+    for (size_t k = nbd; k < nc; ++k) {
+      T[c.cav[k]].v[0] = kDead;
+      c.freel.push_back(c.cav[k]);
+    }
+    for (size_t k = 0; k < nbd; ++k) T[c.ids[k]] = c.nt[k];
+    for (size_t k = 0; k < nbd; ++k)
+      if (c.bnd[k].out >= 0) T[c.bnd[k].out].n[c.bnd[k].back] = c.ids[k];
+    // link the new tetrahedra through their faces containing q
+    size_t tcap = 64;
+    while (tcap < 4 * 3 * nbd) tcap <<= 1;
+    if (c.table.size() < tcap) c.table.assign(tcap, Slot{~0ull, -1, -1});
+    const size_t mask = tcap - 1;
+    c.used.clear();
+    for (size_t k = 0; k < nbd; ++k) {
+      const Tet &r = T[c.ids[k]];
+      const int qs = c.bnd[k].i;
+      for (int j = 0; j < 4; ++j) {
+        if (j == qs) continue;
+        const int s0 = kOther[qs][j][0], s1 = kOther[qs][j][1];
+        const uint32_t e0 = (uint32_t)r.v[s0], e1 = (uint32_t)r.v[s1];
+        const uint64_t key2 = ((uint64_t)std::min(e0, e1) << 32) | std::max(e0, e1);
+        size_t h = (size_t)((key2 * 0x9E3779B97F4A7C15ull) >> 32) & mask;
+        while (c.table[h].key != ~0ull && c.table[h].key != key2) h = (h + 1) & mask;
+        if (c.table[h].key == key2) {
+          const int ot = c.table[h].t, oj = c.table[h].j;
+          T[c.ids[k]].n[j] = ot;
+          T[ot].n[oj] = c.ids[k];
+        } else {
+          c.table[h] = Slot{key2, c.ids[k], j};
+          c.used.push_back((int)h);
+        }
+      }
+    }
+    for (int h : c.used) c.table[h].key = ~0ull;
+    c.last = c.ids[0];
+    vtet[qi] = c.ids[0];
+    c.recent.push_back(c.ids[0]);
+    if (c.recent.size() > 64) c.recent.erase(c.recent.begin(), c.recent.begin() + 32);
+    unlock_all(c);
+    return 1;
+  }
+
+  // Inserts the points order[k0..] on nthreads threads: every 64th point
+  // first on this thread (spread over the whole curve), then the rest in
+  // chunks, then, on this thread alone, those that met a taken lock twice.
+  // The rest is cut into nslabs equal-width slabs along `axis` (each in
+  // curve order) and every slab into chunks; the chunks are handed out
+  // slab after slab in turn (chunk 0 of every slab, then chunk 1, ...), so
+  // that the threads insert in different slabs at any time.
+  int build_parallel(const std::vector<std::pair<uint64_t, int>> &order, size_t k0,
+                     int nthreads, int axis, int nslabs) {
+    const size_t n = order.size();
+    std::vector<int> early, rest0, pred0;
+    vtet.assign((size_t)nloc, -1);
+    for (size_t k = k0; k < n; ++k) {
+      if ((k - k0) % 64 == 0) {
+        early.push_back(order[k].second);
+      } else {
+        rest0.push_back(order[k].second);
+        pred0.push_back(early.back());  // the early point just before it on the curve
+      }
+    }
+    double xlo = std::numeric_limits<double>::infinity(), xhi = -xlo;
+    for (int v : rest0) {
+      xlo = std::min(xlo, pt(v)[axis]);
+      xhi = std::max(xhi, pt(v)[axis]);
+    }
+    const int S = std::max(1, nslabs);
+    const double width = (xhi - xlo) / S;
+    std::vector<std::vector<int>> slab((size_t)S), spred((size_t)S);
+    for (size_t i = 0; i < rest0.size(); ++i) {
+      const int v = rest0[i];
+      int k = width > 0 ? (int)std::floor((pt(v)[axis] - xlo) / width) : 0;
+      k = std::min(S - 1, std::max(0, k));
+      slab[(size_t)k].push_back(v);
+      spred[(size_t)k].push_back(pred0[i]);
+    }
+    slab_sizes.assign((size_t)S, 0);
+    for (int k = 0; k < S; ++k) slab_sizes[k] = (int64_t)slab[k].size();
+    for (int v : early) {
+      int rc = insert(v);
+      if (rc < 0) return rc;
+    }
+    for (int v : std::vector<int>(deferred)) {
+      int rc = insert(v);
+      if (rc < 0) return rc;
+    }
+    deferred.clear();
+    // fixed capacity; the dead slots of the serial phase go to thread 0
+    cap_par = std::max<int64_t>((int64_t)T.size() + 64, 8 * (int64_t)nloc + 4096);
+    if (cap_par > (int64_t)std::numeric_limits<int>::max())
+      return sm_set_error(SM_ERR_CAPACITY, "host 3D Delaunay: too many points");
+    std::atomic<int64_t> nt_count{(int64_t)T.size()};
+    const size_t old_n = T.size();
+    T.resize((size_t)cap_par);
+    for (size_t t = old_n; t < (size_t)cap_par; ++t) T[t].v[0] = kDead;  // fresh
+    for (int f : free_slots) T[f].v[0] = kDead;
+    mark.assign((size_t)cap_par, 0);
+    std::vector<std::atomic<uint32_t>> lk((size_t)cap_par);
+    for (auto &a : lk) a.store(0, std::memory_order_relaxed);
+    std::mutex dmu;
+    locks = &lk;
+    ntet = &nt_count;
+    dup_mu = &dmu;
+    const int nthr = std::max(1, nthreads);
+    std::vector<TCtx> ctx((size_t)nthr);
+    ctx[0].freel = free_slots;
+    free_slots.clear();
+    for (int w = 0; w < nthr; ++w) {
+      ctx[w].me = (uint32_t)w + 1;
+      ctx[w].last = last;
+      ctx[w].recent.push_back(last);
+    }
+    // chunks: (slab, begin, end), interleaved across the slabs
+    const size_t chunk =
+        std::max<size_t>(256, rest0.size() / ((size_t)nthr * 8) + 1);
+    std::vector<std::array<size_t, 3>> chunks;
+    for (size_t r = 0;; ++r) {
+      bool any = false;
+      for (int k = 0; k < S; ++k)
+        if (r * chunk < slab[k].size()) {
+          chunks.push_back({(size_t)k, r * chunk, std::min(slab[k].size(), (r + 1) * chunk)});
+          any = true;
+        }
+      if (!any) break;
+    }
+    const size_t nchunks = chunks.size();
+    std::atomic<size_t> next{0};
+    std::atomic<int> fail{0};
+    auto worker = [&](int w) {
+      TCtx &c = ctx[w];
+      for (;;) {
+        const size_t ch = next.fetch_add(1);
+        if (ch >= nchunks || fail.load(std::memory_order_relaxed)) break;
+        const std::vector<int> &pts = slab[chunks[ch][0]];
+        // start near the chunk, at the tetrahedron made by the early point
+        // before its first point on the curve when that one still lives
+        // (the previous position stays as the fallback)
+        {
+          const int hv = vtet[spred[chunks[ch][0]][chunks[ch][1]]];
+          if (hv >= 0) {
+            c.recent.push_back(c.last);
+            c.last = hv;
+          }
+        }
+        std::vector<int> again;
+        for (size_t k = chunks[ch][1]; k < chunks[ch][2]; ++k) {
+          const int rc = par_insert(c, pts[k]);
+          if (rc == 2) again.push_back(pts[k]);
+          else if (rc < 0) {
+            fail = rc;
+            return;
+          }
+        }
+        for (int v : again) {
+          const int rc = par_insert(c, v);
+          if (rc == 2) c.retry.push_back(v);
+          else if (rc < 0) {
+            fail = rc;
+            return;
+          }
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nthr; ++w) pool.emplace_back(worker, w);
+    worker(0);
+    for (auto &t : pool) t.join();
+    int rc = fail.load();
+    // what is left: one thread, no contention (every lock was released)
+    retried = 0;
+    for (int w = 0; w < nthr; ++w) retried += (int64_t)ctx[w].retry.size();
+    for (int w = 0; w < nthr && rc == 0; ++w)
+      for (int v : ctx[w].retry) {
+        const int r2 = par_insert(ctx[0], v);
+        if (r2 == 2 || r2 < 0) {
+          rc = r2 < 0 ? r2 : SM_ERR_NUMERIC;
+          break;
+        }
+      }
+    // back to the serial layout: dead slots become free slots
+    const int64_t used_n = std::min<int64_t>(nt_count.load(), cap_par);
+    T.resize((size_t)std::max<int64_t>(used_n, (int64_t)old_n));
+    free_slots.clear();
+    for (size_t t = 0; t < T.size(); ++t)
+      if (T[t].v[0] == kDead) free_slots.push_back((int)t);
+    mark.assign(T.size() + 64, 0);
+    locks = nullptr;
+    ntet = nullptr;
+    dup_mu = nullptr;
+    last = 0;
+    for (size_t t = 0; t < T.size(); ++t)
+      if (T[t].v[0] != kDead) {
+        last = (int)t;
+        break;
+      }
+    return rc;
+  }
 };
 
-namespace {
 
-// Half-width of the sphere of a..d along `axis` and its centre coordinate,
-// with a bound on their rounding error; false when the tetrahedron is too
-// flat for the bound to mean anything.
-bool sphere_span(const double *a, const double *b, const double *c, const double *d, int axis,
-                 double *centre, double *radius, double *err) {
-  double u[3], v[3], w[3];
-  for (int k = 0; k < 3; ++k) {
-    u[k] = b[k] - a[k];
-    v[k] = c[k] - a[k];
-    w[k] = d[k] - a[k];
-  }
-  const double vw[3] = {v[1] * w[2] - v[2] * w[1], v[2] * w[0] - v[0] * w[2],
-                        v[0] * w[1] - v[1] * w[0]};
-  const double wu[3] = {w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2],
-                        w[0] * u[1] - w[1] * u[0]};
-  const double uv[3] = {u[1] * v[2] - u[2] * v[1], u[2] * v[0] - u[0] * v[2],
-                        u[0] * v[1] - u[1] * v[0]};
-  const double den = 2.0 * (u[0] * vw[0] + u[1] * vw[1] + u[2] * vw[2]);
-  const double lu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
-  const double lv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-  const double lw = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
-  const double L2 = std::max(lu, std::max(lv, lw));
-  const double L3 = L2 * std::sqrt(L2);
-  if (!(std::fabs(den) > 1e-9 * L3)) return false;
-  double o[3];
-  for (int k = 0; k < 3; ++k) o[k] = (lu * vw[k] + lv * wu[k] + lw * uv[k]) / den;
-  const double r = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
-  const double eps = std::numeric_limits<double>::epsilon();
-  // forward bound: numerator and denominator each within ~10 roundings of
-  // their magnitudes (3 L^4 and 2 L^3), plus the final additions
-  const double e = 64.0 * eps * (3.0 * L2 * L2 + r * 2.0 * L3) / std::fabs(den) +
-                   8.0 * eps * (std::fabs(a[axis]) + r);
-  *centre = a[axis] + o[axis];
-  *radius = r;
-  *err = e;
-  return std::isfinite(*centre) && std::isfinite(r) && std::isfinite(e);
-}
+// ------------------------------------------------------------ driver
+DT3::DT3() : B(new Builder3()) {}
+DT3::~DT3() = default;
 
-}  // namespace
-
-// ------------------------------------------------------------ slab driver
-Slabs3::Slabs3() = default;
-Slabs3::~Slabs3() = default;
-
-int Slabs3::run(const double *p, int64_t n, int axis, int nslabs, double halo, int nthreads,
-                const Emit &emit) {
-  if (n < 0 || (n > 0 && !p) || axis < 0 || axis > 2 || nslabs < 1 || !(halo >= 0.0))
+int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, const Emit &emit) {
+  if (n < 0 || (n > 0 && !p) || axis < 0 || axis > 2 || nslabs < 1 || nthreads < 1)
     return sm_set_error(SM_ERR_ARG, "delaunay3d: bad arguments");
-  if (n > (int64_t)std::numeric_limits<int32_t>::max() - 8)
+  if (n > (int64_t)std::numeric_limits<int32_t>::max() / 8)
     return sm_set_error(SM_ERR_ARG, "delaunay3d: too many points");
   stats = Stats{};
   inserted.assign((size_t)n, 1);
-  const int S = nslabs;
-  certified.resize(S);
-  uncertified.resize(S);
-  for (int k = 0; k < S; ++k) {
-    certified[k].clear();
-    uncertified[k].clear();
+  Builder3 &b = *B;
+  b.nloc = (int)n;
+  b.P.assign(p, p + 3 * (size_t)n);
+  b.gid.resize((size_t)n);
+  for (int64_t i = 0; i < n; ++i) b.gid[i] = i;
+  if (n >= 4) {
+    double lo[3] = {p[0], p[1], p[2]}, hi[3] = {p[0], p[1], p[2]};
+    for (int64_t i = 1; i < n; ++i)
+      for (int k = 0; k < 3; ++k) {
+        lo[k] = std::min(lo[k], p[3 * i + k]);
+        hi[k] = std::max(hi[k], p[3 * i + k]);
+      }
+    double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    if (!(ext > 0)) ext = 1.0;
+    int rc;
+    try {
+      rc = b.build(lo, ext, nthreads, axis, nslabs);
+    } catch (const std::bad_alloc &) {
+      rc = SM_ERR_CAPACITY;
+    }
+    if (rc) return sm_set_error(rc, "host 3D Delaunay failed (%d)", rc);
+    for (int64_t g : b.dropped) inserted[g] = 0;
+    stats.parallel = b.slab_sizes.empty() ? 0 : 1;
+    stats.retried = b.retried;
+    stats.slab_points = b.slab_sizes;
+  } else {
+    b.T.clear();
+    b.free_slots.clear();
   }
-  if (n < 4) {
-    for (int k = 0; k < S; ++k) emit(k, nullptr, 0);
-    emit(-1, nullptr, 0);
-    return SM_OK;
+  // finite tetrahedra, canonical, extracted by the threads in ranges of
+  // slots, then put in increasing order of (lowest id, the rest): a
+  // counting sort on the lowest id and a sort within each of its short
+  // buckets -- the order is then that of the serial build too
+  const int64_t nslot = (int64_t)b.T.size();
+  for (int f : b.free_slots) b.T[f].v[0] = Builder3::kDead;  // free slots: stale contents
+  const int nthr = std::max(1, std::min(nthreads, (int)(nslot / 65536 + 1)));
+  auto on_threads = [nthr](const std::function<void(int)> &fn) {
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nthr; ++w) pool.emplace_back(fn, w);
+    fn(0);
+    for (auto &t : pool) t.join();
+  };
+  std::vector<std::vector<int32_t>> part((size_t)nthr);
+  auto extract = [&](int w) {
+    const int64_t s0 = nslot * w / nthr, s1 = nslot * (w + 1) / nthr;
+    std::vector<int32_t> &out = part[w];
+    out.clear();
+    for (int64_t t = s0; t < s1; ++t) {
+      const Tet &r = b.T[t];
+      if (r.v[0] < 0 || Builder3::inf_slot(r) >= 0) continue;  // dead or infinite
+      int32_t v[4];
+      for (int j = 0; j < 4; ++j) v[j] = (int32_t)b.gid[r.v[j]];
+      canonical_tet(v);
+      out.insert(out.end(), v, v + 4);
+    }
+  };
+  on_threads(extract);
+  int64_t m = 0;
+  for (auto &v : part) m += (int64_t)v.size() / 4;
+  // each thread owns the buckets of a range of lowest ids: it counts them,
+  // then (after one scan of the range totals) places and sorts them
+  start_.assign((size_t)n + 1, 0);
+  sorted_.resize((size_t)m * 4);
+  std::vector<int64_t> below((size_t)nthr + 1, 0);
+  auto lo_id = [&](int w) { return n * w / nthr; };
+  on_threads([&](int w) {
+    const int64_t a = lo_id(w), b = lo_id(w + 1);
+    for (auto &v : part)
+      for (size_t i = 0; i < v.size(); i += 4)
+        if (v[i] >= a && v[i] < b) ++start_[(size_t)v[i] + 1];
+    int64_t s = 0;
+    for (int64_t i = a; i < b; ++i) s += start_[(size_t)i + 1];
+    below[(size_t)w + 1] = s;
+  });
+  for (int w = 0; w < nthr; ++w) below[(size_t)w + 1] += below[(size_t)w];
+  auto *cells = reinterpret_cast<std::array<int32_t, 4> *>(sorted_.data());
+  on_threads([&](int w) {
+    // start_[i + 1] holds the count of bucket i; this thread writes
+    // start_[a + 1 .. b] only
+    const int64_t a = lo_id(w), b = lo_id(w + 1);
+    std::vector<int64_t> fill((size_t)(b - a));
+    int64_t run = below[(size_t)w];
+    for (int64_t i = a; i < b; ++i) {
+      fill[(size_t)(i - a)] = run;
+      run += start_[(size_t)i + 1];
+      start_[(size_t)i + 1] = run;
+    }
+    for (auto &v : part)
+      for (size_t i = 0; i < v.size(); i += 4)
+        if (v[i] >= a && v[i] < b) {
+          const int64_t o = fill[(size_t)(v[i] - a)]++;
+          for (int j = 0; j < 4; ++j) sorted_[4 * o + j] = v[i + j];
+        }
+    for (int64_t i = a; i < b; ++i) {
+      const int64_t s0 = i == a ? below[(size_t)w] : start_[(size_t)i];
+      if (start_[(size_t)i + 1] - s0 > 1) std::sort(cells + s0, cells + start_[(size_t)i + 1]);
+    }
+  });
+  // handed out in nslabs contiguous pieces
+  const int pieces = std::max(1, nslabs);
+  for (int k = 0; k < pieces; ++k) {
+    const int64_t c0 = m * k / pieces, c1 = m * (k + 1) / pieces;
+    emit(k, sorted_.data() + 4 * c0, c1 - c0);
   }
-  double lo[3] = {p[0], p[1], p[2]}, hi[3] = {p[0], p[1], p[2]};
+  emit(-1, nullptr, 0);
+  return SM_OK;
+}
+
+int DT3::verify(const double *p, int64_t n, const int32_t *cells, int64_t count,
+                   int nthreads, int64_t max_bins) {
+  if (count == 0) return 1;
+  if (n <= 0) return 0;
+  bin_points(p, n);
+  std::atomic<int> ok{1};
+  std::atomic<int64_t> next{0};
+  auto worker = [&]() {
+    for (;;) {
+      const int64_t c = next.fetch_add(1);
+      if (c >= count || !ok.load(std::memory_order_relaxed)) return;
+      // every point is tested (an empty skip range)
+      if (sphere_empty(p, cells + 4 * c, 0, 1.0, 0.0, max_bins) != 1) ok = 0;
+    }
+  };
+  const int nthr = (int)std::max<int64_t>(1, std::min<int64_t>(nthreads, count / 16 + 1));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nthr; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto &t : pool) t.join();
+  return ok.load();
+}
+
+
+void DT3::bin_points(const double *p, int64_t n) {
+  // bins of about two points each over the points' box
+  for (int k = 0; k < 3; ++k) blo_[k] = bhi_[k] = p[k];
   for (int64_t i = 1; i < n; ++i)
     for (int k = 0; k < 3; ++k) {
-      lo[k] = std::min(lo[k], p[3 * i + k]);
-      hi[k] = std::max(hi[k], p[3 * i + k]);
+      blo_[k] = std::min(blo_[k], p[3 * i + k]);
+      bhi_[k] = std::max(bhi_[k], p[3 * i + k]);
     }
-  double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
-  if (!(ext > 0)) ext = 1.0;
-  // slab of each point and the window of each slab
-  const double x0 = lo[axis];
-  const double width = (hi[axis] - lo[axis]) / S;
-  core.resize((size_t)n);
-  win_lo.assign(S, -std::numeric_limits<double>::infinity());
-  win_hi.assign(S, std::numeric_limits<double>::infinity());
-  if (S > 1) {
-    for (int k = 0; k < S; ++k) {
-      win_lo[k] = x0 + k * width - halo;
-      win_hi[k] = x0 + (k + 1) * width + halo;
+  double vol = 1.0;
+  int dims = 0;
+  for (int k = 0; k < 3; ++k)
+    if (bhi_[k] > blo_[k]) {
+      vol *= bhi_[k] - blo_[k];
+      ++dims;
     }
-  }
-  members.resize(S);
-  for (int k = 0; k < S; ++k) members[k].clear();
+  const double cell = dims ? std::pow(vol * 2.0 / (double)n, 1.0 / dims) : 1.0;
+  for (int k = 0; k < 3; ++k)
+    nb_[k] = std::max<int64_t>(1, std::min<int64_t>(1024, (int64_t)((bhi_[k] - blo_[k]) / cell) + 1));
+  const int64_t nbins = nb_[0] * nb_[1] * nb_[2];
+  bin_start.assign(nbins + 1, 0);
+  bin_pts.resize((size_t)n);
+  std::vector<int64_t> idx((size_t)n);
   for (int64_t i = 0; i < n; ++i) {
-    const double x = p[3 * i + axis];
-    int s = 0;
-    if (S > 1 && width > 0) s = (int)std::floor((x - x0) / width);
-    s = std::min(S - 1, std::max(0, s));
-    core[i] = s;
-    members[s].push_back((int32_t)i);
-    if (S > 1) {
-      // neighbouring windows that also hold x (and never the core slab twice)
-      for (int k = s - 1; k >= 0 && x >= win_lo[k] && x <= win_hi[k]; --k)
-        members[k].push_back((int32_t)i);
-      for (int k = s + 1; k < S && x >= win_lo[k] && x <= win_hi[k]; ++k)
-        members[k].push_back((int32_t)i);
-    }
+    const double *q = p + 3 * i;
+    idx[i] = (bin_of(q[0], 0) * nb_[1] + bin_of(q[1], 1)) * nb_[2] + bin_of(q[2], 2);
+    ++bin_start[idx[i] + 1];
   }
-  const int nthr = std::max(1, std::min(nthreads, S));
-  if ((int)builders.size() < nthr) builders.resize(nthr);
-  for (auto &b : builders)
-    if (!b) b.reset(new Builder3());
+  for (int64_t b = 0; b < nbins; ++b) bin_start[b + 1] += bin_start[b];
+  std::vector<int32_t> fillp(bin_start.begin(), bin_start.end() - 1);
+  for (int64_t i = 0; i < n; ++i) bin_pts[fillp[idx[i]]++] = (int32_t)i;
+}
 
-  // worker: builds slab k on builder w and sorts its owned tetrahedra into
-  // the certified and uncertified lists
-  auto work = [&](int k, Builder3 &B) -> int {
-    // ids in increasing order (the points were visited in order)
-    const std::vector<int32_t> &ids = members[k];
-    const int m = (int)ids.size();
-    B.nloc = m;
-    B.P.resize(3 * (size_t)m);
-    B.gid.resize((size_t)m);
-    double llo[3], lhi[3];
-    for (int k2 = 0; k2 < 3; ++k2) {
-      llo[k2] = std::numeric_limits<double>::infinity();
-      lhi[k2] = -std::numeric_limits<double>::infinity();
-    }
-    for (int j = 0; j < m; ++j) {
-      const double *q = p + 3 * (size_t)ids[j];
-      for (int k2 = 0; k2 < 3; ++k2) {
-        B.P[3 * (size_t)j + k2] = q[k2];
-        llo[k2] = std::min(llo[k2], q[k2]);
-        lhi[k2] = std::max(lhi[k2], q[k2]);
-      }
-      B.gid[j] = ids[j];
-    }
-    double lext = 0;
-    for (int k2 = 0; k2 < 3; ++k2) lext = std::max(lext, lhi[k2] - llo[k2]);
-    if (!(lext > 0)) lext = 1.0;
-    if (m == 0) return SM_OK;
-    int rc = B.build(llo, lext);
-    if (rc) return rc;
-    for (int64_t g : B.dropped)
-      if (core[g] == k) inserted[g] = 0;
-    std::vector<char> dead(B.T.size(), 0);
-    for (int f : B.free_slots) dead[f] = 1;
-    std::vector<int32_t> &cert = certified[k], &unc = uncertified[k];
-    for (size_t t = 0; t < B.T.size(); ++t) {
-      if (dead[t]) continue;
-      const Tet &r = B.T[t];
-      if (Builder3::inf_slot(r) >= 0) continue;
-      int32_t v[4];
-      for (int j = 0; j < 4; ++j) v[j] = (int32_t)B.gid[r.v[j]];
-      int32_t vmin = std::min(std::min(v[0], v[1]), std::min(v[2], v[3]));
-      if (core[vmin] != k) continue;
-      bool ok = true;
-      if (S > 1) {
-        double cx, rad, err;
-        ok = sphere_span(B.pt(r.v[0]), B.pt(r.v[1]), B.pt(r.v[2]), B.pt(r.v[3]), axis, &cx,
-                         &rad, &err) &&
-             cx - rad - err >= win_lo[k] && cx + rad + err <= win_hi[k];
-      }
-      canonical_tet(v);
-      std::vector<int32_t> &dst = ok ? cert : unc;
-      dst.insert(dst.end(), v, v + 4);
-    }
-    return SM_OK;
-  };
+int64_t DT3::bin_of(double x, int k) const {
+  const double c = bhi_[k] > blo_[k] ? (x - blo_[k]) / (bhi_[k] - blo_[k]) * (double)nb_[k] : 0.0;
+  if (!(c >= 0.0)) return 0;
+  return std::min<int64_t>(nb_[k] - 1, (int64_t)c);
+}
 
-  auto guarded = [&](int k, Builder3 &B) -> int {
-    try {
-      return work(k, B);
-    } catch (const std::bad_alloc &) {
-      return SM_ERR_CAPACITY;
-    }
-  };
-  std::vector<int> status(S, 1);  // 1 pending, 0 done, < 0 failed
-  std::mutex mu;
-  std::condition_variable cv;
-  std::atomic<int> next{0};
-  auto worker = [&](int w) {
-    for (;;) {
-      const int k = next.fetch_add(1);
-      if (k >= S) return;
-      int rc = guarded(k, *builders[w]);
-      {
-        std::lock_guard<std::mutex> lk(mu);
-        status[k] = rc < 0 ? rc : 0;
-      }
-      cv.notify_all();
-    }
-  };
-  std::vector<std::thread> pool;
-  for (int w = 1; w < nthr; ++w) pool.emplace_back(worker, w);
-  // the calling thread builds too, then hands finished slabs out in order
-  int first_err = 0;
-  int handed = 0;
-  auto hand_out = [&](bool wait) {
-    while (handed < S) {
-      int st;
-      {
-        std::unique_lock<std::mutex> lk(mu);
-        if (wait)
-          cv.wait(lk, [&] { return status[handed] != 1; });
-        else if (status[handed] == 1)
-          return;
-        st = status[handed];
-      }
-      if (st < 0 && !first_err) first_err = st;
-      if (!first_err) {
-        const auto &v = certified[handed];
-        emit(handed, v.data(), (int64_t)(v.size() / 4));
-      }
-      ++handed;
-    }
-  };
-  for (;;) {
-    const int k = next.fetch_add(1);
-    if (k >= S) break;
-    int rc = guarded(k, *builders[0]);
-    {
-      std::lock_guard<std::mutex> lk(mu);
-      status[k] = rc < 0 ? rc : 0;
-    }
-    hand_out(false);
+// 1: no point of p outside [skip_lo, skip_hi] along `axis` (all points when
+// the range is empty) lies inside the sphere of the positively oriented
+// cell v; 0: one does; -1: the sphere covers more than max_bins bins.
+int DT3::sphere_empty(const double *p, const int32_t *v, int axis, double skip_lo,
+                         double skip_hi, int64_t max_bins) const {
+  const double *a = p + 3 * (size_t)v[0], *b = p + 3 * (size_t)v[1];
+  const double *cc = p + 3 * (size_t)v[2], *d = p + 3 * (size_t)v[3];
+  double u[3], w[3], z[3];
+  for (int k = 0; k < 3; ++k) {
+    u[k] = b[k] - a[k];
+    w[k] = cc[k] - a[k];
+    z[k] = d[k] - a[k];
   }
-  hand_out(true);
-  for (auto &t : pool) t.join();
-  if (first_err) return sm_set_error(first_err, "host 3D Delaunay failed (%d)", first_err);
-  // the uncertified lists, concatenated, last
-  unc_all.clear();
-  for (int k = 0; k < S; ++k) {
-    unc_all.insert(unc_all.end(), uncertified[k].begin(), uncertified[k].end());
-    stats.certified += (int64_t)certified[k].size() / 4;
-    stats.local_points += (int64_t)members[k].size();
+  const double wz[3] = {w[1] * z[2] - w[2] * z[1], w[2] * z[0] - w[0] * z[2],
+                        w[0] * z[1] - w[1] * z[0]};
+  const double zu[3] = {z[1] * u[2] - z[2] * u[1], z[2] * u[0] - z[0] * u[2],
+                        z[0] * u[1] - z[1] * u[0]};
+  const double uw[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2],
+                        u[0] * w[1] - u[1] * w[0]};
+  const double den = 2.0 * (u[0] * wz[0] + u[1] * wz[1] + u[2] * wz[2]);
+  const double lu = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+  const double lw = w[0] * w[0] + w[1] * w[1] + w[2] * w[2];
+  const double lz = z[0] * z[0] + z[1] * z[1] + z[2] * z[2];
+  double o[3];
+  for (int k = 0; k < 3; ++k) o[k] = (lu * wz[k] + lw * zu[k] + lz * uw[k]) / den;
+  const double r = std::sqrt(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]);
+  if (!std::isfinite(r)) return -1;
+  // a generous margin: the box only limits which points are tested
+  const double L = std::sqrt(std::max(lu, std::max(lw, lz)));
+  const double rr = r * (1.0 + 1e-6) + 1e-9 * L;
+  int64_t b0[3], b1[3], span = 1;
+  for (int k = 0; k < 3; ++k) {
+    b0[k] = bin_of(a[k] + o[k] - rr, k);
+    b1[k] = bin_of(a[k] + o[k] + rr, k);
+    span *= b1[k] - b0[k] + 1;
   }
-  stats.uncertified = (int64_t)unc_all.size() / 4;
-  emit(-1, unc_all.data(), stats.uncertified);
-  return SM_OK;
+  if (span > max_bins) return -1;
+  for (int64_t i = b0[0]; i <= b1[0]; ++i)
+    for (int64_t j = b0[1]; j <= b1[1]; ++j)
+      for (int64_t k = b0[2]; k <= b1[2]; ++k) {
+        const int64_t bb = (i * nb_[1] + j) * nb_[2] + k;
+        for (int32_t s = bin_start[bb]; s < bin_start[bb + 1]; ++s) {
+          const int32_t q = bin_pts[s];
+          if (q == v[0] || q == v[1] || q == v[2] || q == v[3]) continue;
+          const double *Q = p + 3 * (size_t)q;
+          if (Q[axis] >= skip_lo && Q[axis] <= skip_hi) continue;
+          bool dup = false;
+          for (int t = 0; t < 4 && !dup; ++t) {
+            const double *V = p + 3 * (size_t)v[t];
+            dup = V[0] == Q[0] && V[1] == Q[1] && V[2] == Q[2];
+          }
+          if (!dup && insphere_perturbed(a, b, cc, d, Q) > 0) return 0;
+        }
+      }
+  return 1;
 }
 
 }  // namespace smdt
 
 // Returns the number of finite tetrahedra written to `tet` ((m, 4) int32,
-// positively oriented for orient3d of the predicates, canonical order), or
-// an error code.
+// positively oriented for orient3d of the predicates, canonical, in
+// increasing order), or an error code.
 int64_t sm_host_delaunay3d(const double *p, int64_t n, int32_t *tet, int64_t cap,
                            uint8_t *inserted) {
-  int64_t ncert = 0;
-  return sm_host_delaunay3d_slabs(p, n, 0, 1, 0.0, 1, tet, cap, inserted, &ncert);
+  return sm_host_delaunay3d_slabs(p, n, 0, 1, 1, tet, cap, inserted);
 }
 
-int64_t sm_host_delaunay3d_slabs(const double *p, int64_t n, int axis, int nslabs, double halo,
-                                 int nthreads, int32_t *tet, int64_t cap, uint8_t *inserted,
-                                 int64_t *ncertified) {
-  if (n < 0 || (n > 0 && (!p || !tet || !inserted)) || !ncertified)
+int sm_host_delaunay3d_check(const double *p, int64_t n, const int32_t *tet, int64_t count,
+                             int nthreads) {
+  if (n < 0 || count < 0 || (n > 0 && !p) || (count > 0 && !tet) || nthreads < 1)
+    return sm_set_error(SM_ERR_ARG, "host_delaunay3d_check: bad argument");
+  for (int64_t i = 0; i < 4 * count; ++i)
+    if (tet[i] < 0 || tet[i] >= n)
+      return sm_set_error(SM_ERR_ARG, "host_delaunay3d_check: vertex id out of range");
+  smdt::DT3 dt;
+  return dt.verify(p, n, tet, count, nthreads, (int64_t)1 << 40);
+}
+
+int64_t sm_host_delaunay3d_slabs(const double *p, int64_t n, int axis, int nslabs, int nthreads,
+                                 int32_t *tet, int64_t cap, uint8_t *inserted) {
+  if (n < 0 || (n > 0 && (!p || !tet || !inserted)))
     return sm_set_error(SM_ERR_ARG, "host_delaunay3d: null argument");
-  smdt::Slabs3 sl;
+  smdt::DT3 dt;
   int64_t m = 0;
   bool over = false;
-  int rc = sl.run(p, n, axis, nslabs, halo, nthreads,
-                  [&](int slab, const int32_t *v, int64_t count) {
-                    if (slab < 0) *ncertified = m;
-                    if (m + count > cap) {
-                      over = true;
-                      return;
-                    }
-                    if (count) std::memcpy(tet + 4 * m, v, (size_t)count * 4 * sizeof(int32_t));
-                    m += count;
-                  });
+  int rc = dt.run(p, n, axis, nslabs, nthreads, [&](int, const int32_t *v, int64_t count) {
+    if (m + count > cap) {
+      over = true;
+      return;
+    }
+    if (count) std::memcpy(tet + 4 * m, v, (size_t)count * 4 * sizeof(int32_t));
+    m += count;
+  });
   if (rc) return rc;
   if (over) return sm_set_error(SM_ERR_CAPACITY, "host_delaunay3d: output capacity %lld",
                                 (long long)cap);
-  for (int64_t i = 0; i < n; ++i) inserted[i] = sl.inserted[i];
+  for (int64_t i = 0; i < n; ++i) inserted[i] = dt.inserted[i];
   return m;
 }

@@ -481,15 +481,15 @@ __device__ inline void record_error(uint64_t *ctr, double dn, double d) {
   if (err == err) atomicMax((unsigned long long *)ctr, (unsigned long long)__double_as_longlong(err));
 }
 
-// Centroid filter with the surrogate: outside the band |d_nn| < band the
-// network's sign decides (a centroid with d_nn <= -band is kept, one with
-// d_nn >= band dropped); inside the band, or where d_nn is not finite, the
-// exact program decides as in k_keep.  The decisions agree with k_keep
-// wherever |d_nn - d| < band - geps.
+// Centroid filter with the surrogate: outside the band |d_nn| < band(x)
+// the network's sign decides (a centroid with d_nn <= -band is kept, one
+// with d_nn >= band dropped); inside the band, or where d_nn is not finite,
+// the exact program decides as in k_keep.  The decisions agree with k_keep
+// wherever |d_nn - d| < band(x) - geps.
 template <int DIM, int MODE>
 __global__ void k_keep_surrogate(int64_t T, const int32_t *tri, const double *p,
                                  const sm_node *tape, int ntape, double geps, const float *dnn,
-                                 double band, int32_t N, uint8_t *keep, uint64_t *sg) {
+                                 BandView bv, int32_t N, uint8_t *keep, uint64_t *sg) {
   const int64_t t = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (t >= T) return;
   const int32_t *c = tri + (DIM + 1) * t;
@@ -497,15 +497,10 @@ __global__ void k_keep_surrogate(int64_t T, const int32_t *tri, const double *p,
 #pragma unroll
   for (int k = 0; k <= DIM; ++k) real &= (uint32_t)c[k] < (uint32_t)N;
   const double dn = real ? (double)dnn[t] : 0.0;
-  const bool exact = real && !(fabs(dn) >= band);
-  wave_count(sg + 0, real && !exact);
-  wave_count(sg + 1, exact);
-  if (!real) {
-    keep[t] = 0;
-  } else if (!exact) {
-    keep[t] = dn < 0.0;
-  } else {
-    double x[3];
+  double x[3] = {0.0, 0.0, 0.0};
+  // the centroid is needed for a per-region band, or for the exact test
+  const bool need_x = real && (bv.v != nullptr || !(fabs(dn) >= bv.band));
+  if (need_x) {
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
       double s = p[(int64_t)c[0] * DIM + d];
@@ -513,22 +508,38 @@ __global__ void k_keep_surrogate(int64_t T, const int32_t *tri, const double *p,
       for (int k = 1; k <= DIM; ++k) s = s + p[(int64_t)c[k] * DIM + d];
       x[d] = s / (double)(DIM + 1);
     }
+  }
+  const bool exact = real && !(fabs(dn) >= (bv.v ? band_at<DIM>(bv, x) : bv.band));
+  wave_count(sg + 0, real && !exact);
+  wave_count(sg + 1, exact);
+  if (!real) {
+    keep[t] = 0;
+  } else if (!exact) {
+    keep[t] = dn < 0.0;
+  } else {
     const double d = sdf_eval<DIM, MODE>(tape, ntape, x);
     record_error(sg + 2, dn, d);
     keep[t] = d < -geps;
   }
 }
 
-// Projection with the surrogate: points with d_nn <= -band are inside
+// Projection with the surrogate: points with d_nn <= -band(x) are inside
 // (no projection); every other point is tested and projected with the
 // exact program, as in k_vertex_update.
 template <int DIM, int MODE>
-__global__ void k_project_surrogate(int64_t N, double *p, const float *dnn, double band,
+__global__ void k_project_surrogate(int64_t N, double *p, const float *dnn, BandView bv,
                                     const sm_node *tape, int ntape, double deps, double hmin,
                                     uint64_t *sg) {
   const int64_t v = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (v >= N) return;
   const double dn = (double)dnn[v];
+  double band = bv.band;
+  if (bv.v) {
+    double y[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) y[d] = p[v * DIM + d];
+    band = band_at<DIM>(bv, y);
+  }
   const bool exact = !(dn <= -band);
   wave_count(sg + 0, !exact);
   wave_count(sg + 1, exact);
@@ -557,7 +568,7 @@ static void launch_keep_surrogate(sm_mesher *m, int64_t off, int64_t T) {
   const int B = m->blk[kBlkKeep];
   hipLaunchKernelGGL((k_keep_surrogate<DIM, MODE>), dim3(blocks_for(T, B)), dim3(B), 0,
                      m->stream, T, m->d_tri + (DIM + 1) * off, m->d_p, m->d_tapes[0],
-                     m->tape_len[0], m->prm.geps, m->d_dnn + off, m->band, (int32_t)m->N,
+                     m->tape_len[0], m->prm.geps, m->d_dnn + off, band_view(m), (int32_t)m->N,
                      m->d_keep + off, m->d_sg);
 }
 
@@ -567,8 +578,8 @@ static int project_surrogate(sm_mesher *m) {
   if (rc) return rc;
   const int B = 256;
   hipLaunchKernelGGL((k_project_surrogate<DIM, MODE>), dim3(blocks_for(m->N, B)), dim3(B), 0,
-                     m->stream, m->N, m->d_p, m->d_dnn, m->band, m->d_tapes[0], m->tape_len[0],
-                     m->prm.deps, m->prm.h0, m->d_sg);
+                     m->stream, m->N, m->d_p, m->d_dnn, band_view(m), m->d_tapes[0],
+                     m->tape_len[0], m->prm.deps, m->prm.h0, m->d_sg);
   SM_LAUNCH_CHECK();
   return SM_OK;
 }

@@ -2,6 +2,7 @@
 // independent 2D domains).
 #pragma once
 
+#include <algorithm>
 #include <vector>
 
 #include "common.h"
@@ -9,6 +10,8 @@
 #include "sizing_eval.h"
 
 constexpr int kNumStats = 13;
+
+constexpr double kFarSquare = 100.0;
 
 struct sm_mesher {
   sm_mesher_params prm{};
@@ -63,12 +66,21 @@ struct sm_mesher {
   float *d_values = nullptr;
 
   // points: N moving points, then (2D batches) nfar fixed far vertices --
-  // four per domain, at the corners of a square 200 times the domain's
-  // extent -- so each domain's triangulated region is a fixed square
-  // instead of the convex hull of its points (whose straight, nearly
-  // collinear boundaries would otherwise need long sequential chains of
-  // hull flips; only triangles with circumcircles wider than ~100 extents,
-  // which the centroid filter never keeps, differ); NA = N + nfar
+  // four per domain, at the corners of a square 2 kFarSquare times the
+  // domain's extent -- so each domain's triangulated region is a fixed
+  // square instead of the convex hull of its points (whose straight,
+  // nearly collinear boundaries would otherwise need long sequential chains
+  // of hull flips).  Only triangles with circumcircles wider than
+  // ~kFarSquare extents differ from the hull's triangulation: nearly
+  // collinear triples on the hull.  Where the initial lattice's outer row
+  // lies inside a curved boundary, such flat triangles can pass the
+  // centroid filter in the first iteration (circles of 100 - 5000 extents
+  // for a row moved by 1e-3 of a spacing), and the reference keeps them; by
+  // the next iteration the boundary points are on the boundary and those
+  // triangles fail the filter.  A square of 1e6 extents leaves out only
+  // triples straight to 1e-6 of a spacing but makes predicates near the far
+  // vertices uncertain (the batch's exact pass then runs every round), so
+  // 100 it is; NA = N + nfar
   int64_t N = 0, capN = 0, NA = 0;
   int nfar = 0;
   double *d_p = nullptr;
@@ -130,19 +142,23 @@ struct sm_mesher {
   bool surrogate = false;
   sm_mlp net{};
   double band = 0.0;
+  // optional per-region bands: a res^dim grid over [band_lo, band_hi]
+  // (sm_mesher_set_surrogate_bands); the band at x is max(band, the value
+  // of x's cell), and the largest value outside the grid
+  float *d_bands = nullptr;
+  int band_res = 0;
+  double band_lo[3] = {0, 0, 0}, band_inv[3] = {0, 0, 0};
+  double band_max = 0.0;
   float *d_dnn = nullptr;
   int64_t dnn_cap = 0;
   uint64_t *d_sg = nullptr;
 
-  // 3D: the host tetrahedralisation, serial or in slabs on worker threads
-  // (delaunay3d.cpp); slab k's tetrahedra are copied to the device and
-  // filtered there while later slabs are still being built.  The cells from
-  // ncert on could not be certified: if the filter keeps one, the
-  // iteration's triangulation is rebuilt serially and the halo doubled.
+  // 3D: the host tetrahedralisation, built by sl_threads threads inserting
+  // the points of sl_n slabs along sl_axis into one shared
+  // tetrahedralisation (delaunay3d.cpp); its cells are copied to the device
+  // and filtered there piece by piece while the host hands out the next.
   int sl_axis = 1, sl_n = 1, sl_threads = 1;
-  double sl_halo = 0.0;
-  smdt::Slabs3 dt3;
-  int64_t ncert = -1;
+  smdt::DT3 dt3;
   double *h_pts = nullptr;  // pinned staging of the points (N, 3)
   int32_t *h_tet = nullptr;  // pinned staging of the tetrahedra (capT, 4)
   int64_t h_pts_cap = 0, h_tet_cap = 0;
@@ -182,6 +198,40 @@ struct BatchView {
   const int *tape_off = nullptr, *tape_len = nullptr;
   double geps_mult = 0.0, deps_mult = 0.0;
 };
+
+// The surrogate's exact-fallback band at a point (see sm_mesher.band*).
+struct BandView {
+  const float *v = nullptr;
+  int res = 0;
+  double lo[3] = {0, 0, 0}, inv[3] = {0, 0, 0};
+  double band = 0.0, vmax = 0.0;
+};
+
+inline BandView band_view(const sm_mesher *m) {
+  BandView b;
+  b.v = m->d_bands;
+  b.res = m->band_res;
+  for (int k = 0; k < 3; ++k) {
+    b.lo[k] = m->band_lo[k];
+    b.inv[k] = m->band_inv[k];
+  }
+  b.band = m->band;
+  b.vmax = std::max(m->band, m->band_max);
+  return b;
+}
+
+template <int DIM>
+__device__ inline double band_at(const BandView &b, const double *x) {
+  if (!b.v) return b.band;
+  int64_t idx = 0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    const double c = (x[d] - b.lo[d]) * b.inv[d];
+    if (!(c >= 0.0 && c < (double)b.res)) return b.vmax;  // outside the grid, or NaN
+    idx = idx * b.res + (int64_t)c;
+  }
+  return fmax(b.band, (double)b.v[idx]);
+}
 
 inline BatchView batch_view(const sm_mesher *m) {
   BatchView b;

@@ -174,12 +174,13 @@ int host_sizing_pass(sm_mesher *m) {
 }
 
 // 3D: the tetrahedralisation is rebuilt on the host from the current points
-// every iteration (as the reference does with CGAL).  Each slab's
-// tetrahedra go to the device (pinned staging, asynchronous copy) and
-// through the centroid filter as soon as that slab is built, while the
-// worker threads build the next ones; *filtered tells whether the filter
-// ran that way (device programs) or is still to run (host callbacks).
-int host_rebuild3d(sm_mesher *m, bool serial, bool *filtered) {
+// every iteration (as the reference does with CGAL), on sl_threads threads
+// inserting the points of sl_n slabs into one shared tetrahedralisation.
+// Its cells are handed out in pieces; each piece goes to the device (pinned
+// staging, asynchronous copy) and through the centroid filter while the
+// host prepares the next; *filtered tells whether the filter ran that way
+// (device programs) or is still to run (host callbacks).
+int host_rebuild3d(sm_mesher *m, bool *filtered) {
   const auto t0 = std::chrono::steady_clock::now();
   const int64_t N = m->N;
   if (m->h_pts_cap < N) {
@@ -201,45 +202,39 @@ int host_rebuild3d(sm_mesher *m, bool serial, bool *filtered) {
   SM_HIP(hipMemcpyAsync(m->h_pts, m->d_p, N * 3 * sizeof(double), hipMemcpyDeviceToHost,
                         m->stream));
   SM_HIP(hipStreamSynchronize(m->stream));
-  const int S = serial ? 1 : m->sl_n;
-  int64_t off = 0, ncert = 0;
+  int64_t off = 0;
   int err = SM_OK;
   bool over = false, all_filtered = true;
-  int rc = m->dt3.run(
-      m->h_pts, N, m->sl_axis, S, m->sl_halo, serial ? 1 : m->sl_threads,
-      [&](int slab, const int32_t *v, int64_t count) {
-        if (slab < 0) ncert = off;
-        if (err || over || count == 0) return;
-        if (off + count > m->capT) {
-          over = true;
-          return;
-        }
-        std::memcpy(m->h_tet + 4 * off, v, (size_t)count * 4 * sizeof(int32_t));
-        if (hipMemcpyAsync(m->d_tri + 4 * off, m->h_tet + 4 * off,
-                           (size_t)count * 4 * sizeof(int32_t), hipMemcpyHostToDevice,
-                           m->stream) != hipSuccess) {
-          err = sm_set_error(SM_ERR_HIP, "host_rebuild3d: copy of slab %d failed", slab);
-          return;
-        }
-        // the points do not move before the filter, so it may run now
-        bool done = false;
-        int b = prof_begin(m);
-        err = centroid_keep_range(m, off, count, &done);
-        prof_end(m, 1, b);
-        all_filtered &= done;
-        off += count;
-      });
+  auto emit = [&](int piece, const int32_t *v, int64_t count) {
+    if (err || over || count == 0) return;
+    if (off + count > m->capT) {
+      over = true;
+      return;
+    }
+    std::memcpy(m->h_tet + 4 * off, v, (size_t)count * 4 * sizeof(int32_t));
+    if (hipMemcpyAsync(m->d_tri + 4 * off, m->h_tet + 4 * off,
+                       (size_t)count * 4 * sizeof(int32_t), hipMemcpyHostToDevice,
+                       m->stream) != hipSuccess) {
+      err = sm_set_error(SM_ERR_HIP, "host_rebuild3d: copy of piece %d failed", piece);
+      return;
+    }
+    // the points do not move before the filter, so it may run now
+    bool done = false;
+    int b = prof_begin(m);
+    err = centroid_keep_range(m, off, count, &done);
+    prof_end(m, 1, b);
+    all_filtered &= done;
+    off += count;
+  };
+  int rc = m->dt3.run(m->h_pts, N, m->sl_axis, m->sl_n, m->sl_threads, emit);
   if (rc) return rc;
   if (err) return err;
   SM_CHECK(!over, SM_ERR_CAPACITY, "host 3D Delaunay: more than %lld tetrahedra",
            (long long)m->capT);
   m->T = off;
-  m->ncert = ncert;
   m->stats[3] += 1;
-  if (S > 1) {
-    m->stats[9] += 1;
-    m->stats[11] += off - ncert;
-  }
+  m->stats[9] += m->dt3.stats.parallel;
+  m->stats[10] += m->dt3.stats.retried;
   m->have_tri = true;
   *filtered = all_filtered;
   m->stats[12] += (int64_t)std::chrono::duration_cast<std::chrono::microseconds>(
@@ -252,34 +247,13 @@ int triangulate(sm_mesher *m) {
   m->nsl = -1;  // a sliver selection refers to the previous cells
   if (m->dim == 3) {
     bool filtered = false;
-    int rc = host_rebuild3d(m, false, &filtered);
+    int rc = host_rebuild3d(m, &filtered);
     if (rc) return rc;
     if (!filtered) {
       int b = prof_begin(m);
       rc = centroid_keep(m);
       prof_end(m, 1, b);
       if (rc) return rc;
-    }
-    if (m->ncert < m->T) {
-      // an uncertified tetrahedron that the filter keeps: rebuild serially
-      const int64_t nu = m->T - m->ncert;
-      std::vector<uint8_t> kept((size_t)nu);
-      SM_HIP(hipMemcpyAsync(kept.data(), m->d_keep + m->ncert, (size_t)nu,
-                            hipMemcpyDeviceToHost, m->stream));
-      SM_HIP(hipStreamSynchronize(m->stream));
-      bool any = false;
-      for (uint8_t k : kept) any |= k != 0;
-      if (any) {
-        m->stats[10] += 1;
-        if (m->sl_halo > 0) m->sl_halo *= 2.0;
-        if ((rc = host_rebuild3d(m, true, &filtered))) return rc;
-        if (!filtered) {
-          int b = prof_begin(m);
-          rc = centroid_keep(m);
-          prof_end(m, 1, b);
-          if (rc) return rc;
-        }
-      }
     }
     return SM_OK;
   }
@@ -532,6 +506,7 @@ extern "C" int sm_mesher_destroy(sm_mesher *m) {
   if (m->h_status) (void)hipHostFree(m->h_status);
   if (m->h_sums) (void)hipHostFree(m->h_sums);
   if (m->h_maxF) (void)hipHostFree(m->h_maxF);
+  dfree(m->d_bands);
   if (m->h_pts) (void)hipHostFree(m->h_pts);
   if (m->h_tet) (void)hipHostFree(m->h_tet);
   for (hipEvent_t e : m->ev_pool) (void)hipEventDestroy(e);
@@ -571,7 +546,7 @@ extern "C" int sm_mesher_set_points(sm_mesher *m, const double *p, int64_t n) {
         }
       const double ext = std::max(std::max(hi[0] - lo[0], hi[1] - lo[1]), 1e-300);
       const double c[2] = {(lo[0] + hi[0]) / 2, (lo[1] + hi[1]) / 2};
-      const double K = 100.0 * ext;
+      const double K = kFarSquare * ext;
       const double sx[4] = {-1, 1, 1, -1}, sy[4] = {-1, -1, 1, 1};
       for (int k = 0; k < 4; ++k) {
         far[2 * (4 * d + k)] = c[0] + K * sx[k];
@@ -831,19 +806,47 @@ extern "C" int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats) {
   return SM_OK;
 }
 
-extern "C" int sm_mesher_set_slabs(sm_mesher *m, int axis, int nslabs, double halo,
-                                   int nthreads) {
+extern "C" int sm_mesher_set_slabs(sm_mesher *m, int axis, int nslabs, int nthreads) {
   SM_CHECK(m, SM_ERR_ARG, "mesher_set_slabs: null handle");
   SM_CHECK(m->dim == 3, SM_ERR_ARG, "mesher_set_slabs: slabs split the 3D tetrahedralisation");
   SM_CHECK(axis >= 0 && axis < 3, SM_ERR_ARG, "mesher_set_slabs: axis must be 0, 1 or 2");
   SM_CHECK(nslabs >= 1 && nslabs <= 4096, SM_ERR_ARG, "mesher_set_slabs: bad slab count %d",
            nslabs);
-  SM_CHECK(halo >= 0.0 && nthreads >= 1 && nthreads <= 1024, SM_ERR_ARG,
-           "mesher_set_slabs: bad halo or thread count");
+  SM_CHECK(nthreads >= 1 && nthreads <= 1024, SM_ERR_ARG, "mesher_set_slabs: bad thread count");
   m->sl_axis = axis;
   m->sl_n = nslabs;
-  m->sl_halo = halo;
   m->sl_threads = nthreads;
+  return SM_OK;
+}
+
+extern "C" int sm_mesher_set_surrogate_bands(sm_mesher *m, int res, const double *lo,
+                                             const double *hi, const float *bands) {
+  SM_CHECK(m, SM_ERR_ARG, "mesher_set_surrogate_bands: null handle");
+  SM_HIP(hipSetDevice(m->device));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  dfree(m->d_bands);
+  m->band_res = 0;
+  m->band_max = 0.0;
+  if (!bands) return SM_OK;
+  SM_CHECK(lo && hi && res >= 1 && res <= 512, SM_ERR_ARG,
+           "mesher_set_surrogate_bands: bad grid (res %d)", res);
+  int64_t cells = 1;
+  for (int k = 0; k < m->dim; ++k) {
+    SM_CHECK(hi[k] > lo[k], SM_ERR_ARG, "mesher_set_surrogate_bands: empty box on axis %d", k);
+    m->band_lo[k] = lo[k];
+    m->band_inv[k] = res / (hi[k] - lo[k]);
+    cells *= res;
+  }
+  double vmax = 0.0;
+  for (int64_t i = 0; i < cells; ++i) {
+    SM_CHECK(bands[i] >= 0.0f, SM_ERR_ARG, "mesher_set_surrogate_bands: negative band");
+    vmax = std::max(vmax, (double)bands[i]);
+  }
+  int rc = dalloc(&m->d_bands, cells);
+  if (rc) return rc;
+  SM_HIP(hipMemcpy(m->d_bands, bands, cells * sizeof(float), hipMemcpyHostToDevice));
+  m->band_res = res;
+  m->band_max = vmax;
   return SM_OK;
 }
 

@@ -1,12 +1,13 @@
 // Neural signed-distance surrogate: a multilayer perceptron (coordinates
 // -> distance) with SiLU activations.
 //
-// Inference (k_mlp_eval) is one fused kernel per batch of points: the input
-// layer in FP32 on the vector ALUs, each width x width hidden layer as BF16
-// matrix-core products (v_mfma_f32_32x32x16_bf16, FP32 accumulation) on
+// Inference (k_mlp_eval_lds) is one fused kernel per batch of points: the
+// input layer in FP32 on the vector ALUs, each width x width hidden layer as
+// BF16 matrix-core products (v_mfma_f32_32x32x16_bf16, FP32 accumulation) on
 // activations held in LDS, bias + SiLU in the epilogue of every layer, the
 // output layer as a dot product.  Each wave owns 32 points (the M of one
-// 32x32x16 product); a hidden layer is 8 column blocks x 16 k-steps.
+// 32x32x16 product); a hidden layer is 8 column blocks x 16 k-steps, whose
+// weight fragments the workgroup's 8 waves share through LDS.
 //
 // Training uses FP32 kernels: a tiled GEMM with strided operands and an
 // optional deterministic split over k (forward X W^T, backward dZ W,
@@ -27,7 +28,6 @@ namespace {
 
 constexpr int kW = 256;       // hidden width of the inference kernel
 constexpr int kRow = kW + 8;  // LDS row stride in bf16 (+16 B: row r starts 4 banks after r-1)
-constexpr int kWaves = 4;
 constexpr int kMaxHidden = 16;
 
 __device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
@@ -65,21 +65,29 @@ __device__ inline float silu_fast(float z) {
 
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
-// PT point tiles of 32 per wave (WAVES waves per workgroup): every weight
-// fragment loaded into registers feeds PT matrix products, so PT = 2 halves
-// the weight stream per FLOP at the price of one wave per SIMD (256
-// accumulator registers, 33.8 KB of LDS per wave); the launch uses PT = 1.
+// The same network with the hidden layers' weights shared through LDS: a
+// workgroup of WAVES waves stages each k-step's eight weight fragments
+// (8 KiB) in LDS once -- double-buffered, the next k-step's fragments
+// loaded from global memory while this one's products run -- instead of
+// every wave streaming the whole layer from L2, and each wave multiplies
+// every fragment it reads into PT point tiles of 32.  The launch uses 8
+// waves of one tile each (132 KiB of activations + 16 KiB of fragments per
+// workgroup, two waves per SIMD to hide the fragment loads).  Measured on
+// MI355X at 4M points: 610 TFLOP/s, against 511 with every wave streaming
+// the weights from L2 and 453 with 4 waves of two tiles (one wave per SIMD).
+constexpr int kEvalWaves = 8;
+constexpr int kEvalPT = 1;
+
 template <int IN, bool CELLS, int PT, int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_mlp_eval(sm_mlp net, const double *x,
-                                                         const int32_t *cells, int64_t n,
-                                                         float *out) {
+__global__ __launch_bounds__(64 * WAVES) void k_mlp_eval_lds(sm_mlp net, const double *x,
+                                                             const int32_t *cells, int64_t n,
+                                                             float *out) {
   __shared__ __align__(16) __bf16 hs[WAVES][32 * PT][kRow];
+  __shared__ __align__(16) bf16x8 wsh[2][8 * 64];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int r = lane % 32, half = lane / 32;
   const int64_t p0 = ((int64_t)blockIdx.x * WAVES + wave) * (32 * PT);
   bool valid[PT];
-  // input layer: this lane computes features [128 half, 128 half + 128) of
-  // its point r of every tile, stored 8 at a time
 #pragma unroll
   for (int pt = 0; pt < PT; ++pt) {
     const int64_t pi = p0 + 32 * pt + r;
@@ -106,20 +114,24 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_eval(sm_mlp net, const doubl
       *reinterpret_cast<bf16x8 *>(&hs[wave][32 * pt + r][half * (kW / 2) + j0]) = v;
     }
   }
-  __syncthreads();
-  // Hidden layers as C^T = W A^T: the A operand is a weight fragment (32
-  // features x 16 k: lane holds feature 32 nb + r, k = 8 half .. 8 half + 7,
-  // packed by k_pack_hidden), the B operand the activations (16 k x 32
-  // points: lane holds point r of a tile, same k), and the accumulator
-  // gives lane r the features 32 nb + 8 (i / 4) + 4 half + i % 4
-  // (i = 0..15) of point r -- four consecutive features per group, one
-  // 8-byte LDS store each.
   const bf16x8 *whid = reinterpret_cast<const bf16x8 *>(net.w_hid);
+  constexpr int kStep = 8 * 64;                       // fragments of one k-step
+  constexpr int kPer = kStep / (64 * WAVES);          // per thread and k-step
+  static_assert(kStep % (64 * WAVES) == 0, "k-step fragments per thread");
   float s[PT];
 #pragma unroll
   for (int pt = 0; pt < PT; ++pt) s[pt] = 0.0f;
   for (int l = 0; l < net.hidden; ++l) {
-    const bf16x8 *W = whid + (size_t)l * (kW * kW / 8) + lane;
+    const bf16x8 *W = whid + (size_t)l * (kW * kW / 8);
+    // k-step 0 of this layer into buffer 0, k-step 1 into registers
+    bf16x8 g[kPer];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) g[j] = W[threadIdx.x + j * 64 * WAVES];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) wsh[0][threadIdx.x + j * 64 * WAVES] = g[j];
+#pragma unroll
+    for (int j = 0; j < kPer; ++j) g[j] = W[kStep + threadIdx.x + j * 64 * WAVES];
+    __syncthreads();  // buffer 0 and the layer's input activations are in place
     floatx16 acc[PT][8];
 #pragma unroll
     for (int pt = 0; pt < PT; ++pt)
@@ -127,61 +139,68 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_eval(sm_mlp net, const doubl
       for (int nb = 0; nb < 8; ++nb)
 #pragma unroll
         for (int i = 0; i < 16; ++i) acc[pt][nb][i] = 0.0f;
-    // the next k-step's weight fragments are loaded before this step's products
-    bf16x8 wc[8], wn[8];
-#pragma unroll
-    for (int nb = 0; nb < 8; ++nb) wc[nb] = W[nb * 64];
     for (int ks = 0; ks < kW / 16; ++ks) {
+      const bf16x8 *buf = wsh[ks & 1];
+      bf16x8 a[PT];
+#pragma unroll
+      for (int pt = 0; pt < PT; ++pt)
+        a[pt] = *reinterpret_cast<const bf16x8 *>(&hs[wave][32 * pt + r][16 * ks + 8 * half]);
+#pragma unroll
+      for (int nb = 0; nb < 8; ++nb) {
+        const bf16x8 w = buf[nb * 64 + lane];
+#pragma unroll
+        for (int pt = 0; pt < PT; ++pt)
+          acc[pt][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, a[pt], acc[pt][nb], 0, 0, 0);
+      }
       if (ks + 1 < kW / 16) {
+        // the other buffer was last read in k-step ks - 1, before the
+        // barrier that ended it
 #pragma unroll
-        for (int nb = 0; nb < 8; ++nb) wn[nb] = W[((ks + 1) * 8 + nb) * 64];
+        for (int j = 0; j < kPer; ++j) wsh[(ks + 1) & 1][threadIdx.x + j * 64 * WAVES] = g[j];
+        if (ks + 2 < kW / 16) {
+#pragma unroll
+          for (int j = 0; j < kPer; ++j)
+            g[j] = W[(ks + 2) * kStep + threadIdx.x + j * 64 * WAVES];
+        }
       }
-#pragma unroll
-      for (int pt = 0; pt < PT; ++pt) {
-        const bf16x8 a =
-            *reinterpret_cast<const bf16x8 *>(&hs[wave][32 * pt + r][16 * ks + 8 * half]);
-#pragma unroll
-        for (int nb = 0; nb < 8; ++nb)
-          acc[pt][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wc[nb], a, acc[pt][nb], 0, 0, 0);
-      }
-#pragma unroll
-      for (int nb = 0; nb < 8; ++nb) wc[nb] = wn[nb];
+      __syncthreads();
     }
     const float *bias = net.b_hid + (size_t)l * kW;
     const bool last = l + 1 == net.hidden;
-    if (!last) __syncthreads();  // every lane has read the layer's input
 #pragma unroll
     for (int nb = 0; nb < 8; ++nb)
 #pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const int f0 = 32 * nb + 8 * g + 4 * half;
+      for (int g4 = 0; g4 < 4; ++g4) {
+        const int f0 = 32 * nb + 8 * g4 + 4 * half;
         const float4 b4 = *reinterpret_cast<const float4 *>(bias + f0);
         const float bb[4] = {b4.x, b4.y, b4.z, b4.w};
 #pragma unroll
         for (int pt = 0; pt < PT; ++pt) {
           bf16x4 v;
 #pragma unroll
-          for (int e = 0; e < 4; ++e) v[e] = (__bf16)silu_fast(acc[pt][nb][4 * g + e] + bb[e]);
+          for (int e = 0; e < 4; ++e) v[e] = (__bf16)silu_fast(acc[pt][nb][4 * g4 + e] + bb[e]);
           if (last) {
-            // output layer straight from the registers (FP32 weights over
-            // the BF16-rounded activations)
 #pragma unroll
             for (int e = 0; e < 4; ++e) s[pt] = fmaf((float)v[e], net.w_out[f0 + e], s[pt]);
           } else {
+            // every wave has finished reading this layer's input (the
+            // barrier of the last k-step), so the output may overwrite it
             *reinterpret_cast<bf16x4 *>(&hs[wave][32 * pt + r][f0]) = v;
           }
         }
       }
-    if (!last) __syncthreads();
   }
+  if (net.hidden == 0) {
+    __syncthreads();
 #pragma unroll
-  for (int pt = 0; pt < PT; ++pt) {
-    if (net.hidden == 0) {
+    for (int pt = 0; pt < PT; ++pt)
       for (int j = 0; j < kW / 2; ++j) {
         const int f = half * (kW / 2) + j;
         s[pt] = fmaf((float)hs[wave][32 * pt + r][f], net.w_out[f], s[pt]);
       }
-    }
+  }
+#pragma unroll
+  for (int pt = 0; pt < PT; ++pt) {
     float t = s[pt] + __shfl_xor(s[pt], 32);
     const int64_t pi = p0 + 32 * pt + r;
     if (half == 0 && valid[pt]) out[pi] = (float)(net.out_scale * (double)(t + net.b_out));
@@ -389,16 +408,14 @@ template <bool CELLS>
 int launch_eval(const sm_mlp *net, const double *x, const int32_t *cells, int64_t n, float *d,
                 hipStream_t s) {
   if (n == 0) return SM_OK;
-  // one point tile per wave, 4 waves per workgroup (2 tiles per wave
-  // measured slower: 476 against 514 TFLOP/s at 4M points, one wave per
-  // SIMD cannot hide the weight loads)
-  const unsigned g = blocks_for(n, 32 * kWaves);
+  constexpr int P = kEvalPT, Wv = kEvalWaves;
+  const unsigned g = blocks_for(n, 32 * P * Wv);
   if (net->in_dim == 2)
-    hipLaunchKernelGGL((k_mlp_eval<2, CELLS, 1, kWaves>), dim3(g), dim3(64 * kWaves), 0, s,
-                       *net, x, cells, n, d);
+    hipLaunchKernelGGL((k_mlp_eval_lds<2, CELLS, P, Wv>), dim3(g), dim3(64 * Wv), 0, s, *net, x,
+                       cells, n, d);
   else
-    hipLaunchKernelGGL((k_mlp_eval<3, CELLS, 1, kWaves>), dim3(g), dim3(64 * kWaves), 0, s,
-                       *net, x, cells, n, d);
+    hipLaunchKernelGGL((k_mlp_eval_lds<3, CELLS, P, Wv>), dim3(g), dim3(64 * Wv), 0, s, *net, x,
+                       cells, n, d);
   SM_LAUNCH_CHECK();
   return SM_OK;
 }

@@ -272,20 +272,38 @@ class Mesher:
         _lib.check(_lib.lib.sm_mesher_profile_read(self.h, ms, n, 8), "sm_mesher_profile_read")
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.PHASES)}
 
-    def set_surrogate(self, net, band):
+    def set_surrogate(self, net, band, bands=None):
         """Let `net` (a neural.NeuralSDF of this handle's dimension) answer
         the centroid filter and the projection test where |d_nn| >= band;
-        net=None switches it off.  The handle keeps the weights the network
-        has now (train before, not after)."""
+        net=None switches it off.  `bands` = (grid, lo, hi) from
+        NeuralSDF.calibrate_bands gives each region its own band (at least
+        `band`).  The handle keeps the weights the network has now (train
+        before, not after)."""
         if net is None:
             self._net = None
             _lib.check(_lib.lib.sm_mesher_set_surrogate(self.h, None, 0.0),
                        "sm_mesher_set_surrogate")
+            _lib.check(_lib.lib.sm_mesher_set_surrogate_bands(self.h, 0, None, None, None),
+                       "sm_mesher_set_surrogate_bands")
             return
         st = net.struct()
         self._net = (net, st)
         _lib.check(_lib.lib.sm_mesher_set_surrogate(self.h, ctypes.byref(st), float(band)),
                    "sm_mesher_set_surrogate")
+        if bands is None:
+            _lib.check(_lib.lib.sm_mesher_set_surrogate_bands(self.h, 0, None, None, None),
+                       "sm_mesher_set_surrogate_bands")
+        else:
+            grid, lo, hi = bands
+            grid = np.ascontiguousarray(grid, dtype=np.float32)
+            lo = np.ascontiguousarray(lo, dtype=np.float64)
+            hi = np.ascontiguousarray(hi, dtype=np.float64)
+            if grid.ndim != self.dim or len(set(grid.shape)) != 1:
+                raise ValueError("bands must be a res^dim grid")
+            self._bands = (grid, lo, hi)
+            _lib.check(_lib.lib.sm_mesher_set_surrogate_bands(
+                self.h, int(grid.shape[0]), lo.ctypes.data, hi.ctypes.data, grid.ctypes.data),
+                "sm_mesher_set_surrogate_bands")
 
     def surrogate_stats(self):
         """(exact evaluations skipped, exact evaluations made, max |d_nn - d|
@@ -328,8 +346,8 @@ class Mesher:
         return ms.value
 
     STATS = ("iterations", "flip_rounds", "flips", "host_rebuilds", "exact_predicates",
-             "rollbacks", "kinetic_repairs", "unresolved", "relocations", "slab_builds",
-             "slab_fallbacks", "uncertified_cells", "host_tet_us")
+             "rollbacks", "kinetic_repairs", "unresolved", "relocations", "parallel_builds",
+             "retried_points", "reserved", "host_tet_us")
 
     def stats(self):
         k = len(self.STATS)
@@ -337,18 +355,17 @@ class Mesher:
         _lib.check(_lib.lib.sm_mesher_stats(self.h, s, k), "sm_mesher_stats")
         return {name: int(s[i]) for i, name in enumerate(self.STATS)}
 
-    def set_slabs(self, nslabs, halo, axis=1, threads=None):
-        """3D: build each iteration's tetrahedralisation as `nslabs`
-        equal-width slabs along coordinate `axis` (the reference's `axis`
-        option), each with the points within `halo` of it, on `threads` host
-        threads (default: one per slab, at most the CPU count).  The cells
-        are those of the serial build (uncertified ones that the filter
-        keeps trigger a serial rebuild and a doubled halo)."""
+    def set_slabs(self, nslabs, axis=1, threads=None):
+        """3D: build each iteration's tetrahedralisation on `threads` host
+        threads (default: one per slab, at most the CPU count) inserting the
+        points of `nslabs` equal-width slabs along coordinate `axis` (the
+        reference's `axis` option) into one shared tetrahedralisation: the
+        cells are those of the serial build."""
         import os
         if threads is None:
             threads = max(1, min(int(nslabs), os.cpu_count() or 1))
-        _lib.check(_lib.lib.sm_mesher_set_slabs(self.h, int(axis), int(nslabs), float(halo),
-                                                int(threads)), "sm_mesher_set_slabs")
+        _lib.check(_lib.lib.sm_mesher_set_slabs(self.h, int(axis), int(nslabs), int(threads)),
+                   "sm_mesher_set_slabs")
 
 
 def _placeholder(dim):
@@ -497,10 +514,9 @@ def default_slabs(npoints, dim, requested=None):
 
 def make_mesher(st, far_square=False, slabs=None):
     """A force-loop handle for a prepared problem (initial points not set).
-    3D handles build their tetrahedralisation in slabs along the `axis`
-    option (default_slabs; `slabs` or the `slabs` option overrides), with a
-    halo of 5 h0 -- the reference's subdomain extents (mesh_generator.py:
-    867-877)."""
+    3D handles build their tetrahedralisation on one thread per slab along
+    the `axis` option (default_slabs; `slabs` or the `slabs` option
+    overrides)."""
     m = Mesher(st["levels"], st["fh"], st["h0"], st["delta_t"], st["geps"], st["deps"],
                st["nfix"], precision=st["opts"]["precision"], dim=st["dim"],
                far_square=far_square)
@@ -508,7 +524,7 @@ def make_mesher(st, far_square=False, slabs=None):
         req = slabs if slabs is not None else st["opts"].get("slabs")
         ns = default_slabs(len(st["p"]), 3, req)
         if ns > 1:
-            m.set_slabs(ns, 5.0 * st["h0"], axis=int(st["opts"].get("axis", 1)))
+            m.set_slabs(ns, axis=int(st["opts"].get("axis", 1)))
     return m
 
 

@@ -368,6 +368,42 @@ class NeuralSDF:
                              "targets: train longer or truncate wider" % (out, self.trunc))
         return out, err
 
+    def calibrate_bands(self, domain, band, res=16, nval=1000000, near=None, safety=2.0,
+                        seed=54321):
+        """Per-region bands for Mesher.set_surrogate: the largest |d_nn - d|
+        over validation points (half uniform in the box, half within `near`
+        of the surface) in each cell of a res^dim grid over the box and its
+        neighbouring cells, times `safety`, at least `band` (e.g. 2 h); a
+        cell without validation points gets the largest error overall.
+        Returns (bands (res,)*dim float32, lo, hi, largest error)."""
+        if near is None:
+            near = band
+        x, d = self.sample(domain, self.bbox, nval, near, seed=seed, pad=0.0)
+        dn = self.eval(x).astype(np.float64)
+        if self.trunc:
+            dn = np.clip(dn, -self.trunc, self.trunc)
+            d = np.clip(d, -self.trunc, self.trunc)
+        err = np.abs(dn - d)
+        lo, hi = self.bbox[:, 0].copy(), self.bbox[:, 1].copy()
+        idx = np.clip(((x - lo) / (hi - lo) * res).astype(np.int64), 0, res - 1)
+        flat = np.ravel_multi_index(tuple(idx.T), (res,) * self.dim)
+        cell = np.full(res ** self.dim, -1.0)
+        np.maximum.at(cell, flat, err)
+        emax = float(err.max())
+        cell[cell < 0] = emax
+        cell = cell.reshape((res,) * self.dim)
+        # each cell takes its neighbours' errors too (points near a cell
+        # face are judged by either cell)
+        pad = np.pad(cell, 1, mode="edge")
+        out = np.zeros_like(cell)
+        for off in np.ndindex(*(3,) * self.dim):
+            sl = tuple(slice(o, o + res) for o in off)
+            out = np.maximum(out, pad[sl])
+        bands = np.maximum(safety * out, band).astype(np.float32)
+        if self.trunc and float(bands.max()) >= self.trunc:
+            bands = np.minimum(bands, np.float32(0.999 * self.trunc))
+        return bands, lo, hi, emax
+
     def max_error(self, domain, x):
         """max |d_nn - d| over points x, judged by the exact program (both
         clipped to +-trunc for a network trained on a truncated SDF)."""

@@ -85,7 +85,7 @@ def sliver_removal(points, domain, edge_length, comm=None, **kwargs):
                    precision=64, dim=3)
     ns = gen.default_slabs(len(p), 3, opts.get("slabs"))
     if ns > 1:
-        m.set_slabs(ns, 5.0 * h0, axis=int(opts.get("axis", 1)))
+        m.set_slabs(ns, axis=int(opts.get("axis", 1)))
     try:
         m.set_points(p)
         count = 0

@@ -142,23 +142,22 @@ def test_3d_sliver_cylinder():
     assert abs(mu.simp_vol(p, t).sum() - math.pi) < h
 
 
-@pytest.mark.parametrize("axis,nslabs,halo_h", [(1, 8, 5.0), (0, 5, 1.0), (2, 3, 0.5)])
-def test_slab_build_iterations_identical_to_serial(axis, nslabs, halo_h):
-    """The 3D loop with its tetrahedralisation built in slabs on worker
-    threads (each slab's cells copied and filtered while the next slabs are
-    built) moves the points exactly as the serial build does, from the
-    degenerate initial lattice on; a narrow halo only makes more cells
-    uncertified (and, when the filter keeps one, the iteration is rebuilt
-    serially)."""
+@pytest.mark.parametrize("axis,nslabs,threads", [(1, 16, 8), (0, 5, 5), (2, 3, 2)])
+def test_slab_build_iterations_identical_to_serial(axis, nslabs, threads):
+    """The 3D loop with its tetrahedralisation built by several threads
+    (slabs inserted into one shared triangulation, each piece of cells
+    copied and filtered while the next is handed out) moves the points
+    exactly as the serial build does, from the degenerate initial lattice
+    on."""
     from seismicmesh_amd import generation, geometry as geo
     dom = geo.Difference([geo.Union([geo.Ball([0.0, 0.0, 0.0], 0.8),
                                      geo.Cube((-0.2, 1.0, -0.3, 0.3, -0.3, 0.3))]),
                           geo.Ball([0.3, 0.3, 0.3], 0.3)])
-    h = 0.07
+    h = 0.05
     st = generation.prepare(dom, h, verbose=0, precision=32)
     a = generation.make_mesher(st, slabs=1)
     b = generation.make_mesher(st, slabs=1)
-    b.set_slabs(nslabs, halo_h * h, axis=axis, threads=4)
+    b.set_slabs(nslabs, axis=axis, threads=threads)
     a.set_points(st["p"])
     b.set_points(st["p"])
     for it in range(4):
@@ -167,11 +166,12 @@ def test_slab_build_iterations_identical_to_serial(axis, nslabs, halo_h):
         assert np.array_equal(a.points(), b.points()), "iteration %d" % it
     a.triangulate()
     b.triangulate()
-    assert {tuple(c) for c in a.cells().tolist()} == {tuple(c) for c in b.cells().tolist()}
+    assert np.array_equal(a.cells(), b.cells())
     s = b.stats()
-    print("slabs %d axis %d halo %.1f h: %d slab builds, %d serial redos, %d uncertified cells"
-          % (nslabs, axis, halo_h, s["slab_builds"], s["slab_fallbacks"],
-             s["uncertified_cells"]))
-    assert s["slab_builds"] >= 5
+    print("%d points, slabs %d axis %d threads %d: %d parallel builds, %d points retried, "
+          "host %.1f ms per build" % (len(st["p"]), nslabs, axis, threads, s["parallel_builds"],
+                                      s["retried_points"], s["host_tet_us"] / 1e3 /
+                                      s["host_rebuilds"]))
+    assert s["parallel_builds"] == s["host_rebuilds"] >= 5
     a.close()
     b.close()

@@ -120,25 +120,27 @@ def test_3d_matches_qhull_and_skips_duplicates():
 
 
 # ------------------------------------------------- 3D: uniqueness and slabs
-def _tets_slabs(p, nslabs, halo, axis=1, threads=4):
-    import ctypes
+def _tets_slabs(p, nslabs, axis=1, threads=4):
+    """The tetrahedralisation built by `threads` threads inserting the points
+    of `nslabs` slabs along `axis` into one shared triangulation."""
     from seismicmesh_amd import _lib
     p = np.ascontiguousarray(p, dtype=np.float64)
     n = len(p)
     cap = 12 * n + 64
     t = np.zeros((cap, 4), np.int32)
     ins = np.zeros(n, np.uint8)
-    nc = ctypes.c_int64(-1)
-    m = _lib.lib.sm_host_delaunay3d_slabs(p.ctypes.data, n, axis, nslabs, float(halo), threads,
-                                          t.ctypes.data, cap, ins.ctypes.data, ctypes.byref(nc))
+    m = _lib.lib.sm_host_delaunay3d_slabs(p.ctypes.data, n, axis, nslabs, threads, t.ctypes.data,
+                                          cap, ins.ctypes.data)
     assert m >= 0, _lib.last_error()
-    return t[:m], nc.value, ins
+    return t[:m], ins
 
 
 def _canon_ok(p, t):
-    # lowest id first, second-lowest second, positively oriented
+    # lowest id first, second-lowest second, positively oriented, sorted
     from oracle.predicates import orient3d
     assert (t[:, 0] < t[:, 1]).all() and (t[:, 1] < t[:, 2]).all() and (t[:, 1] < t[:, 3]).all()
+    key = [tuple(x) for x in t.tolist()]
+    assert key == sorted(key)
     for tt in t[:: max(1, len(t) // 300)]:
         assert orient3d(*p[tt]) > 0
 
@@ -164,14 +166,12 @@ def test_3d_lattice_unique_whatever_the_order():
     vol = np.einsum("ij,ij->i", np.cross(g[t0[:, 1]] - g[t0[:, 0]], g[t0[:, 2]] - g[t0[:, 0]]),
                     g[t0[:, 3]] - g[t0[:, 0]]) / 6
     assert vol.min() > 0 and abs(vol.sum() - 6 * 5 * 4) < 1e-9
-    want = {tuple(x) for x in t0.tolist()}
     rng = np.random.default_rng(3)
     for _ in range(3):
         perm = rng.permutation(len(g))
         t1, _ = _tets(g[perm])
         t1 = perm[t1]
         assert {tuple(sorted(x)) for x in t1.tolist()} == {tuple(sorted(x)) for x in t0.tolist()}
-    assert len(want) == len(t0)
 
 
 def test_3d_perturbed_empty_spheres_on_degenerate_points():
@@ -194,45 +194,39 @@ def test_3d_perturbed_empty_spheres_on_degenerate_points():
     assert abs(vol.sum() - ConvexHull(p).volume) < 1e-12
 
 
-@pytest.mark.parametrize("kind", ["lattice", "jittered", "random"])
-def test_3d_slabs_equal_serial_build(kind):
-    """Slab builds (threads, halos) report exactly the serial cells as
-    certified, except cells whose circumsphere is wider than the halo, which
-    come back uncertified; on the lattice nothing is left out."""
+@pytest.mark.parametrize("kind", ["lattice", "jittered", "random", "clustered"])
+def test_3d_threads_equal_serial_build(kind):
+    """Several threads inserting the points of slabs into one shared
+    tetrahedralisation return the serial build's cells, in the same order,
+    for any slab count, axis and thread count -- on cospherical lattices,
+    jittered lattices, random and strongly clustered points."""
     rng = np.random.default_rng(11)
     if kind == "random":
-        p = rng.random((6000, 3)) * [3.0, 2.0, 1.0]
-        h = (6.0 / 6000) ** (1 / 3)
+        p = rng.random((30000, 3)) * [3.0, 2.0, 1.0]
+    elif kind == "clustered":
+        c = rng.random((20, 3))
+        p = (c[rng.integers(0, 20, 30000)] + rng.normal(0, 0.02, (30000, 3)))
     else:
-        h = 0.1
+        h = 0.06
         g = np.mgrid[0:2.0 + h / 2:h, 0:1.5 + h / 2:h, 0:0.8 + h / 2:h].reshape(3, -1).T
         p = g if kind == "lattice" else g + rng.uniform(-0.2, 0.2, g.shape) * h
-    serial, _ = _tets(p)
-    S = {tuple(x) for x in serial.tolist()}
-    for axis, nslabs, halo in [(0, 4, 2 * h), (1, 7, 3 * h), (2, 3, 5 * h), (0, 1, 0.0)]:
-        t, nc, ins = _tets_slabs(p, nslabs, halo, axis=axis)
-        assert ins.all()
-        _canon_ok(p, t)
-        cert = [tuple(x) for x in t[:nc].tolist()]
-        assert len(set(cert)) == len(cert)
-        assert set(cert) <= S
-        missing = np.array(sorted(S - set(cert)), dtype=np.int64).reshape(-1, 4)
-        if nslabs == 1 or kind == "lattice":
-            assert len(missing) == 0 and nc == len(t)
-        if len(missing):
-            # only cells whose sphere does not fit a halo around their slab
-            assert (_circumradius(p, missing) > 0.5 * halo * (1 - 1e-9)).all()
+    serial, ins0 = _tets(p)
+    _canon_ok(p, serial)
+    for axis, nslabs, threads in [(0, 4, 4), (1, 7, 3), (2, 16, 8), (1, 1, 8)]:
+        t, ins = _tets_slabs(p, nslabs, axis=axis, threads=threads)
+        assert np.array_equal(ins, ins0)
+        assert np.array_equal(t, serial), (axis, nslabs, threads)
 
 
-def test_3d_slabs_duplicates_keep_lowest_id():
+def test_3d_threads_duplicates_keep_lowest_id():
     rng = np.random.default_rng(5)
-    p = rng.random((2000, 3))
-    q = np.vstack([p[1500:], p[:1500], p[:40]])  # duplicates of ids 500.. at the end
+    p = rng.random((20000, 3))
+    q = np.vstack([p[15000:], p[:15000], p[:400]])  # duplicates of ids 5000.. at the end
     t_s, ins_s = _tets(q)
-    t_p, nc, ins_p = _tets_slabs(q, 5, 0.3, axis=0)
+    t_p, ins_p = _tets_slabs(q, 6, axis=0, threads=6)
     assert np.array_equal(ins_s, ins_p)
-    assert ins_s[:2000].all() and not ins_s[2000:].any()
-    assert t_s.max() < 2000 and t_p.max() < 2000
+    assert ins_s[:20000].all() and not ins_s[20000:].any()
+    assert t_s.max() < 20000 and np.array_equal(t_s, t_p)
 
 
 def test_3d_convex_hull_cells_match_qhull_on_a_jittered_lattice():
@@ -254,3 +248,28 @@ def test_3d_convex_hull_cells_match_qhull_on_a_jittered_lattice():
     assert vol.min() > 0
     # the hull's flat cells are there: some have a circumradius far above h
     assert _circumradius(p, t).max() > 50 * h
+
+
+def test_3d_check_accepts_delaunay_cells_and_rejects_others():
+    """sm_host_delaunay3d_check passes the cells of the tetrahedralisation
+    and fails cells that are not Delaunay."""
+    from seismicmesh_amd import _lib
+    rng = np.random.default_rng(4)
+    p = rng.random((3000, 3)) * [2.0, 1.0, 1.0]
+
+    def check(cells):
+        c = np.ascontiguousarray(cells, dtype=np.int32).reshape(-1, 4)
+        r = _lib.lib.sm_host_delaunay3d_check(p.ctypes.data, len(p), c.ctypes.data, len(c), 4)
+        assert r >= 0, _lib.last_error()
+        return r
+
+    serial, _ = _tets(p)
+    assert check(serial) == 1
+    assert check(serial[::7]) == 1
+    # a cell of the tetrahedralisation with one vertex swapped for a far point
+    from oracle.predicates import orient3d
+    bad = serial[0].copy()
+    bad[3] = np.argmax(np.linalg.norm(p - p[bad].mean(0), axis=1))
+    if orient3d(*p[bad]) < 0:
+        bad[[2, 3]] = bad[[3, 2]]
+    assert check(bad) == 0
