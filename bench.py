@@ -501,6 +501,8 @@ def main():
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
     ap.add_argument("--cpu-iters", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--far-square", action="store_true",
+                    help="many: triangulate each domain inside a far square, not its hull")
     ap.add_argument("--no-mlp", action="store_true",
                     help="skip the surrogate inference measurement")
     ap.add_argument("--workgroups", default=None, metavar="N,N,N,N,N,N",
@@ -539,10 +541,11 @@ def main():
         doms, hs = workloads.many_domains(n=args.domains)
         sts = batch.prepare_batch(doms, hs)
         p0 = np.vstack([s_["p"] for s_ in sts])
-        m = batch.make_batch_mesher(sts)
+        m = batch.make_batch_mesher(sts, far_square=args.far_square)
         st = None
         wl = {"workload": "many_domains_2d", "domains": args.domains, "seed0": 1000,
-              "n_per_domain": "log-uniform [500, 2e5]", "max_iter": 50}
+              "n_per_domain": "log-uniform [500, 2e5]", "max_iter": 50,
+              "triangulated_region": "far square" if args.far_square else "convex hull"}
     else:
         if args.workload == "bp2004":
             domain, ef, info = workloads.bp2004(scale=args.scale)

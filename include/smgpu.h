@@ -242,14 +242,15 @@ typedef struct {
   int32_t nfix;           /* the first nfix points are fixed (pfix)          */
   int32_t nlevels;        /* number of level sets (domain + subdomains)      */
   double h0, delta_t, geps, deps, L0mult;
-  int32_t far_square;     /* 2D: 1 = triangulate inside a fixed square of four
-                             far vertices (100 extents from the points)
-                             instead of the convex hull of the points: no
-                             sequential hull-flip chains along straight
-                             boundaries; hull slivers whose circumcircle
-                             reaches that far (kept by the reference on
-                             degenerate initial lattices) are not formed.
-                             Batches always use it.  0 = convex hull.       */
+  int32_t far_square;     /* 2D: 1 = triangulate (each domain of a batch)
+                             inside a fixed square of four far vertices
+                             (100 extents from the points) instead of the
+                             convex hull of the points: no sequential
+                             hull-flip chains along straight boundaries;
+                             hull slivers whose circumcircle reaches that
+                             far (kept by the reference where their
+                             centroid passes the filter) are not formed.
+                             0 = convex hull, as the reference.             */
   int32_t reserved;
 } sm_mesher_params;
 

@@ -9,6 +9,14 @@ array, each domain keeps its own triangulation (no triangle or neighbour
 link crosses domains), its own L0 scale, centroid filter and projection
 (csrc: sm_mesher_create_batch).  Set-up (initial points) and termination
 are the reference's, per domain, on the host.  Forces are FP32.
+
+Each domain is triangulated over the convex hull of its points, as the
+reference's CGAL triangulation is, by default.  far_square=True
+triangulates it inside a fixed square of four far vertices at 100 times its
+extent instead: fewer flip rounds along straight, nearly collinear hull
+edges, at the price of leaving out the hull triangles whose circumcircle
+reaches a far vertex (flat triangles of nearly collinear hull points, which
+the reference keeps when their centroid passes the filter).
 """
 import ctypes
 
@@ -25,7 +33,8 @@ class BatchMesher(gen.Mesher):
     """A libsmgpu batch handle: `domains` (geometry objects), their uniform
     sizes `h`, fixed-point counts `nfix` and point counts `counts`."""
 
-    def __init__(self, domains, h, nfix, counts, delta_t=0.30, geps_mult=0.1, device=0):
+    def __init__(self, domains, h, nfix, counts, delta_t=0.30, geps_mult=0.1, device=0,
+                 far_square=False):
         _lib.require_gpu()
         self.dim = 2
         nd = len(domains)
@@ -44,7 +53,7 @@ class BatchMesher(gen.Mesher):
         prm.geps = geps_mult
         prm.deps = float(np.sqrt(np.finfo(np.double).eps))
         prm.L0mult = 1 + 0.4 / 2 ** (2 - 1)
-        prm.far_square = 1
+        prm.far_square = int(bool(far_square))
         hd = ctypes.c_void_p()
         _lib.check(_lib.lib.sm_mesher_create_batch(
             ctypes.byref(prm), nd, ptrs, lens, self.hsize.ctypes.data, self.nfix.ctypes.data,
@@ -76,12 +85,13 @@ def prepare_batch(domains, edge_lengths, seed=0, **kwargs):
     return sts
 
 
-def generate_mesh_batch(domains, edge_lengths, max_iter=50, seed=0, verbose=0, **kwargs):
+def generate_mesh_batch(domains, edge_lengths, max_iter=50, seed=0, verbose=0, far_square=False,
+                        **kwargs):
     """Mesh every domain of `domains` with its uniform edge length; returns
     a list of (points, cells), one per domain, each as generate_mesh would
     return it (FP32 forces)."""
     sts = prepare_batch(domains, edge_lengths, seed=seed, max_iter=max_iter, **kwargs)
-    m = make_batch_mesher(sts)
+    m = make_batch_mesher(sts, far_square=far_square)
     try:
         m.set_points(np.vstack([st["p"] for st in sts]))
         m.run(max(max_iter - 1, 0))
@@ -100,9 +110,9 @@ def generate_mesh_batch(domains, edge_lengths, max_iter=50, seed=0, verbose=0, *
     return out
 
 
-def make_batch_mesher(sts):
+def make_batch_mesher(sts, far_square=False):
     """A batch handle for prepared domains (initial points not set)."""
     return BatchMesher([st["dom"] for st in sts], [st["h0"] for st in sts],
                        [st["nfix"] for st in sts], [len(st["p"]) for st in sts],
                        delta_t=sts[0]["delta_t"],
-                       geps_mult=sts[0]["opts"]["geps_mult"])
+                       geps_mult=sts[0]["opts"]["geps_mult"], far_square=far_square)

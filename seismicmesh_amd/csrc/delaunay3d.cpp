@@ -78,6 +78,36 @@ struct Tet {
   int n[4];
 };
 
+// Growable array of tetrahedra whose new elements are left uninitialised
+// (the concurrent build sizes it for its whole capacity up front and
+// writes each slot before anything reads it).
+class TetArray {
+ public:
+  size_t size() const { return n_; }
+  Tet &operator[](size_t i) { return buf_[i]; }
+  const Tet &operator[](size_t i) const { return buf_[i]; }
+  void clear() { n_ = 0; }
+  void resize(size_t m) {
+    reserve(m);
+    n_ = m;
+  }
+  void push_back(const Tet &t) {
+    if (n_ == cap_) reserve(std::max<size_t>(64, 2 * cap_));
+    buf_[n_++] = t;
+  }
+  void reserve(size_t c) {
+    if (c <= cap_) return;
+    std::unique_ptr<Tet[]> nb(new Tet[c]);
+    if (n_) std::memcpy(nb.get(), buf_.get(), n_ * sizeof(Tet));
+    buf_ = std::move(nb);
+    cap_ = c;
+  }
+
+ private:
+  std::unique_ptr<Tet[]> buf_;
+  size_t n_ = 0, cap_ = 0;
+};
+
 inline uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
   // Skilling's transpose-form Hilbert index in 3D
   uint32_t X[3] = {x, y, z};
@@ -182,7 +212,7 @@ struct Builder3 {
   std::vector<double> P;          // local points (n, 3)
   std::vector<int64_t> gid;       // local -> global id
   std::vector<int64_t> dropped;   // global ids of skipped duplicates
-  std::vector<Tet> T;
+  TetArray T;
   std::vector<int> free_slots;
   std::vector<char> mark;
   std::vector<std::pair<uint64_t, int>> key;
@@ -443,6 +473,15 @@ struct Builder3 {
     return k;
   }
 
+  // fn(0 .. nt-1), on nt threads (this one included)
+  template <class Fn>
+  static void on_threads(int nt, Fn &&fn) {
+    std::vector<std::thread> pool;
+    for (int w = 1; w < nt; ++w) pool.emplace_back([&fn, w] { fn(w); });
+    fn(0);
+    for (auto &t : pool) t.join();
+  }
+
   // Triangulates the nloc points of P in Hilbert order (on nthreads
   // threads, in nslabs slabs along `axis`, when that is more than one).
   int build(const double lo[3], double ext, int nthreads = 1, int axis = 1, int nslabs = 1) {
@@ -452,14 +491,35 @@ struct Builder3 {
     const int n = nloc;
     last = 0;
     if (mark.size() < 64) mark.assign(64, 0);
+    // Hilbert keys (30 bits) on the threads, then a stable two-pass radix
+    // sort: the order of std::sort on (key, id)
     key.resize(n);
-    for (int i = 0; i < n; ++i) {
-      uint32_t g[3];
-      for (int k = 0; k < 3; ++k)
-        g[k] = (uint32_t)std::min(1023.0, std::max(0.0, (pt(i)[k] - lo[k]) / ext * 1023.0));
-      key[i] = {hilbert3(g[0], g[1], g[2], 10), i};
+    const int kt = std::max(1, std::min(nthreads, n / 32768 + 1));
+    on_threads(kt, [&](int w) {
+      for (int i = (int)((int64_t)n * w / kt); i < (int)((int64_t)n * (w + 1) / kt); ++i) {
+        uint32_t g[3];
+        for (int k = 0; k < 3; ++k)
+          g[k] = (uint32_t)std::min(1023.0, std::max(0.0, (pt(i)[k] - lo[k]) / ext * 1023.0));
+        key[i] = {hilbert3(g[0], g[1], g[2], 10), i};
+      }
+    });
+    {
+      std::vector<std::pair<uint64_t, int>> tmp((size_t)n);
+      std::vector<int> cnt(1 << 15);
+      for (int pass = 0; pass < 2; ++pass) {
+        const int sh = 15 * pass;
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (int i = 0; i < n; ++i) ++cnt[(key[i].first >> sh) & 0x7FFF];
+        int s = 0;
+        for (int &c : cnt) {
+          const int x = c;
+          c = s;
+          s += x;
+        }
+        for (int i = 0; i < n; ++i) tmp[(size_t)cnt[(key[i].first >> sh) & 0x7FFF]++] = key[i];
+        key.swap(tmp);
+      }
     }
-    std::sort(key.begin(), key.end());
     const size_t k0 = start(key);
     if (k0 == 0) {
       T.clear();
@@ -508,14 +568,17 @@ struct Builder3 {
     std::vector<int> used;
     int last = 0;
     int64_t blk = 0, blk_end = 0;  // this thread's block of fresh slots
+    std::vector<int> skipped;      // fresh slots passed over (found locked)
   };
-  std::vector<std::atomic<uint32_t>> *locks = nullptr;
+  std::atomic<uint32_t> *locks = nullptr;  // one per slot of the capacity
+  std::unique_ptr<std::atomic<uint32_t>[]> lockbuf;  // kept between builds, all 0
+  size_t lockcap = 0;
   std::atomic<int64_t> *ntet = nullptr;
   int64_t cap_par = 0;
   std::mutex *dup_mu = nullptr;
 
   bool trylock(TCtx &c, int t) {
-    std::atomic<uint32_t> &l = (*locks)[t];
+    std::atomic<uint32_t> &l = locks[t];
     uint32_t cur = l.load(std::memory_order_relaxed);
     if (cur == c.me) return true;
     if (cur != 0) return false;
@@ -528,7 +591,7 @@ struct Builder3 {
     for (int h : c.held)
       if (h != t) {
         mark[h] = 0;
-        (*locks)[h].store(0, std::memory_order_release);
+        locks[h].store(0, std::memory_order_release);
       }
     c.held.clear();
     c.held.push_back(t);
@@ -536,7 +599,7 @@ struct Builder3 {
   void unlock_all(TCtx &c) {
     for (int t : c.held) {
       mark[t] = 0;
-      (*locks)[t].store(0, std::memory_order_release);
+      locks[t].store(0, std::memory_order_release);
     }
     c.held.clear();
   }
@@ -558,6 +621,7 @@ struct Builder3 {
       }
       const int t = (int)c.blk++;
       if (trylock(c, t)) return t;
+      c.skipped.push_back(t);
     }
   }
   bool live_locked(TCtx &c, int t) {
@@ -739,44 +803,41 @@ struct Builder3 {
     return 1;
   }
 
-  // Inserts the points order[k0..] on nthreads threads: every 64th point
-  // first on this thread (spread over the whole curve), then the rest in
-  // chunks, then, on this thread alone, those that met a taken lock twice.
-  // The rest is cut into nslabs equal-width slabs along `axis` (each in
-  // curve order) and every slab into chunks; the chunks are handed out
-  // slab after slab in turn (chunk 0 of every slab, then chunk 1, ...), so
-  // that the threads insert in different slabs at any time.
+  // Inserts the points order[k0..] on nthreads threads, coarse to fine:
+  // every 4096th point on this thread, then every 64th and then the rest
+  // concurrently, then, on this thread alone, those that met a taken lock
+  // twice.  In each concurrent pass the points are cut into nslabs
+  // equal-width slabs along `axis` (each in curve order) and every slab
+  // into chunks; the chunks are handed out slab after slab in turn (chunk 0
+  // of every slab, then chunk 1, ...), so that the threads insert in
+  // different slabs at any time, and each chunk starts its walk at the
+  // tetrahedron made by the coarser point before it on the curve.
   int build_parallel(const std::vector<std::pair<uint64_t, int>> &order, size_t k0,
                      int nthreads, int axis, int nslabs) {
     const size_t n = order.size();
-    std::vector<int> early, rest0, pred0;
+    std::vector<int> lvl0, lvl1, pred1, rest, pred2;
     vtet.assign((size_t)nloc, -1);
+    int last0 = -1, last1 = -1;
     for (size_t k = k0; k < n; ++k) {
-      if ((k - k0) % 64 == 0) {
-        early.push_back(order[k].second);
+      const int v = order[k].second;
+      if ((k - k0) % 4096 == 0) {
+        lvl0.push_back(v);
+        last0 = last1 = v;
+      } else if ((k - k0) % 64 == 0) {
+        lvl1.push_back(v);
+        pred1.push_back(last0);
+        last1 = v;
       } else {
-        rest0.push_back(order[k].second);
-        pred0.push_back(early.back());  // the early point just before it on the curve
+        rest.push_back(v);
+        pred2.push_back(last1);
       }
     }
     double xlo = std::numeric_limits<double>::infinity(), xhi = -xlo;
-    for (int v : rest0) {
-      xlo = std::min(xlo, pt(v)[axis]);
-      xhi = std::max(xhi, pt(v)[axis]);
+    for (size_t k = k0; k < n; ++k) {
+      xlo = std::min(xlo, pt(order[k].second)[axis]);
+      xhi = std::max(xhi, pt(order[k].second)[axis]);
     }
-    const int S = std::max(1, nslabs);
-    const double width = (xhi - xlo) / S;
-    std::vector<std::vector<int>> slab((size_t)S), spred((size_t)S);
-    for (size_t i = 0; i < rest0.size(); ++i) {
-      const int v = rest0[i];
-      int k = width > 0 ? (int)std::floor((pt(v)[axis] - xlo) / width) : 0;
-      k = std::min(S - 1, std::max(0, k));
-      slab[(size_t)k].push_back(v);
-      spred[(size_t)k].push_back(pred0[i]);
-    }
-    slab_sizes.assign((size_t)S, 0);
-    for (int k = 0; k < S; ++k) slab_sizes[k] = (int64_t)slab[k].size();
-    for (int v : early) {
+    for (int v : lvl0) {
       int rc = insert(v);
       if (rc < 0) return rc;
     }
@@ -785,20 +846,23 @@ struct Builder3 {
       if (rc < 0) return rc;
     }
     deferred.clear();
-    // fixed capacity; the dead slots of the serial phase go to thread 0
+    // fixed capacity, fresh slots left unwritten until handed out; the
+    // dead slots of the serial phase go to thread 0
     cap_par = std::max<int64_t>((int64_t)T.size() + 64, 8 * (int64_t)nloc + 4096);
     if (cap_par > (int64_t)std::numeric_limits<int>::max())
       return sm_set_error(SM_ERR_CAPACITY, "host 3D Delaunay: too many points");
     std::atomic<int64_t> nt_count{(int64_t)T.size()};
     const size_t old_n = T.size();
     T.resize((size_t)cap_par);
-    for (size_t t = old_n; t < (size_t)cap_par; ++t) T[t].v[0] = kDead;  // fresh
     for (int f : free_slots) T[f].v[0] = kDead;
     mark.assign((size_t)cap_par, 0);
-    std::vector<std::atomic<uint32_t>> lk((size_t)cap_par);
-    for (auto &a : lk) a.store(0, std::memory_order_relaxed);
+    if (lockcap < (size_t)cap_par) {
+      lockbuf.reset(new std::atomic<uint32_t>[(size_t)cap_par]);
+      for (size_t i = 0; i < (size_t)cap_par; ++i) lockbuf[i].store(0, std::memory_order_relaxed);
+      lockcap = (size_t)cap_par;
+    }
     std::mutex dmu;
-    locks = &lk;
+    locks = lockbuf.get();
     ntet = &nt_count;
     dup_mu = &dmu;
     const int nthr = std::max(1, nthreads);
@@ -810,65 +874,25 @@ struct Builder3 {
       ctx[w].last = last;
       ctx[w].recent.push_back(last);
     }
-    // chunks: (slab, begin, end), interleaved across the slabs
-    const size_t chunk =
-        std::max<size_t>(256, rest0.size() / ((size_t)nthr * 8) + 1);
-    std::vector<std::array<size_t, 3>> chunks;
-    for (size_t r = 0;; ++r) {
-      bool any = false;
-      for (int k = 0; k < S; ++k)
-        if (r * chunk < slab[k].size()) {
-          chunks.push_back({(size_t)k, r * chunk, std::min(slab[k].size(), (r + 1) * chunk)});
-          any = true;
-        }
-      if (!any) break;
-    }
-    const size_t nchunks = chunks.size();
-    std::atomic<size_t> next{0};
-    std::atomic<int> fail{0};
-    auto worker = [&](int w) {
-      TCtx &c = ctx[w];
-      for (;;) {
-        const size_t ch = next.fetch_add(1);
-        if (ch >= nchunks || fail.load(std::memory_order_relaxed)) break;
-        const std::vector<int> &pts = slab[chunks[ch][0]];
-        // start near the chunk, at the tetrahedron made by the early point
-        // before its first point on the curve when that one still lives
-        // (the previous position stays as the fallback)
-        {
-          const int hv = vtet[spred[chunks[ch][0]][chunks[ch][1]]];
-          if (hv >= 0) {
-            c.recent.push_back(c.last);
-            c.last = hv;
-          }
-        }
-        std::vector<int> again;
-        for (size_t k = chunks[ch][1]; k < chunks[ch][2]; ++k) {
-          const int rc = par_insert(c, pts[k]);
-          if (rc == 2) again.push_back(pts[k]);
-          else if (rc < 0) {
-            fail = rc;
-            return;
-          }
-        }
-        for (int v : again) {
-          const int rc = par_insert(c, v);
-          if (rc == 2) c.retry.push_back(v);
-          else if (rc < 0) {
-            fail = rc;
-            return;
-          }
-        }
-      }
-    };
-    std::vector<std::thread> pool;
-    for (int w = 1; w < nthr; ++w) pool.emplace_back(worker, w);
-    worker(0);
-    for (auto &t : pool) t.join();
-    int rc = fail.load();
-    // what is left: one thread, no contention (every lock was released)
+    slab_sizes.assign((size_t)std::max(1, nslabs), 0);
+    int rc = concurrent_pass(lvl1, pred1, ctx, axis, nslabs, xlo, xhi);
+    if (rc == 0) rc = concurrent_pass(rest, pred2, ctx, axis, nslabs, xlo, xhi);
+    // points that met taken locks twice: more concurrent passes while there
+    // are many of them (contention is a matter of timing), then one thread
     retried = 0;
     for (int w = 0; w < nthr; ++w) retried += (int64_t)ctx[w].retry.size();
+    for (int round = 0; round < 3 && rc == 0; ++round) {
+      std::vector<int> pend;
+      for (auto &c : ctx) {
+        pend.insert(pend.end(), c.retry.begin(), c.retry.end());
+        c.retry.clear();
+      }
+      if (pend.size() < 2048) {
+        ctx[0].retry.swap(pend);
+        break;
+      }
+      rc = concurrent_pass(pend, pend, ctx, axis, nslabs, xlo, xhi, false);
+    }
     for (int w = 0; w < nthr && rc == 0; ++w)
       for (int v : ctx[w].retry) {
         const int r2 = par_insert(ctx[0], v);
@@ -877,6 +901,11 @@ struct Builder3 {
           break;
         }
       }
+    // slots handed out in blocks but never used were never written
+    for (auto &c : ctx) {
+      for (int64_t t = c.blk; t < c.blk_end; ++t) T[(size_t)t].v[0] = kDead;
+      for (int t : c.skipped) T[(size_t)t].v[0] = kDead;
+    }
     // back to the serial layout: dead slots become free slots
     const int64_t used_n = std::min<int64_t>(nt_count.load(), cap_par);
     T.resize((size_t)std::max<int64_t>(used_n, (int64_t)old_n));
@@ -895,6 +924,74 @@ struct Builder3 {
       }
     return rc;
   }
+
+  // One concurrent pass over pts (pred[i]: a point inserted before pts[i],
+  // next to it on the curve).
+  int concurrent_pass(const std::vector<int> &pts, const std::vector<int> &pred,
+                      std::vector<TCtx> &ctx, int axis, int nslabs, double xlo, double xhi,
+                      bool count = true) {
+    const int S = std::max(1, nslabs);
+    const double width = (xhi - xlo) / S;
+    std::vector<std::vector<int>> slab((size_t)S), spred((size_t)S);
+    for (size_t i = 0; i < pts.size(); ++i) {
+      const int v = pts[i];
+      int k = width > 0 ? (int)std::floor((pt(v)[axis] - xlo) / width) : 0;
+      k = std::min(S - 1, std::max(0, k));
+      slab[(size_t)k].push_back(v);
+      spred[(size_t)k].push_back(pred[i]);
+    }
+    if (count)
+      for (int k = 0; k < S; ++k) slab_sizes[k] += (int64_t)slab[k].size();
+    const int nthr = (int)ctx.size();
+    const size_t chunk = std::max<size_t>(256, pts.size() / ((size_t)nthr * 8) + 1);
+    std::vector<std::array<size_t, 3>> chunks;
+    for (size_t r = 0;; ++r) {
+      bool any = false;
+      for (int k = 0; k < S; ++k)
+        if (r * chunk < slab[k].size()) {
+          chunks.push_back({(size_t)k, r * chunk, std::min(slab[k].size(), (r + 1) * chunk)});
+          any = true;
+        }
+      if (!any) break;
+    }
+    const size_t nchunks = chunks.size();
+    std::atomic<size_t> next{0};
+    std::atomic<int> fail{0};
+    on_threads(nthr, [&](int w) {
+      TCtx &c = ctx[w];
+      std::vector<int> again;
+      for (;;) {
+        const size_t ch = next.fetch_add(1);
+        if (ch >= nchunks || fail.load(std::memory_order_relaxed)) break;
+        const std::vector<int> &sp = slab[chunks[ch][0]];
+        // start near the chunk when that tetrahedron still lives (the
+        // previous position stays as the fallback)
+        const int hv = vtet[spred[chunks[ch][0]][chunks[ch][1]]];
+        if (hv >= 0) {
+          c.recent.push_back(c.last);
+          c.last = hv;
+        }
+        again.clear();
+        for (size_t k = chunks[ch][1]; k < chunks[ch][2]; ++k) {
+          const int rc = par_insert(c, sp[k]);
+          if (rc == 2) again.push_back(sp[k]);
+          else if (rc < 0) {
+            fail = rc;
+            return;
+          }
+        }
+        for (int v : again) {
+          const int rc = par_insert(c, v);
+          if (rc == 2) c.retry.push_back(v);
+          else if (rc < 0) {
+            fail = rc;
+            return;
+          }
+        }
+      }
+    });
+    return fail.load();
+  }
 };
 
 
@@ -902,7 +999,8 @@ struct Builder3 {
 DT3::DT3() : B(new Builder3()) {}
 DT3::~DT3() = default;
 
-int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, const Emit &emit) {
+int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, const Emit &emit,
+             int32_t *out, int64_t out_cap) {
   if (n < 0 || (n > 0 && !p) || axis < 0 || axis > 2 || nslabs < 1 || nthreads < 1)
     return sm_set_error(SM_ERR_ARG, "delaunay3d: bad arguments");
   if (n > (int64_t)std::numeric_limits<int32_t>::max() / 8)
@@ -945,47 +1043,49 @@ int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, con
   const int64_t nslot = (int64_t)b.T.size();
   for (int f : b.free_slots) b.T[f].v[0] = Builder3::kDead;  // free slots: stale contents
   const int nthr = std::max(1, std::min(nthreads, (int)(nslot / 65536 + 1)));
-  auto on_threads = [nthr](const std::function<void(int)> &fn) {
-    std::vector<std::thread> pool;
-    for (int w = 1; w < nthr; ++w) pool.emplace_back(fn, w);
-    fn(0);
-    for (auto &t : pool) t.join();
-  };
-  std::vector<std::vector<int32_t>> part((size_t)nthr);
-  auto extract = [&](int w) {
+  // thread w extracts slots [nslot w / nthr, ...) into part[w][r], r the
+  // range of lowest ids [n r / nthr, n (r + 1) / nthr) the cell belongs to
+  auto lo_id = [&](int w) { return n * w / nthr; };
+  std::vector<std::vector<std::vector<int32_t>>> part(
+      (size_t)nthr, std::vector<std::vector<int32_t>>((size_t)nthr));
+  Builder3::on_threads(nthr, [&](int w) {
     const int64_t s0 = nslot * w / nthr, s1 = nslot * (w + 1) / nthr;
-    std::vector<int32_t> &out = part[w];
-    out.clear();
+    for (auto &o : part[w]) o.clear();
     for (int64_t t = s0; t < s1; ++t) {
       const Tet &r = b.T[t];
       if (r.v[0] < 0 || Builder3::inf_slot(r) >= 0) continue;  // dead or infinite
       int32_t v[4];
       for (int j = 0; j < 4; ++j) v[j] = (int32_t)b.gid[r.v[j]];
       canonical_tet(v);
-      out.insert(out.end(), v, v + 4);
+      int rg = (int)((int64_t)v[0] * nthr / std::max<int64_t>(n, 1));
+      while (rg + 1 < nthr && lo_id(rg + 1) <= v[0]) ++rg;
+      while (rg > 0 && lo_id(rg) > v[0]) --rg;
+      part[w][rg].insert(part[w][rg].end(), v, v + 4);
     }
-  };
-  on_threads(extract);
+  });
   int64_t m = 0;
-  for (auto &v : part) m += (int64_t)v.size() / 4;
-  // each thread owns the buckets of a range of lowest ids: it counts them,
-  // then (after one scan of the range totals) places and sorts them
+  for (auto &pw : part)
+    for (auto &v : pw) m += (int64_t)v.size() / 4;
+  // each thread owns the buckets of its range of lowest ids: it counts
+  // them, then (after one scan of the range totals) places and sorts them
   start_.assign((size_t)n + 1, 0);
-  sorted_.resize((size_t)m * 4);
+  int32_t *dst = out;
+  if (!out || out_cap < m) {
+    sorted_.resize((size_t)m * 4);
+    dst = sorted_.data();
+  }
   std::vector<int64_t> below((size_t)nthr + 1, 0);
-  auto lo_id = [&](int w) { return n * w / nthr; };
-  on_threads([&](int w) {
+  Builder3::on_threads(nthr, [&](int w) {
     const int64_t a = lo_id(w), b = lo_id(w + 1);
-    for (auto &v : part)
-      for (size_t i = 0; i < v.size(); i += 4)
-        if (v[i] >= a && v[i] < b) ++start_[(size_t)v[i] + 1];
     int64_t s = 0;
+    for (auto &pw : part)
+      for (size_t i = 0; i < pw[w].size(); i += 4) ++start_[(size_t)pw[w][i] + 1];
     for (int64_t i = a; i < b; ++i) s += start_[(size_t)i + 1];
     below[(size_t)w + 1] = s;
   });
   for (int w = 0; w < nthr; ++w) below[(size_t)w + 1] += below[(size_t)w];
-  auto *cells = reinterpret_cast<std::array<int32_t, 4> *>(sorted_.data());
-  on_threads([&](int w) {
+  auto *cells = reinterpret_cast<std::array<int32_t, 4> *>(dst);
+  Builder3::on_threads(nthr, [&](int w) {
     // start_[i + 1] holds the count of bucket i; this thread writes
     // start_[a + 1 .. b] only
     const int64_t a = lo_id(w), b = lo_id(w + 1);
@@ -996,12 +1096,13 @@ int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, con
       run += start_[(size_t)i + 1];
       start_[(size_t)i + 1] = run;
     }
-    for (auto &v : part)
-      for (size_t i = 0; i < v.size(); i += 4)
-        if (v[i] >= a && v[i] < b) {
-          const int64_t o = fill[(size_t)(v[i] - a)]++;
-          for (int j = 0; j < 4; ++j) sorted_[4 * o + j] = v[i + j];
-        }
+    for (auto &pw : part) {
+      const std::vector<int32_t> &v = pw[w];
+      for (size_t i = 0; i < v.size(); i += 4) {
+        const int64_t o = fill[(size_t)(v[i] - a)]++;
+        for (int j = 0; j < 4; ++j) dst[4 * o + j] = v[i + j];
+      }
+    }
     for (int64_t i = a; i < b; ++i) {
       const int64_t s0 = i == a ? below[(size_t)w] : start_[(size_t)i];
       if (start_[(size_t)i + 1] - s0 > 1) std::sort(cells + s0, cells + start_[(size_t)i + 1]);
@@ -1011,7 +1112,7 @@ int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, con
   const int pieces = std::max(1, nslabs);
   for (int k = 0; k < pieces; ++k) {
     const int64_t c0 = m * k / pieces, c1 = m * (k + 1) / pieces;
-    emit(k, sorted_.data() + 4 * c0, c1 - c0);
+    emit(k, dst + 4 * c0, c1 - c0);
   }
   emit(-1, nullptr, 0);
   return SM_OK;

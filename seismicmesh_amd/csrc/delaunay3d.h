@@ -34,8 +34,12 @@ struct DT3 {
   DT3();
   ~DT3();
   // nthreads threads insert the points of nslabs slabs along `axis` into
-  // one shared tetrahedralisation (one thread: the serial build).
-  int run(const double *p, int64_t n, int axis, int nslabs, int nthreads, const Emit &emit);
+  // one shared tetrahedralisation (one thread: the serial build).  The
+  // sorted cells are written to `out` (room for out_cap cells) when it is
+  // given and large enough, else to an internal buffer; the pieces point
+  // into it.
+  int run(const double *p, int64_t n, int axis, int nslabs, int nthreads, const Emit &emit,
+          int32_t *out = nullptr, int64_t out_cap = 0);
   // Checks cells (count, 4; positively oriented) against every one of the n
   // points: 1 when no point lies inside any cell's sphere (perturbed test),
   // 0 when one does or a sphere is too wide to check here (more than

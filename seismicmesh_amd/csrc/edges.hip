@@ -75,6 +75,81 @@ struct PairSrc {
   }
 };
 
+// Tetrahedra: with the ids sorted, s0 < s1 < s2 < s3, the six edges are
+// three in bucket s0 (partners s1, s2, s3), two in bucket s1 (s2, s3) and
+// one in bucket s2 (s3): three contiguous runs instead of six pairs.  The
+// host tetrahedralisation hands out cells sorted by their lowest id, so
+// consecutive lanes share s0: the lanes of a run of equal s0 (consecutive,
+// all taking part) reserve their slots with one atomic by the run's first
+// lane.  Returns false for a lane without a cell.
+struct TetRun {
+  int32_t s[4];
+  bool ok, lead;
+  int len;    // lanes in the run (leader only)
+  int first;  // the run's first lane
+};
+
+__device__ inline TetRun tet_run(const PairSrc &src, int64_t i) {
+  TetRun r;
+  r.ok = i < src.n && (!src.keep || src.keep[i]);
+  if (r.ok) {
+    const int32_t *t = src.cells + 4 * i;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) r.s[k] = t[k];
+    // sorting network on four values
+#define SM_CSWAP(a, b)                      \
+  {                                         \
+    const int32_t x = r.s[a], y = r.s[b];   \
+    r.s[a] = x < y ? x : y;                 \
+    r.s[b] = x < y ? y : x;                 \
+  }
+    SM_CSWAP(0, 1) SM_CSWAP(2, 3) SM_CSWAP(0, 2) SM_CSWAP(1, 3) SM_CSWAP(1, 2)
+#undef SM_CSWAP
+  } else {
+    r.s[0] = r.s[1] = r.s[2] = r.s[3] = -1;
+  }
+  const int lane = threadIdx.x % 64;
+  const int32_t prev = __shfl_up(r.s[0], 1);
+  const uint64_t okm = __ballot(r.ok);
+  const bool prev_ok = lane > 0 && ((okm >> (lane - 1)) & 1);
+  r.lead = r.ok && !(prev_ok && prev == r.s[0]);
+  const uint64_t brk = __ballot(!r.ok || r.lead);
+  const uint64_t after = lane == 63 ? 0 : brk >> (lane + 1);
+  r.len = after ? 1 + __builtin_ctzll(after) : 64 - lane;
+  const uint64_t starts = __ballot(r.lead) & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+  r.first = starts ? 63 - __builtin_clzll(starts) : lane;
+  return r;
+}
+
+__global__ void k_count_tets(PairSrc src, int32_t *cnt) {
+  const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  const TetRun r = tet_run(src, i);
+  if (r.lead) atomicAdd(&cnt[r.s[0]], 3 * r.len);
+  if (r.ok) {
+    atomicAdd(&cnt[r.s[1]], 2);
+    atomicAdd(&cnt[r.s[2]], 1);
+  }
+}
+
+__global__ void k_scatter_tets(PairSrc src, const int32_t *rowptr, int32_t *cur,
+                               int32_t *col) {
+  const int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
+  const TetRun r = tet_run(src, i);
+  int32_t base = 0;
+  if (r.lead) base = rowptr[r.s[0]] + atomicAdd(&cur[r.s[0]], 3 * r.len);
+  base = __shfl(base, r.first);
+  if (!r.ok) return;
+  const int lane = threadIdx.x % 64;
+  const int32_t p0 = base + 3 * (lane - r.first);
+  col[p0] = r.s[1];
+  col[p0 + 1] = r.s[2];
+  col[p0 + 2] = r.s[3];
+  const int32_t p1 = rowptr[r.s[1]] + atomicAdd(&cur[r.s[1]], 2);
+  col[p1] = r.s[2];
+  col[p1 + 1] = r.s[3];
+  col[rowptr[r.s[2]] + atomicAdd(&cur[r.s[2]], 1)] = r.s[3];
+}
+
 // cnt2 (optional): the same edges counted by their second endpoint
 __global__ void k_count(PairSrc src, int32_t *cnt, int32_t *cnt2) {
   int64_t i = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
@@ -128,25 +203,37 @@ __device__ inline void sort16(int32_t (&v)[16]) {
   }
 }
 
+// Appends `want` lanes' values to list (count at *n) with one atomic per
+// wave: the first wanting lane reserves the wave's slots, every lane takes
+// the slot of its rank among the wanting lanes.
+__device__ inline void wave_append(bool want, int32_t value, int32_t *list, int32_t *n) {
+  const uint64_t m = __ballot(want);
+  if (!m) return;
+  const int lane = threadIdx.x % 64;
+  const int leader = __builtin_ctzll(m);
+  int32_t base = 0;
+  if (lane == leader) base = atomicAdd(n, (int32_t)__popcll(m));
+  base = __shfl(base, leader);
+  if (want) list[base + (int32_t)__popcll(m & ((1ull << lane) - 1))] = value;
+}
+
 // Per-bucket sort + unique count for short buckets (one thread each);
 // medium buckets (a 3D mesh's: each edge comes once per incident cell) are
-// listed for the one-wave pass, long ones for the workgroup pass.
+// listed for the one-wave pass, long ones for the workgroup pass (one
+// atomic per wave and list: in 3D nearly every row is listed, and one
+// atomic per row on the same two counters serialises).
 __global__ void k_row_sort(int64_t nrows, const int32_t *rowptr, int32_t *col, int32_t *ucnt,
                            int32_t *long_rows, int32_t *nlong, int32_t *mid_rows,
                            int32_t *nmid) {
   int64_t r = (int64_t)xcd_block() * blockDim.x + threadIdx.x;
   if (r == 0) ucnt[nrows] = 0;  // the scan's last input (its total slot)
-  if (r >= nrows) return;
-  int32_t s = rowptr[r], e = rowptr[r + 1];
-  int32_t len = e - s;
+  const bool row = r < nrows;
+  const int32_t s = row ? rowptr[r] : 0, e = row ? rowptr[r + 1] : 0;
+  const int32_t len = e - s;
+  wave_append(row && len > kRegSort && len <= kWaveSort, (int32_t)r, mid_rows, nmid);
+  wave_append(row && len > kWaveSort, (int32_t)r, long_rows, nlong);
+  if (!row) return;
   if (len > kRegSort) {
-    if (len <= kWaveSort) {
-      int32_t slot = atomicAdd(nmid, 1);
-      mid_rows[slot] = (int32_t)r;
-    } else {
-      int32_t slot = atomicAdd(nlong, 1);
-      long_rows[slot] = (int32_t)r;
-    }
     ucnt[r] = 0;  // filled by the wave or workgroup pass
     return;
   }
@@ -354,8 +441,13 @@ int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *
   PairSrc src{pairs, cells, keep, adj, dim, n};
   const int B = 256;
   int32_t *total = count32 ? count32 : ctr + 3;
+  // tetrahedra without neighbour links: three runs per cell
+  const bool tets = cells && dim == 3 && !adj && !second;
   if (n > 0) {
-    hipLaunchKernelGGL(k_count, dim3(blocks_for(n, B)), dim3(B), 0, s, src, rowptr, sl.ptr);
+    if (tets)
+      hipLaunchKernelGGL(k_count_tets, dim3(blocks_for(n, B)), dim3(B), 0, s, src, rowptr);
+    else
+      hipLaunchKernelGGL(k_count, dim3(blocks_for(n, B)), dim3(B), 0, s, src, rowptr, sl.ptr);
     SM_LAUNCH_CHECK();
   }
   // with adjacency the buckets hold no duplicates: their offsets and total
@@ -366,7 +458,11 @@ int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *
     rc = scan_exclusive(rowptr, rowptr, nvert + 1, adj ? total : ctr + 1, sws, s);
   if (rc) return rc;
   if (n > 0) {
-    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, B)), dim3(B), 0, s, src, rowptr, cur, col);
+    if (tets)
+      hipLaunchKernelGGL(k_scatter_tets, dim3(blocks_for(n, B)), dim3(B), 0, s, src, rowptr, cur,
+                         col);
+    else
+      hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n, B)), dim3(B), 0, s, src, rowptr, cur, col);
     SM_LAUNCH_CHECK();
   }
   if (nvert > 0) {

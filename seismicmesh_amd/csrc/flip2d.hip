@@ -617,40 +617,38 @@ int time_flip_mark(sm_mesher *m, int reps, double *ms) {
   SM_CHECK(reps >= 1 && reps <= 100000, SM_ERR_ARG, "time_kernel: bad repetition count");
   hipStream_t s = m->stream;
   if (!m->d_tctr) SM_HIP(hipMalloc((void **)&m->d_tctr, 8 * sizeof(int32_t)));
-  // counters of an open round (not finished, round id 0, no inverted or
-  // uncertain triangle: the loop's usual state, in which k_mark does not
-  // read orientations); uncertain predicates are only counted, so the
-  // launches write claims and candidates, which every flip round
-  // recomputes before use
-  int rc = fill<int32_t>(m->d_tctr, 8, 0, s);
-  if (rc) return rc;
+  // every launch starts from the counters of an open round (not
+  // finished, round id 0, no inverted or uncertain triangle: the loop's
+  // usual state, in which k_mark does not read orientations), cleared
+  // before its event pair -- an uncertain incircle test counted by one
+  // launch would otherwise switch every later one to the tangled path,
+  // which also loads the orientations.  The launches write claims and
+  // candidates, which every flip round recomputes before use.
   const int64_t T = m->T;
   const int B = m->blk[kBlkMark];
   const unsigned g = blocks_for(T, B);
   hipEvent_t e0, e1;
   SM_HIP(hipEventCreate(&e0));
   SM_HIP(hipEventCreate(&e1));
-  hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, m->d_p,
-                     m->d_claim, m->d_cand, m->d_queue + T, m->d_tctr, 0, ActView(), 0);
-  SM_HIP(hipEventRecord(e0, s));
-  for (int i = 0; i < reps; ++i)
+  double total = 0.0;
+  int rc = SM_OK;
+  for (int i = 0; i <= reps && rc == SM_OK; ++i) {
+    if ((rc = fill<int32_t>(m->d_tctr, 8, 0, s))) break;
+    if (hipEventRecord(e0, s) != hipSuccess) rc = sm_set_error(SM_ERR_HIP, "time_kernel: event");
     hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, m->d_p,
                        m->d_claim, m->d_cand, m->d_queue + T, m->d_tctr, 0, ActView(), 0);
-  SM_HIP(hipEventRecord(e1, s));
-  SM_LAUNCH_CHECK();
-  SM_HIP(hipEventSynchronize(e1));
-  float el = 0.0f;
-  SM_HIP(hipEventElapsedTime(&el, e0, e1));
+    if (hipGetLastError() != hipSuccess || hipEventRecord(e1, s) != hipSuccess ||
+        hipEventSynchronize(e1) != hipSuccess)
+      rc = sm_set_error(SM_ERR_HIP, "time_kernel: launch or event failed");
+    float el = 0.0f;
+    if (rc == SM_OK && hipEventElapsedTime(&el, e0, e1) != hipSuccess)
+      rc = sm_set_error(SM_ERR_HIP, "time_kernel: elapsed time");
+    if (i > 0) total += el;  // the first launch warms up
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  // an uncertain incircle test in any launch sets counter 5, after which
-  // every later launch also loads the orientations (the tangled path): the
-  // figure would then not be the usual state's, so refuse it
-  int32_t tc[8];
-  SM_HIP(hipMemcpy(tc, m->d_tctr, sizeof(tc), hipMemcpyDeviceToHost));
-  SM_CHECK(tc[5] == 0, SM_ERR_STATE,
-           "time_kernel: %d uncertain predicates switched k_mark to the tangled path", tc[5]);
-  *ms = (double)el / reps;
+  if (rc) return rc;
+  *ms = total / reps;
   return SM_OK;
 }
 
@@ -939,7 +937,7 @@ int relocate_inverted(sm_mesher *m, int64_t *moved) {
   if ((int64_t)m->h_prel.size() != 4 * NA) m->h_prel.assign(4 * NA, 0.0);
   double *hp = m->h_prel.data(), *hold = hp + 2 * NA;
   if (m->ndom > 0) {
-    // points of the failed domains and their far vertices
+    // points of the failed domains (and their far vertices)
     for (int d = 0; d < m->ndom; ++d) {
       if (!m->h_dfail[d]) continue;
       const int64_t b = m->h_voff[d], n = m->h_voff[d + 1] - b, f = m->N + 4 * d;
@@ -947,6 +945,7 @@ int relocate_inverted(sm_mesher *m, int64_t *moved) {
                             hipMemcpyDeviceToHost, s));
       SM_HIP(hipMemcpyAsync(hold + 2 * b, m->d_pvalid + 2 * b, 2 * n * sizeof(double),
                             hipMemcpyDeviceToHost, s));
+      if (m->nfar == 0) continue;
       SM_HIP(hipMemcpyAsync(hp + 2 * f, m->d_p + 2 * f, 8 * sizeof(double),
                             hipMemcpyDeviceToHost, s));
       SM_HIP(hipMemcpyAsync(hold + 2 * f, m->d_pvalid + 2 * f, 8 * sizeof(double),
@@ -1073,10 +1072,12 @@ static int64_t batch_delaunay(sm_mesher *m, const double *hp, int32_t *tri, int3
     for (int k; (k = next.fetch_add(1)) < (int)todo.size();) {
       const int d = todo[k];
       const int64_t b = m->h_voff[d], n = m->h_voff[d + 1] - b;
-      // the domain's points, then its four far vertices
+      // the domain's points, then its four far vertices if it has them
+      // (otherwise its convex hull, ringed by ghost triangles)
       loc.assign(hp + 2 * b, hp + 2 * (b + n));
-      loc.insert(loc.end(), hp + 2 * (N + 4 * d), hp + 2 * (N + 4 * d + 4));
-      const int64_t na = n + 4, tb = m->h_tbase[d], cap = m->h_tbase[d + 1] - tb;
+      const int64_t nf = m->nfar > 0 ? 4 : 0;
+      if (nf) loc.insert(loc.end(), hp + 2 * (N + 4 * d), hp + 2 * (N + 4 * d + 4));
+      const int64_t na = n + nf, tb = m->h_tbase[d], cap = m->h_tbase[d + 1] - tb;
       int32_t *td = tri + 3 * base[k], *ad = adj + 3 * base[k];
       ins.resize(na);
       int64_t c = sm_host_delaunay2d(loc.data(), na, td, ad, cap, ins.data());
@@ -1129,8 +1130,9 @@ static int rebuild_flagged_domains(sm_mesher *m) {
     const int64_t b = m->h_voff[d], n = m->h_voff[d + 1] - b, f = m->N + 4 * d;
     SM_HIP(hipMemcpyAsync(hp + 2 * b, m->d_p + 2 * b, 2 * n * sizeof(double),
                           hipMemcpyDeviceToHost, s));
-    SM_HIP(hipMemcpyAsync(hp + 2 * f, m->d_p + 2 * f, 8 * sizeof(double), hipMemcpyDeviceToHost,
-                          s));
+    if (m->nfar > 0)
+      SM_HIP(hipMemcpyAsync(hp + 2 * f, m->d_p + 2 * f, 8 * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
   }
   SM_HIP(hipStreamSynchronize(s));
   std::unique_ptr<int32_t[]> tri(new int32_t[3 * slots]), adj(new int32_t[3 * slots]);
@@ -1170,6 +1172,8 @@ int host_rebuild2d(sm_mesher *m) {
     T = sm_host_delaunay2d(hp, NA, tri, adj, m->capT, ins.data());
   } else {
     // fixed slot ranges: domain d owns 2 (n_d + 4) slots from h_tbase[d]
+    // (2 n_d + 6 triangles inside a far square, 2 n_d - 2 real and ghost
+    // triangles over its convex hull; the rest are dead)
     if ((int)m->h_tbase.size() != m->ndom + 1) {
       m->h_tbase.assign(m->ndom + 1, 0);
       std::vector<int32_t> tdom;

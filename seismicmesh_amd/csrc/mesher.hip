@@ -83,12 +83,13 @@ int alloc_point_state(sm_mesher *m, int64_t N) {
   free_point_state(m);
   const int dim = m->dim;
   m->capN = N;
-  m->nfar = dim == 2 && (m->ndom > 0 || m->prm.far_square) ? 4 * std::max(m->ndom, 1) : 0;
+  m->nfar = dim == 2 && m->prm.far_square ? 4 * std::max(m->ndom, 1) : 0;
   const int64_t NA = N + m->nfar;
-  // 2D: real + ghost triangles of the triangulated sphere, 2 NA - 2;
-  // 3D: tetrahedra of a Delaunay tetrahedralisation, ~6.5N for well-spaced
+  // 2D: real + ghost triangles of the triangulated sphere, 2 NA - 2 (a
+  // batch domain owns 2 (n_d + 4) slots, far square or not); 3D:
+  // tetrahedra of a Delaunay tetrahedralisation, ~6.5N for well-spaced
   // points, bounded here generously
-  m->capT = dim == 2 ? 2 * NA + 16 : 10 * N + 64;
+  m->capT = dim == 2 ? 2 * (N + 4 * (int64_t)std::max(m->ndom, 1)) + 16 : 10 * N + 64;
   const int64_t T = m->capT;
   const int64_t npairs = T * (dim == 2 ? 3 : 6);
   m->capE = npairs;
@@ -211,7 +212,8 @@ int host_rebuild3d(sm_mesher *m, bool *filtered) {
       over = true;
       return;
     }
-    std::memcpy(m->h_tet + 4 * off, v, (size_t)count * 4 * sizeof(int32_t));
+    if (v != m->h_tet + 4 * off)
+      std::memcpy(m->h_tet + 4 * off, v, (size_t)count * 4 * sizeof(int32_t));
     if (hipMemcpyAsync(m->d_tri + 4 * off, m->h_tet + 4 * off,
                        (size_t)count * 4 * sizeof(int32_t), hipMemcpyHostToDevice,
                        m->stream) != hipSuccess) {
@@ -226,7 +228,8 @@ int host_rebuild3d(sm_mesher *m, bool *filtered) {
     all_filtered &= done;
     off += count;
   };
-  int rc = m->dt3.run(m->h_pts, N, m->sl_axis, m->sl_n, m->sl_threads, emit);
+  // the cells are sorted straight into the pinned staging buffer
+  int rc = m->dt3.run(m->h_pts, N, m->sl_axis, m->sl_n, m->sl_threads, emit, m->h_tet, m->capT);
   if (rc) return rc;
   if (err) return err;
   SM_CHECK(!over, SM_ERR_CAPACITY, "host 3D Delaunay: more than %lld tetrahedra",

@@ -5,7 +5,10 @@
 // fill_value=None) over float32 linspace axes (z, x[, y]) -- the interval of
 // a query is the one searchsorted(axis, x) - 1 selects, clipped to the grid,
 // so queries outside the box are linearly extrapolated.  Index and weight
-// arithmetic is FP64; grid values are FP32.
+// arithmetic is FP64; grid values are FP32.  The corner sums are taken in
+// scipy's order (2D: its written formula left to right; 3D: the corners of
+// the hypercube in itertools.product order, each weight a left-to-right
+// product), which reproduces its float64 results exactly.
 #pragma once
 
 #include "common.h"
@@ -50,8 +53,9 @@ __device__ inline double grid_eval(const GridView &g, const double *x) {
     const float *v = g.values + (int64_t)i[0] * g.n[1] + i[1];
     double v00 = v[0], v01 = v[1];
     double v10 = v[g.n[1]], v11 = v[g.n[1] + 1];
-    double a = (1.0 - w[0]), b = w[0], c = (1.0 - w[1]), d = w[1];
-    return (v00 * (a * c) + v01 * (a * d)) + (v10 * (b * c) + v11 * (b * d));
+    const double a = 1.0 - w[0], b = w[0], c = 1.0 - w[1], d = w[1];
+    // scipy's 2D linear formula, left to right (its results bit for bit)
+    return v00 * a * c + v01 * a * d + v10 * b * c + v11 * b * d;
   } else {
     const int64_t s1 = g.n[2], s0 = (int64_t)g.n[1] * g.n[2];
     const float *v = g.values + (int64_t)i[0] * s0 + (int64_t)i[1] * s1 + i[2];

@@ -1,9 +1,10 @@
 """Many domains in one handle (GPU) against one handle per domain.
 
-Each domain of a batch has its own triangulation inside its own far
-square, so the batch's cells and edges of every domain are identical to a
-single-domain handle's with a far square on the same points in general
-position; point updates differ only through the order of the FP32 L0 sums
+Each domain of a batch has its own triangulation -- of its convex hull,
+or inside its own far square -- so the batch's cells and edges of every
+domain are identical to a single-domain handle's triangulating the same
+region on the same points in general position; point updates differ only
+through the order of the FP32 L0 sums
 (per-domain reduction instead of the single handle's block partials), so
 one iteration agrees to 1e-5 of the update."""
 import numpy as np
@@ -40,11 +41,12 @@ def _jitter(sts, seed=7):
     return sts
 
 
-def test_batch_matches_single_handles():
+@pytest.mark.parametrize("far_square", [False, True], ids=["hull", "far_square"])
+def test_batch_matches_single_handles(far_square):
     from seismicmesh_amd import batch, generation
     doms, hs = _domains()
     sts = _jitter(batch.prepare_batch(doms, hs))
-    bm = batch.make_batch_mesher(sts)
+    bm = batch.make_batch_mesher(sts, far_square=far_square)
     bm.set_graph(False)
     bm.set_points(np.vstack([st["p"] for st in sts]))
     bm.triangulate()
@@ -52,7 +54,7 @@ def test_batch_matches_single_handles():
     be = bm.edges()
     singles = []
     for d, st in enumerate(sts):
-        m = generation.make_mesher(st, far_square=True)
+        m = generation.make_mesher(st, far_square=far_square)
         m.set_graph(False)
         m.set_points(st["p"])
         m.triangulate()
@@ -76,11 +78,13 @@ def test_batch_matches_single_handles():
     bm.close()
 
 
-def test_batch_graph_matches_plain():
+@pytest.mark.parametrize("far_square", [False, True], ids=["hull", "far_square"])
+def test_batch_graph_matches_plain(far_square):
     from seismicmesh_amd import batch
     doms, hs = _domains()
     sts = batch.prepare_batch(doms, hs)
-    a, b = batch.make_batch_mesher(sts), batch.make_batch_mesher(sts)
+    a = batch.make_batch_mesher(sts, far_square=far_square)
+    b = batch.make_batch_mesher(sts, far_square=far_square)
     a.set_graph(False)
     b.set_graph(True)
     p0 = np.vstack([st["p"] for st in sts])

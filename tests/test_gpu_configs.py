@@ -137,14 +137,43 @@ def test_bp2004_full_size_end_to_end_vs_oracle():
             assert abs(qa.min() - qb.min()) <= 0.03
 
 
-def test_many_domains_batch_vs_oracle():
-    """Per domain, kept cells and edges equal the oracle's and updates agree
-    within 1e-5, from the points after one oracle iteration on.  (Before
-    it, the lattice's outer row lies inside curved boundaries and the
-    convex hull has flat triangles there whose centroids pass the filter:
-    the reference keeps them, a batch triangulates inside a far square of
-    100 extents and does not; from the first projection on those triangles
-    fail the filter.)"""
+def _far_square_misses(p, t_oracle, t_dev):
+    """Cells the oracle keeps and a far-square triangulation leaves out must
+    be triangles with an edge on the convex hull whose circumcircle holds a
+    far vertex (100 extents out); a far-square triangulation keeps no other
+    cell than the oracle.  Returns the number of such cells."""
+    A = {tuple(sorted(c)) for c in np.asarray(t_oracle).tolist()}
+    B = {tuple(sorted(c)) for c in np.asarray(t_dev).tolist()}
+    assert B <= A, "%d cells only in the far-square triangulation" % len(B - A)
+    lo, hi = p.min(0), p.max(0)
+    ext, c = (hi - lo).max(), (lo + hi) / 2
+    far = c + 100 * ext * np.array([[-1, -1], [1, -1], [1, 1], [-1, 1]])
+    from scipy.spatial import ConvexHull
+    hull = {tuple(sorted(e)) for e in ConvexHull(p).simplices.tolist()}
+    for cell in A - B:
+        assert any(tuple(sorted((cell[i], cell[j]))) in hull for i, j in ((0, 1), (1, 2), (0, 2))), \
+            "left-out cell %s has no hull edge" % (cell,)
+        a, b, q = (p[i] for i in cell)
+        # circumcentre
+        d = 2 * (a[0] * (b[1] - q[1]) + b[0] * (q[1] - a[1]) + q[0] * (a[1] - b[1]))
+        ux = ((a @ a) * (b[1] - q[1]) + (b @ b) * (q[1] - a[1]) + (q @ q) * (a[1] - b[1])) / d
+        uy = ((a @ a) * (q[0] - b[0]) + (b @ b) * (a[0] - q[0]) + (q @ q) * (b[0] - a[0])) / d
+        r2 = (a[0] - ux) ** 2 + (a[1] - uy) ** 2
+        assert (((far - [ux, uy]) ** 2).sum(1) < r2).any(), \
+            "left-out cell %s: no far vertex in its circle" % (cell,)
+    return len(A - B)
+
+
+@pytest.mark.parametrize("far_square", [False, True], ids=["hull", "far_square"])
+def test_many_domains_batch_vs_oracle(far_square):
+    """Per domain, from jittered initial lattices: over each domain's convex
+    hull (the default, as the reference) kept cells and edges equal the
+    oracle's every iteration; inside a far square the kept cells are the
+    oracle's but for hull triangles whose circumcircle reaches a far vertex
+    (flat triangles of nearly collinear outer-row points, whose centroids
+    can pass the filter before the boundary rows are projected).  Updates
+    agree within 1e-5 of the oracle's step from the same points whenever
+    the cells agree."""
     from oracle import distmesh as odm
     from oracle import geometry as og
     from seismicmesh_amd import batch, workloads
@@ -152,28 +181,38 @@ def test_many_domains_batch_vs_oracle():
     odoms, ohs = workloads.many_domains(n=6, nmax=2.0e4, geom=og)
     assert hs == ohs
     sts = batch.prepare_batch(doms, hs)
-    for k, (st, od) in enumerate(zip(sts, odoms)):
-        p0 = _jitter(st["p"], st["h0"], 100 + k)
-        s = _oracle_state(od, hs[k], p0)
-        st["p"], _, _ = odm.force_step(p0, odm.delaunay(p0), s, list(range(st["nfix"])))
-    bm = batch.make_batch_mesher(sts)
+    for k, st in enumerate(sts):
+        st["p"] = _jitter(st["p"], st["h0"], 100 + k)
+    bm = batch.make_batch_mesher(sts, far_square=far_square)
     p0 = np.vstack([st["p"] for st in sts])
     bm.set_points(p0)
     ps = [st["p"] for st in sts]
-    for it in range(2):
+    missed = 0
+    for it in range(3):
         bm.triangulate()
         parts = bm.split(bm.points(), bm.cells())
         be = bm.edges()
+        same = []
         for d, (st, od) in enumerate(zip(sts, odoms)):
             s = _oracle_state(od, hs[d], ps[d])
             t = odm.remove_triangles_outside(ps[d], odm.delaunay(ps[d]), s["fd"], s["geps"])
+            if far_square:
+                k = _far_square_misses(ps[d], t, parts[d][1])
+                missed += k
+                same.append(k == 0)
+                if k:
+                    continue
             _same_cells(parts[d][1], t, "domain %d, iteration %d" % (d, it))
             b, e = bm.voff[d], bm.voff[d + 1]
             mine = be[(be[:, 0] >= b) & (be[:, 0] < e)] - b
             _same_edges(mine, odm.get_edges(t), "domain %d, iteration %d" % (d, it))
+            same.append(True)
         bm.run(1)
         pb = bm.points()
         for d, (st, od) in enumerate(zip(sts, odoms)):
+            if not same[d]:
+                ps[d] = pb[bm.voff[d]:bm.voff[d + 1]].copy()
+                continue
             s = _oracle_state(od, hs[d], ps[d])
             q_ref, _, _ = odm.force_step(ps[d], odm.delaunay(ps[d]), s, list(range(st["nfix"])))
             b, e = bm.voff[d], bm.voff[d + 1]
@@ -185,6 +224,7 @@ def test_many_domains_batch_vs_oracle():
                   % (d, len(ps[d]), it, err))
             assert err <= 1e-5, "domain %d, iteration %d" % (d, it)
             ps[d] = pb[b:e].copy()
+    print("far square: %s, hull cells left out %d" % (far_square, missed))
     bm.close()
 
 

@@ -188,6 +188,44 @@ def test_cell_edges_bit_exact(dev):
         assert np.array_equal(got, odm.get_edges(t))
 
 
+def test_tet_edges_in_every_cell_order(dev):
+    """Tetrahedra count their edges by runs of equal lowest id (one atomic
+    per run of lanes): cells sorted by lowest id (the host
+    tetrahedralisation's order, long runs), reversed, shuffled, with their
+    vertices permuted, and with half of them masked out of a 3D handle's
+    keep flags all give the reference's sorted unique edges."""
+    import ctypes
+    import torch
+    from oracle import distmesh as odm
+    from seismicmesh_amd import _lib
+    rng = np.random.default_rng(8)
+    p = np.ascontiguousarray(rng.random((30000, 3)))
+    cap = 8 * len(p) + 64
+    tet = np.zeros((cap, 4), np.int32)
+    ins = np.zeros(len(p), np.uint8)
+    m = _lib.lib.sm_host_delaunay3d(p.ctypes.data_as(ctypes.c_void_p), len(p),
+                                    tet.ctypes.data_as(ctypes.c_void_p), cap,
+                                    ins.ctypes.data_as(ctypes.c_void_p))
+    assert m > 0
+    t0 = tet[:m]
+    assert (np.diff(t0[:, 0]) >= 0).all()  # sorted by lowest id
+    orders = {"sorted": t0, "reversed": t0[::-1], "shuffled": t0[rng.permutation(m)],
+              "permuted vertices": t0[:, rng.permutation(4)],
+              "half": t0[rng.random(m) < 0.5]}
+    for name, t in orders.items():
+        t = np.ascontiguousarray(t, dtype=np.int32)
+        tt = _t(t, dev)
+        out = torch.empty((6 * len(t), 2), dtype=torch.int32, device=dev)
+        cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+        wsb = _lib.lib.sm_cell_edges_workspace(len(t), 3, len(p))
+        ws = torch.empty(wsb, dtype=torch.uint8, device=dev)
+        _lib.check(_lib.lib.sm_cell_edges(tt.data_ptr(), len(t), 3, len(p), out.data_ptr(),
+                                          cnt.data_ptr(), ws.data_ptr(), wsb, None), "cell_edges")
+        torch.cuda.synchronize()
+        got = out[:int(cnt.item())].cpu().numpy()
+        assert np.array_equal(got, odm.get_edges(t)), name
+
+
 def test_dihedral_and_circumsphere_grad(dev):
     import torch
     from oracle import geometry as og
