@@ -224,6 +224,18 @@ size_t sm_cell_edges_workspace(int64_t ncells, int dim, int64_t nvert);
 int sm_cell_edges(const int32_t *cells, int64_t ncells, int dim, int64_t nvert,
                   int32_t *edges, int64_t *count, void *ws, size_t ws_bytes, void *stream);
 
+/* sm_unique_edges without a workspace argument, allocating nothing: no
+ * counterpart in the reference; for callers that cannot allocate device
+ * memory (HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The workspace is a library-owned device array of
+ * SM_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), shared by
+ * every call on that GPU: calls that overlap on one GPU race.  Inputs whose
+ * sm_unique_edges_workspace exceeds it are refused with SM_ERR_CAPACITY
+ * before anything is launched. */
+#define SM_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int sm_unique_edges_noalloc(const int32_t *pairs, int64_t npairs, int64_t nvert,
+                            int32_t *edges, int64_t *count, void *stream);
+
 /* ----------------------------------------------------------- the loop
  * A 2D/3D force-loop handle owning device state for one domain:
  * points (FP64), triangulation, edges, per-vertex edge lists, forces.

@@ -28,6 +28,8 @@ lib = ctypes.CDLL(LIB_PATH)
 
 SM_OK = 0
 SM_ERR_NUMERIC = -5
+# device scratch of sm_unique_edges_noalloc per GPU (SM_NOALLOC_SCRATCH_BYTES)
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 # SDF program opcodes and transform flags (smgpu.h)
 OP_DISK, OP_RECT, OP_BALL, OP_CUBE, OP_TORUS, OP_PRISM, OP_CYLINDER = 1, 2, 3, 4, 5, 6, 7
@@ -101,6 +103,7 @@ _PROTOS = {
     "sm_sizing_eval": (_int, [ctypes.POINTER(Grid), _vp, _i64, _vp, _vp]),
     "sm_unique_edges_workspace": (_sz, [_i64, _i64]),
     "sm_unique_edges": (_int, [_vp, _i64, _i64, _vp, _vp, _vp, _sz, _vp]),
+    "sm_unique_edges_noalloc": (_int, [_vp, _i64, _i64, _vp, _vp, _vp]),
     "sm_cell_edges_workspace": (_sz, [_i64, _int, _i64]),
     "sm_cell_edges": (_int, [_vp, _i64, _int, _i64, _vp, _vp, _vp, _sz, _vp]),
     "sm_mesher_create": (_int, [ctypes.POINTER(MesherParams), _vp, _vp, ctypes.POINTER(Grid),
