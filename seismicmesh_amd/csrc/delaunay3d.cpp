@@ -56,6 +56,9 @@
 #include <algorithm>
 #include <array>
 #include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <condition_variable>
 #include <cstdint>
@@ -146,6 +149,21 @@ inline bool lex_greater(const double *a, const double *b) {
   if (a[0] != b[0]) return a[0] > b[0];
   if (a[1] != b[1]) return a[1] > b[1];
   return a[2] > b[2];
+}
+
+// SMGPU_TRACE_DT3=1 prints each build's phase times (stderr)
+bool trace_dt3() {
+  static const bool on = [] {
+    const char *e = std::getenv("SMGPU_TRACE_DT3");
+    return e && *e && *e != '0';
+  }();
+  return on;
+}
+
+double now_ms() {
+  return std::chrono::duration<double, std::milli>(
+             std::chrono::steady_clock::now().time_since_epoch())
+      .count();
 }
 
 }  // namespace
@@ -493,6 +511,8 @@ struct Builder3 {
     if (mark.size() < 64) mark.assign(64, 0);
     // Hilbert keys (30 bits) on the threads, then a stable two-pass radix
     // sort: the order of std::sort on (key, id)
+    for (double &x : ph) x = 0.0;
+    double t0 = now_ms();
     key.resize(n);
     const int kt = std::max(1, std::min(nthreads, n / 32768 + 1));
     on_threads(kt, [&](int w) {
@@ -520,6 +540,7 @@ struct Builder3 {
         key.swap(tmp);
       }
     }
+    ph[0] = now_ms() - t0;
     const size_t k0 = start(key);
     if (k0 == 0) {
       T.clear();
@@ -559,6 +580,7 @@ struct Builder3 {
   std::vector<int64_t> slab_sizes;  // points per slab of the last parallel build
   std::vector<int> vtet;            // a tetrahedron made when each point went in
   int64_t retried = 0;              // points left to the single-thread pass
+  double ph[8] = {};                // phase times of the last build (ms), traced
   struct TCtx {
     uint32_t me = 0;
     std::vector<int> held, cav, stack, ids, freel, retry, recent;
@@ -837,6 +859,7 @@ struct Builder3 {
       xlo = std::min(xlo, pt(order[k].second)[axis]);
       xhi = std::max(xhi, pt(order[k].second)[axis]);
     }
+    double t0 = now_ms();
     for (int v : lvl0) {
       int rc = insert(v);
       if (rc < 0) return rc;
@@ -846,6 +869,8 @@ struct Builder3 {
       if (rc < 0) return rc;
     }
     deferred.clear();
+    ph[1] = now_ms() - t0;
+    t0 = now_ms();
     // fixed capacity, fresh slots left unwritten until handed out; the
     // dead slots of the serial phase go to thread 0
     cap_par = std::max<int64_t>((int64_t)T.size() + 64, 8 * (int64_t)nloc + 4096);
@@ -875,8 +900,14 @@ struct Builder3 {
       ctx[w].recent.push_back(last);
     }
     slab_sizes.assign((size_t)std::max(1, nslabs), 0);
+    ph[2] = now_ms() - t0;
+    t0 = now_ms();
     int rc = concurrent_pass(lvl1, pred1, ctx, axis, nslabs, xlo, xhi);
+    ph[3] = now_ms() - t0;
+    t0 = now_ms();
     if (rc == 0) rc = concurrent_pass(rest, pred2, ctx, axis, nslabs, xlo, xhi);
+    ph[4] = now_ms() - t0;
+    t0 = now_ms();
     // points that met taken locks twice: more concurrent passes while there
     // are many of them (contention is a matter of timing), then one thread
     retried = 0;
@@ -901,6 +932,8 @@ struct Builder3 {
           break;
         }
       }
+    ph[5] = now_ms() - t0;
+    t0 = now_ms();
     // slots handed out in blocks but never used were never written
     for (auto &c : ctx) {
       for (int64_t t = c.blk; t < c.blk_end; ++t) T[(size_t)t].v[0] = kDead;
@@ -922,6 +955,7 @@ struct Builder3 {
         last = (int)t;
         break;
       }
+    ph[6] = now_ms() - t0;
     return rc;
   }
 
@@ -1005,6 +1039,7 @@ int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, con
     return sm_set_error(SM_ERR_ARG, "delaunay3d: bad arguments");
   if (n > (int64_t)std::numeric_limits<int32_t>::max() / 8)
     return sm_set_error(SM_ERR_ARG, "delaunay3d: too many points");
+  const double r0 = now_ms();
   stats = Stats{};
   inserted.assign((size_t)n, 1);
   Builder3 &b = *B;
@@ -1040,74 +1075,93 @@ int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, con
   // slots, then put in increasing order of (lowest id, the rest): a
   // counting sort on the lowest id and a sort within each of its short
   // buckets -- the order is then that of the serial build too
+  const double t_ex = now_ms();
   const int64_t nslot = (int64_t)b.T.size();
   for (int f : b.free_slots) b.T[f].v[0] = Builder3::kDead;  // free slots: stale contents
   const int nthr = std::max(1, std::min(nthreads, (int)(nslot / 65536 + 1)));
-  // thread w extracts slots [nslot w / nthr, ...) into part[w][r], r the
-  // range of lowest ids [n r / nthr, n (r + 1) / nthr) the cell belongs to
+  // Thread w owns slots [nslot w / nthr, ...) for the extraction and the
+  // range of lowest ids [n r / nthr, n (r + 1) / nthr) with r = w for the
+  // sort.  Pass 1 counts each thread's cells per range, pass 2 writes them
+  // canonical into the staging array grouped by range (a range's group
+  // spans exactly the positions its cells take in the output), pass 3
+  // counting-sorts each group by lowest id into the output and sorts every
+  // bucket.
   auto lo_id = [&](int w) { return n * w / nthr; };
-  std::vector<std::vector<std::vector<int32_t>>> part(
-      (size_t)nthr, std::vector<std::vector<int32_t>>((size_t)nthr));
+  auto range_of = [&](int32_t v0) {
+    int rg = (int)((int64_t)v0 * nthr / std::max<int64_t>(n, 1));
+    while (rg + 1 < nthr && lo_id(rg + 1) <= v0) ++rg;
+    while (rg > 0 && lo_id(rg) > v0) --rg;
+    return rg;
+  };
+  auto live = [&](const Tet &r) { return r.v[0] >= 0 && Builder3::inf_slot(r) < 0; };
+  std::vector<int64_t> cnt((size_t)nthr * nthr, 0);
   Builder3::on_threads(nthr, [&](int w) {
     const int64_t s0 = nslot * w / nthr, s1 = nslot * (w + 1) / nthr;
-    for (auto &o : part[w]) o.clear();
+    int64_t *c = cnt.data() + (size_t)w * nthr;
     for (int64_t t = s0; t < s1; ++t) {
       const Tet &r = b.T[t];
-      if (r.v[0] < 0 || Builder3::inf_slot(r) >= 0) continue;  // dead or infinite
-      int32_t v[4];
-      for (int j = 0; j < 4; ++j) v[j] = (int32_t)b.gid[r.v[j]];
-      canonical_tet(v);
-      int rg = (int)((int64_t)v[0] * nthr / std::max<int64_t>(n, 1));
-      while (rg + 1 < nthr && lo_id(rg + 1) <= v[0]) ++rg;
-      while (rg > 0 && lo_id(rg) > v[0]) --rg;
-      part[w][rg].insert(part[w][rg].end(), v, v + 4);
+      if (!live(r)) continue;  // dead or infinite
+      int64_t lo = b.gid[r.v[0]];
+      for (int j = 1; j < 4; ++j) lo = std::min(lo, b.gid[r.v[j]]);
+      ++c[range_of((int32_t)lo)];
     }
   });
-  int64_t m = 0;
-  for (auto &pw : part)
-    for (auto &v : pw) m += (int64_t)v.size() / 4;
-  // each thread owns the buckets of its range of lowest ids: it counts
-  // them, then (after one scan of the range totals) places and sorts them
-  start_.assign((size_t)n + 1, 0);
+  // below[r]: cells with lowest id below range r; pos[w][r]: where thread
+  // w's cells of range r start
+  std::vector<int64_t> below((size_t)nthr + 1, 0), pos((size_t)nthr * nthr);
+  for (int r = 0; r < nthr; ++r) {
+    int64_t o = below[(size_t)r];
+    for (int w = 0; w < nthr; ++w) {
+      pos[(size_t)w * nthr + r] = o;
+      o += cnt[(size_t)w * nthr + r];
+    }
+    below[(size_t)r + 1] = o;
+  }
+  const int64_t m = below[(size_t)nthr];
+  stage_.resize((size_t)m * 4);
   int32_t *dst = out;
   if (!out || out_cap < m) {
     sorted_.resize((size_t)m * 4);
     dst = sorted_.data();
   }
-  std::vector<int64_t> below((size_t)nthr + 1, 0);
   Builder3::on_threads(nthr, [&](int w) {
-    const int64_t a = lo_id(w), b = lo_id(w + 1);
-    int64_t s = 0;
-    for (auto &pw : part)
-      for (size_t i = 0; i < pw[w].size(); i += 4) ++start_[(size_t)pw[w][i] + 1];
-    for (int64_t i = a; i < b; ++i) s += start_[(size_t)i + 1];
-    below[(size_t)w + 1] = s;
+    const int64_t s0 = nslot * w / nthr, s1 = nslot * (w + 1) / nthr;
+    int64_t *p = pos.data() + (size_t)w * nthr;
+    for (int64_t t = s0; t < s1; ++t) {
+      const Tet &r = b.T[t];
+      if (!live(r)) continue;
+      int32_t v[4];
+      for (int j = 0; j < 4; ++j) v[j] = (int32_t)b.gid[r.v[j]];
+      canonical_tet(v);
+      const int64_t o = p[range_of(v[0])]++;
+      std::memcpy(&stage_[(size_t)o * 4], v, sizeof(v));
+    }
   });
-  for (int w = 0; w < nthr; ++w) below[(size_t)w + 1] += below[(size_t)w];
+  start_.assign((size_t)n + 1, 0);
   auto *cells = reinterpret_cast<std::array<int32_t, 4> *>(dst);
   Builder3::on_threads(nthr, [&](int w) {
-    // start_[i + 1] holds the count of bucket i; this thread writes
-    // start_[a + 1 .. b] only
-    const int64_t a = lo_id(w), b = lo_id(w + 1);
-    std::vector<int64_t> fill((size_t)(b - a));
-    int64_t run = below[(size_t)w];
-    for (int64_t i = a; i < b; ++i) {
+    // start_[i + 1] ends bucket i; this thread writes start_[a + 1 .. b] only
+    const int64_t a = lo_id(w), bb = lo_id(w + 1), g0 = below[(size_t)w], g1 = below[(size_t)w + 1];
+    std::vector<int64_t> fill((size_t)(bb - a), 0);
+    for (int64_t k = g0; k < g1; ++k) ++fill[(size_t)(stage_[(size_t)k * 4] - a)];
+    int64_t run = g0;
+    for (int64_t i = a; i < bb; ++i) {
+      const int64_t c = fill[(size_t)(i - a)];
       fill[(size_t)(i - a)] = run;
-      run += start_[(size_t)i + 1];
+      run += c;
       start_[(size_t)i + 1] = run;
     }
-    for (auto &pw : part) {
-      const std::vector<int32_t> &v = pw[w];
-      for (size_t i = 0; i < v.size(); i += 4) {
-        const int64_t o = fill[(size_t)(v[i] - a)]++;
-        for (int j = 0; j < 4; ++j) dst[4 * o + j] = v[i + j];
-      }
+    for (int64_t k = g0; k < g1; ++k) {
+      const int32_t *v = &stage_[(size_t)k * 4];
+      const int64_t o = fill[(size_t)(v[0] - a)]++;
+      std::memcpy(dst + 4 * o, v, 4 * sizeof(int32_t));
     }
-    for (int64_t i = a; i < b; ++i) {
-      const int64_t s0 = i == a ? below[(size_t)w] : start_[(size_t)i];
-      if (start_[(size_t)i + 1] - s0 > 1) std::sort(cells + s0, cells + start_[(size_t)i + 1]);
+    for (int64_t i = a; i < bb; ++i) {
+      const int64_t c0 = i == a ? g0 : start_[(size_t)i];
+      if (start_[(size_t)i + 1] - c0 > 1) std::sort(cells + c0, cells + start_[(size_t)i + 1]);
     }
   });
+  const double r1 = now_ms();
   // handed out in nslabs contiguous pieces
   const int pieces = std::max(1, nslabs);
   for (int k = 0; k < pieces; ++k) {
@@ -1115,6 +1169,15 @@ int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, con
     emit(k, dst + 4 * c0, c1 - c0);
   }
   emit(-1, nullptr, 0);
+  if (trace_dt3()) {
+    const double *ph = b.ph;
+    std::fprintf(stderr,
+                 "dt3: %lld points, %lld cells, %d threads: keys+sort %.1f, serial %.1f, "
+                 "setup %.1f, pass 1 %.1f, pass 2 %.1f, retries %.1f (%lld), teardown %.1f, "
+                 "extract+sort %.1f, pieces %.1f, total %.1f ms\n",
+                 (long long)n, (long long)m, nthreads, ph[0], ph[1], ph[2], ph[3], ph[4], ph[5],
+                 (long long)b.retried, ph[6], r1 - t_ex, now_ms() - r1, now_ms() - r0);
+  }
   return SM_OK;
 }
 

@@ -58,7 +58,7 @@ struct DT3 {
 
   std::unique_ptr<Builder3> B;
   std::vector<int64_t> start_;
-  std::vector<int32_t> sorted_;
+  std::vector<int32_t> sorted_, stage_;
   // points binned on a uniform grid (verify)
   double blo_[3] = {0, 0, 0}, bhi_[3] = {0, 0, 0};
   int64_t nb_[3] = {1, 1, 1};
