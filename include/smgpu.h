@@ -325,6 +325,10 @@ int sm_mesher_sizes(sm_mesher *m, int64_t *npoints, int64_t *ncells, int64_t *ne
 int sm_mesher_get_points(sm_mesher *m, double *p, int64_t cap);
 int sm_mesher_get_cells(sm_mesher *m, int32_t *cells, int64_t cap);
 int sm_mesher_get_edges(sm_mesher *m, int32_t *edges, int64_t cap);
+/* 2D: the whole triangulation the handle keeps (real and ghost triangles,
+ * ghost vertex -1, dead slots -2, neighbour j opposite vertex j); returns
+ * the slot count or an error code.  For checking the device repair. */
+int64_t sm_mesher_get_triangulation(sm_mesher *m, int32_t *tri, int32_t *adj, int64_t cap);
 /* Statistics since creation: [0] iterations, [1] flip rounds, [2] flips,
  * [3] host rebuilds (every 3D triangulation is one), [4] exact predicate
  * evaluations, [5] speculative (graph) iterations redone because their flips

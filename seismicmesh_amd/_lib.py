@@ -120,6 +120,7 @@ _PROTOS = {
     "sm_mesher_get_points": (_int, [_vp, _vp, _i64]),
     "sm_mesher_get_cells": (_int, [_vp, _vp, _i64]),
     "sm_mesher_get_edges": (_int, [_vp, _vp, _i64]),
+    "sm_mesher_get_triangulation": (_i64, [_vp, _vp, _vp, _i64]),
     "sm_mesher_stats": (_int, [_vp, _vp, _int]),
     "sm_mesher_set_slabs": (_int, [_vp, _int, _int, _int]),
     "sm_mesher_set_graph": (_int, [_vp, _int]),

@@ -144,6 +144,18 @@ __global__ void k_fix_orient(int64_t T, const int32_t *tri, int32_t *adj,
   if (skip) return;
   claim[t] = 0ull;
   int a = tri[3 * t], b = tri[3 * t + 1], cc = tri[3 * t + 2];
+  // Two triangles sharing two edges (a vertex left with only two ghosts
+  // glued around it, after flips next to collinear hull points) make the
+  // triangulation invalid although no triangle is inverted: counted as an
+  // inverted triangle, so that the repair fails and the exact rebuild takes
+  // over.
+  if (a != kDeadVertex) {
+    const int32_t n0 = adj[3 * t], n1 = adj[3 * t + 1], n2 = adj[3 * t + 2];
+    if (n0 == n1 || n1 == n2 || n0 == n2) {
+      atomicAdd(&c[1], 1);
+      if (av.act) av.act[(2 + par) * av.ndom + av.tdom[t]] = 1;
+    }
+  }
   if (a < 0 || b < 0 || cc < 0) {
     ori[t] = kGhostOri;
     return;
@@ -617,21 +629,42 @@ int time_flip_mark(sm_mesher *m, int reps, double *ms) {
   SM_CHECK(reps >= 1 && reps <= 100000, SM_ERR_ARG, "time_kernel: bad repetition count");
   hipStream_t s = m->stream;
   if (!m->d_tctr) SM_HIP(hipMalloc((void **)&m->d_tctr, 8 * sizeof(int32_t)));
-  // every launch starts from the counters of an open round (not
-  // finished, round id 0, no inverted or uncertain triangle: the loop's
-  // usual state, in which k_mark does not read orientations), cleared
-  // before its event pair -- an uncertain incircle test counted by one
-  // launch would otherwise switch every later one to the tangled path,
-  // which also loads the orientations.  The launches write claims and
-  // candidates, which every flip round recomputes before use.
+  // The launches start from the counters of an open round (not finished,
+  // round id 0, no inverted or uncertain triangle: the loop's usual state,
+  // in which k_mark does not read orientations) and run back to back.  An
+  // uncertain incircle test counted by one launch would switch every later
+  // one to the tangled path, which also loads the orientations: then every
+  // launch is timed alone from a cleared counter block (its event pair adds
+  // a few microseconds).  The launches write claims and candidates, which
+  // every flip round recomputes before use.
   const int64_t T = m->T;
   const int B = m->blk[kBlkMark];
   const unsigned g = blocks_for(T, B);
   hipEvent_t e0, e1;
   SM_HIP(hipEventCreate(&e0));
   SM_HIP(hipEventCreate(&e1));
+  int rc = fill<int32_t>(m->d_tctr, 8, 0, s);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, m->d_p,
+                     m->d_claim, m->d_cand, m->d_queue + T, m->d_tctr, 0, ActView(), 0);
+  SM_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; ++i)
+    hipLaunchKernelGGL(k_mark, dim3(g), dim3(B), 0, s, T, m->d_tri, m->d_adj, m->d_ori, m->d_p,
+                       m->d_claim, m->d_cand, m->d_queue + T, m->d_tctr, 0, ActView(), 0);
+  SM_HIP(hipEventRecord(e1, s));
+  SM_LAUNCH_CHECK();
+  SM_HIP(hipEventSynchronize(e1));
+  float back = 0.0f;
+  SM_HIP(hipEventElapsedTime(&back, e0, e1));
+  int32_t tc[8];
+  SM_HIP(hipMemcpy(tc, m->d_tctr, sizeof(tc), hipMemcpyDeviceToHost));
+  if (tc[5] == 0) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = (double)back / reps;
+    return SM_OK;
+  }
   double total = 0.0;
-  int rc = SM_OK;
   for (int i = 0; i <= reps && rc == SM_OK; ++i) {
     if ((rc = fill<int32_t>(m->d_tctr, 8, 0, s))) break;
     if (hipEventRecord(e0, s) != hipSuccess) rc = sm_set_error(SM_ERR_HIP, "time_kernel: event");

@@ -177,10 +177,13 @@ int host_sizing_pass(sm_mesher *m) {
 // 3D: the tetrahedralisation is rebuilt on the host from the current points
 // every iteration (as the reference does with CGAL), on sl_threads threads
 // inserting the points of sl_n slabs into one shared tetrahedralisation.
-// Its cells are handed out in pieces; each piece goes to the device (pinned
-// staging, asynchronous copy) and through the centroid filter while the
-// host prepares the next; *filtered tells whether the filter ran that way
-// (device programs) or is still to run (host callbacks).
+// Its cells are sorted straight into pinned staging and handed out in
+// pieces, each copied to the device asynchronously as it comes; the
+// centroid filter then runs once over all of them (one launch keeps the
+// GPU full: the SDF program is long per cell, and a piece's launch of a
+// few tens of thousands of cells leaves most SIMDs idle).  *filtered tells
+// whether the filter ran that way (device programs) or is still to run
+// (host callbacks).
 int host_rebuild3d(sm_mesher *m, bool *filtered) {
   const auto t0 = std::chrono::steady_clock::now();
   const int64_t N = m->N;
@@ -207,7 +210,18 @@ int host_rebuild3d(sm_mesher *m, bool *filtered) {
   int err = SM_OK;
   bool over = false, all_filtered = true;
   auto emit = [&](int piece, const int32_t *v, int64_t count) {
-    if (err || over || count == 0) return;
+    if (err || over) return;
+    if (piece < 0) {
+      // all pieces are on their way: the points do not move before the
+      // filter, so it may run now
+      bool done = false;
+      int b = prof_begin(m);
+      err = centroid_keep_range(m, 0, off, &done);
+      prof_end(m, 1, b);
+      all_filtered = done;
+      return;
+    }
+    if (count == 0) return;
     if (off + count > m->capT) {
       over = true;
       return;
@@ -220,12 +234,6 @@ int host_rebuild3d(sm_mesher *m, bool *filtered) {
       err = sm_set_error(SM_ERR_HIP, "host_rebuild3d: copy of piece %d failed", piece);
       return;
     }
-    // the points do not move before the filter, so it may run now
-    bool done = false;
-    int b = prof_begin(m);
-    err = centroid_keep_range(m, off, count, &done);
-    prof_end(m, 1, b);
-    all_filtered &= done;
     off += count;
   };
   // the cells are sorted straight into the pinned staging buffer
@@ -793,6 +801,19 @@ extern "C" int sm_mesher_get_cells(sm_mesher *m, int32_t *cells, int64_t cap) {
     ++c;
   }
   return SM_OK;
+}
+
+extern "C" int64_t sm_mesher_get_triangulation(sm_mesher *m, int32_t *tri, int32_t *adj,
+                                               int64_t cap) {
+  SM_CHECK(m && tri && adj, SM_ERR_ARG, "mesher_get_triangulation: null argument");
+  SM_CHECK(m->dim == 2 && m->have_tri, SM_ERR_STATE,
+           "mesher_get_triangulation: needs a 2D handle with a triangulation");
+  SM_CHECK(cap >= m->T, SM_ERR_CAPACITY, "mesher_get_triangulation: buffer too small");
+  SM_HIP(hipSetDevice(m->device));
+  SM_HIP(hipStreamSynchronize(m->stream));
+  SM_HIP(hipMemcpy(tri, m->d_tri, m->T * 3 * sizeof(int32_t), hipMemcpyDeviceToHost));
+  SM_HIP(hipMemcpy(adj, m->d_adj, m->T * 3 * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return m->T;
 }
 
 extern "C" int sm_mesher_get_edges(sm_mesher *m, int32_t *edges, int64_t cap) {

@@ -249,6 +249,16 @@ class Mesher:
         _lib.check(_lib.lib.sm_mesher_get_cells(self.h, t.ctypes.data, c), "sm_mesher_get_cells")
         return t
 
+    def triangulation(self):
+        """2D: (triangles, neighbours) of every slot, ghosts (-1) and dead
+        slots (-2) included."""
+        cap = 2 * (len(self.points()) + 4 * 65536) + 64
+        tri = np.empty((cap, 3), dtype=np.int32)
+        adj = np.empty((cap, 3), dtype=np.int32)
+        n = _lib.lib.sm_mesher_get_triangulation(self.h, tri.ctypes.data, adj.ctypes.data, cap)
+        _lib.check(int(n) if n < 0 else 0, "sm_mesher_get_triangulation")
+        return tri[:n].copy(), adj[:n].copy()
+
     def edges(self):
         _, _, e = self.sizes()
         out = np.empty((e, 2), dtype=np.int32)

@@ -374,7 +374,10 @@ class NeuralSDF:
         over validation points (half uniform in the box, half within `near`
         of the surface) in each cell of a res^dim grid over the box and its
         neighbouring cells, times `safety`, at least `band` (e.g. 2 h); a
-        cell without validation points gets the largest error overall.
+        cell without validation points gets the largest error overall.  For
+        a network trained on a truncated SDF both values are clipped to
+        +-trunc first, and a band of trunc or more becomes 1e30 (never
+        skipped: the network's output cannot get past it).
         Returns (bands (res,)*dim float32, lo, hi, largest error)."""
         if near is None:
             near = band
@@ -400,8 +403,12 @@ class NeuralSDF:
             sl = tuple(slice(o, o + res) for o in off)
             out = np.maximum(out, pad[sl])
         bands = np.maximum(safety * out, band).astype(np.float32)
-        if self.trunc and float(bands.max()) >= self.trunc:
-            bands = np.minimum(bands, np.float32(0.999 * self.trunc))
+        if self.trunc:
+            # a network trained on an SDF truncated at trunc never leaves
+            # [-trunc, trunc]: where the band reaches trunc no decision is
+            # safe there, and the band is made unreachable (the exact
+            # program decides every evaluation in that region)
+            bands[bands >= np.float32(self.trunc)] = np.float32(1e30)
         return bands, lo, hi, emax
 
     def max_error(self, domain, x):

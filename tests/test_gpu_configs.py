@@ -144,6 +144,54 @@ def test_bp2004_full_size_end_to_end_vs_oracle():
             assert abs(qa.min() - qb.min()) <= 0.03
 
 
+def _check_sphere(tri, adj, npts):
+    """A valid triangulated sphere: every point in a real triangle, links
+    mutual, no two triangles sharing two edges."""
+    live = tri[:, 0] != -2
+    idx = np.nonzero(live)[0]
+    t, a = tri[idx], adj[idx]
+    real = t[(t >= 0).all(1)]
+    present = np.zeros(npts, bool)
+    present[real.ravel()] = True
+    assert present.all(), "points in no real triangle: %s" % np.nonzero(~present)[0][:10]
+    assert ((a[:, 0] != a[:, 1]) & (a[:, 1] != a[:, 2]) & (a[:, 0] != a[:, 2])).all(), \
+        "two triangles share two edges"
+    for k in range(3):
+        nb = a[:, k]
+        assert (adj[nb] == idx[:, None]).any(1).all(), "links not mutual"
+
+
+def test_bp2004_device_repair_keeps_collinear_hull_points():
+    """BP2004, iteration 1 from jittered points with the fixed corners in
+    place: boundary points projected onto the box's sides lie exactly on the
+    line through a fixed corner, and the device repair must keep that
+    corner in the triangulation (collinear hull points are all vertices, as
+    in the reference's CGAL triangulation): a valid triangulated sphere
+    whose kept cells equal the oracle's."""
+    from oracle import distmesh as odm
+    from oracle import geometry as og
+    from oracle.sizing import SizeFunction, build_interpolant
+    from seismicmesh_amd import generation, workloads
+    domain, ef, info = workloads.bp2004()
+    st = generation.prepare(domain, ef, verbose=0)
+    p0 = _jitter(st["p"], st["h0"], 7, st["nfix"])
+    st["p"] = p0
+    m = generation.make_mesher(st)
+    m.set_points(p0)
+    m.run(1)
+    m.triangulate()
+    p1 = m.points()
+    tri, adj = m.triangulation()
+    _check_sphere(tri, adj, len(p1))
+    grid = ef.cell_size
+    oef = SizeFunction(info["bbox"], build_interpolant(grid.values.astype(np.float64), grid.bbox),
+                       ef.hmin)
+    s = _oracle_state(og.Rectangle(info["bbox"]), oef, p0)
+    t = odm.remove_triangles_outside(p1, odm.delaunay(p1), s["fd"], s["geps"])
+    _same_cells(m.cells(), t, "BP2004 iteration 1")
+    m.close()
+
+
 def _far_square_misses(p, t_oracle, t_dev):
     """Cells the oracle keeps and a far-square triangulation leaves out must
     be triangles with an edge on the convex hull whose circumcircle holds a
