@@ -984,53 +984,68 @@ struct Builder3 {
       for (int k = 0; k < S; ++k) slab_sizes[k] += (int64_t)slab[k].size();
     const int nthr = (int)ctx.size();
     const size_t chunk = std::max<size_t>(256, pts.size() / ((size_t)nthr * 8) + 1);
-    std::vector<std::array<size_t, 3>> chunks;
-    for (size_t r = 0;; ++r) {
-      bool any = false;
-      for (int k = 0; k < S; ++k)
-        if (r * chunk < slab[k].size()) {
-          chunks.push_back({(size_t)k, r * chunk, std::min(slab[k].size(), (r + 1) * chunk)});
-          any = true;
-        }
-      if (!any) break;
-    }
-    const size_t nchunks = chunks.size();
-    std::atomic<size_t> next{0};
+    // each slab's chunks in curve order, taken by one atomic cursor per
+    // slab; thread w works through the slabs w, w + nthr, ... first (it
+    // stays where its caches are warm) and then takes chunks from the
+    // others
+    std::vector<size_t> nch((size_t)S);
+    for (int k = 0; k < S; ++k) nch[(size_t)k] = (slab[(size_t)k].size() + chunk - 1) / chunk;
+    std::unique_ptr<std::atomic<size_t>[]> cursor(new std::atomic<size_t>[(size_t)S]);
+    for (int k = 0; k < S; ++k) cursor[(size_t)k].store(0);
     std::atomic<int> fail{0};
     on_threads(nthr, [&](int w) {
       TCtx &c = ctx[w];
       std::vector<int> again;
-      for (;;) {
-        const size_t ch = next.fetch_add(1);
-        if (ch >= nchunks || fail.load(std::memory_order_relaxed)) break;
-        const std::vector<int> &sp = slab[chunks[ch][0]];
-        // start near the chunk when that tetrahedron still lives (the
-        // previous position stays as the fallback)
-        const int hv = vtet[spred[chunks[ch][0]][chunks[ch][1]]];
-        if (hv >= 0) {
-          c.recent.push_back(c.last);
-          c.last = hv;
-        }
-        again.clear();
-        for (size_t k = chunks[ch][1]; k < chunks[ch][2]; ++k) {
-          const int rc = par_insert(c, sp[k]);
-          if (rc == 2) again.push_back(sp[k]);
-          else if (rc < 0) {
-            fail = rc;
-            return;
-          }
-        }
-        for (int v : again) {
-          const int rc = par_insert(c, v);
-          if (rc == 2) c.retry.push_back(v);
-          else if (rc < 0) {
-            fail = rc;
-            return;
+      int prev_slab = -1;
+      for (int round = 0; round < 2; ++round) {
+        for (int j = 0; j < S; ++j) {
+          // round 0: own slabs; round 1: every slab, starting next to w
+          const int k = round == 0 ? w + j * nthr : (w + 1 + j) % S;
+          if (round == 0 && k >= S) break;
+          for (;;) {
+            if (fail.load(std::memory_order_relaxed)) return;
+            const size_t r = cursor[(size_t)k].fetch_add(1);
+            if (r >= nch[(size_t)k]) break;
+            const size_t b = r * chunk, e = std::min(slab[(size_t)k].size(), b + chunk);
+            if (k != prev_slab || r == 0) {
+              // in another slab, start near the chunk when that
+              // tetrahedron still lives (the previous position stays as
+              // the fallback); the next chunk of the same slab starts
+              // where the last one ended
+              const int hv = vtet[spred[(size_t)k][b]];
+              if (hv >= 0) {
+                c.recent.push_back(c.last);
+                c.last = hv;
+              }
+            }
+            prev_slab = k;
+            if (int rc = insert_chunk(c, slab[(size_t)k], b, e, again)) {
+              fail = rc;
+              return;
+            }
           }
         }
       }
     });
     return fail.load();
+  }
+
+  // Inserts sp[b..e) on thread context c; points that meet a taken lock go
+  // once more at the end of the chunk, then to c.retry.
+  int insert_chunk(TCtx &c, const std::vector<int> &sp, size_t b, size_t e,
+                   std::vector<int> &again) {
+    again.clear();
+    for (size_t k = b; k < e; ++k) {
+      const int rc = par_insert(c, sp[k]);
+      if (rc == 2) again.push_back(sp[k]);
+      else if (rc < 0) return rc;
+    }
+    for (int v : again) {
+      const int rc = par_insert(c, v);
+      if (rc == 2) c.retry.push_back(v);
+      else if (rc < 0) return rc;
+    }
+    return 0;
   }
 };
 

@@ -340,7 +340,11 @@ __device__ inline void long_sort_rows(unsigned b, unsigned nb, const int32_t *lo
   }
 }
 
-constexpr unsigned kLongBlocks = 64;
+// workgroups for the LDS-sized long buckets: a graded 3D mesh has
+// thousands of rows over 64 ids (vertices where the size changes), and
+// 64 workgroups working through them one bitonic sort at a time took
+// 150-400 us per launch on the EAGE-shaped workload; idle ones exit at once
+constexpr unsigned kLongBlocks = 1024;
 
 // The bucket sorts after the per-thread pass, in one launch of 256-thread
 // workgroups (the passes touch disjoint buckets): blocks [0, nwave) take
@@ -375,6 +379,34 @@ __global__ void k_emit(int64_t nrows, const int32_t *rowptr, const int32_t *col,
       edges[2 * (int64_t)o + 1] = b;
       if (second.list) second.list[second.ptr[b] + atomicAdd(&second.cur[b], 1)] = o;
       ++o;
+    }
+  }
+}
+
+// The same for the buckets of tetrahedra (dozens of sorted ids with
+// duplicates): one wave per row, 64 ids at a time, each distinct id's
+// output slot from a ballot of the first occurrences (grid-stride over the
+// rows).
+__global__ void k_emit_wave(int64_t nrows, const int32_t *rowptr, const int32_t *col,
+                            const int32_t *eptr, int32_t *edges, SecondLists second) {
+  const int lane = threadIdx.x % 64;
+  const int64_t nw = (int64_t)gridDim.x * (blockDim.x / 64);
+  for (int64_t r = ((int64_t)xcd_block() * blockDim.x + threadIdx.x) / 64; r < nrows; r += nw) {
+    const int32_t s = rowptr[r], e = rowptr[r + 1];
+    int32_t o = eptr[r];
+    for (int32_t k0 = s; k0 < e; k0 += 64) {
+      const int32_t k = k0 + lane;
+      const bool in = k < e;
+      const int32_t b = in ? col[k] : 0;
+      const bool first = in && (k == s || col[k - 1] != b);
+      const uint64_t m = __ballot(first);
+      if (first) {
+        const int32_t oo = o + (int32_t)__popcll(m & ((1ull << lane) - 1));
+        edges[2 * (int64_t)oo] = (int32_t)r;
+        edges[2 * (int64_t)oo + 1] = b;
+        if (second.list) second.list[second.ptr[b] + atomicAdd(&second.cur[b], 1)] = oo;
+      }
+      o += (int32_t)__popcll(m);
     }
   }
 }
@@ -481,7 +513,13 @@ int unique_edges_run(const int32_t *pairs, const int32_t *cells, const uint8_t *
     if (nvert == 0 && (rc = fill<int32_t>(eptr, 1, 0, s))) return rc;
     if ((rc = scan_exclusive(eptr, eptr, nvert + 1, total, sws, s))) return rc;
   }
-  if (nvert > 0) {
+  if (nvert > 0 && tets) {
+    unsigned gw = blocks_for(nvert * 64, 256);
+    if (gw > 16384) gw = 16384;
+    hipLaunchKernelGGL(k_emit_wave, dim3(gw), dim3(256), 0, s, nvert, rowptr, col,
+                       (const int32_t *)eptr, edges, sl);
+    SM_LAUNCH_CHECK();
+  } else if (nvert > 0) {
     hipLaunchKernelGGL(k_emit, dim3(blocks_for(nvert, B)), dim3(B), 0, s, nvert, rowptr, col,
                        adj ? (const int32_t *)rowptr : (const int32_t *)eptr, edges, sl);
     SM_LAUNCH_CHECK();

@@ -727,6 +727,128 @@ SM_HD inline int incircle_fix(const double *a, const double *b, const double *c,
   return fix_sign(s);
 }
 
+// 3D, same scheme: the coordinate differences of the points against the
+// last one as 62-bit integers; orient3d's determinant is below 2^190 (a
+// 256-bit sum of 64 x 128-bit products), insphere's below 2^318 (lifts
+// below 2^126 times 3x3 minors below 2^190: 384-bit sums).
+struct Fix384 {
+  uint64_t w[6];  // two's complement, least significant word first
+};
+
+// v negated in place (two's complement over n words)
+SM_HD inline void fix_negate(uint64_t *v, int n) {
+  uint64_t carry = 1;
+  for (int i = 0; i < n; ++i) {
+    const uint64_t t = ~v[i] + carry;
+    carry = (carry && t == 0) ? 1 : 0;
+    v[i] = t;
+  }
+}
+
+// x * y for x < 2^128 and a signed 256-bit y below 2^255 in magnitude
+SM_HD inline Fix384 fix_mul384(fix_u128 x, const Fix256 &y) {
+  uint64_t ym[4] = {y.w[0], y.w[1], y.w[2], y.w[3]};
+  const bool neg = (y.w[3] >> 63) != 0;
+  if (neg) fix_negate(ym, 4);
+  const uint64_t xm[2] = {(uint64_t)x, (uint64_t)(x >> 64)};
+  Fix384 r = {{0, 0, 0, 0, 0, 0}};
+  for (int i = 0; i < 2; ++i) {
+    uint64_t carry = 0;
+    for (int j = 0; j < 4; ++j) {
+      const fix_u128 t = (fix_u128)xm[i] * ym[j] + r.w[i + j] + carry;
+      r.w[i + j] = (uint64_t)t;
+      carry = (uint64_t)(t >> 64);
+    }
+    r.w[i + 4] += carry;
+  }
+  if (neg) fix_negate(r.w, 6);
+  return r;
+}
+
+template <int NW, class T>
+SM_HD inline void fix_addw(T &a, const T &b) {
+  uint64_t c = 0;
+  for (int i = 0; i < NW; ++i) {
+    uint64_t s = a.w[i] + c;
+    uint64_t c1 = s < c;
+    s += b.w[i];
+    c1 += s < b.w[i];
+    a.w[i] = s;
+    c = c1;
+  }
+}
+
+template <int NW, class T>
+SM_HD inline int fix_signw(const T &a) {
+  if (a.w[NW - 1] >> 63) return -1;
+  uint64_t o = 0;
+  for (int i = 0; i < NW; ++i) o |= a.w[i];
+  return o != 0;
+}
+
+// z * m for a 62-bit z and a 128-bit m, as a signed 256-bit integer
+SM_HD inline Fix256 fix_mul_zm(int64_t z, fix_i128 m) {
+  const fix_u128 zm = z < 0 ? (fix_u128)(-(fix_i128)z) : (fix_u128)z;
+  return fix_mul(zm, z < 0 ? -m : m);
+}
+
+// the 3x3 determinant | x y z | of rows i, j, k (rows of 62-bit integers),
+// expanded along z with the 2x2 minors in x, y
+SM_HD inline Fix256 fix_det3(const int64_t (&v)[4][3], int i, int j, int k) {
+  const fix_i128 mjk = (fix_i128)v[j][0] * v[k][1] - (fix_i128)v[k][0] * v[j][1];
+  const fix_i128 mik = (fix_i128)v[i][0] * v[k][1] - (fix_i128)v[k][0] * v[i][1];
+  const fix_i128 mij = (fix_i128)v[i][0] * v[j][1] - (fix_i128)v[j][0] * v[i][1];
+  Fix256 s = fix_mul_zm(v[i][2], mjk);
+  fix_addw<4>(s, fix_mul_zm(v[j][2], -mik));
+  fix_addw<4>(s, fix_mul_zm(v[k][2], mij));
+  return s;
+}
+
+// orient3d's sign convention (> 0: b - a, c - a, d - a positively
+// oriented), i.e. the sign of -det[a - d; b - d; c - d]
+SM_HD inline int orient3d_fix(const double *a, const double *b, const double *c,
+                              const double *d) {
+  const double *P[4] = {a, b, c, d};
+  FixNum f[4][3];
+  int E;
+  if (!fix_split_all<4, 3>(P, f, E)) return SM_PRED_UNRESOLVED;
+  int64_t v[4][3];
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k)
+      if (!fix_diff(f[i][k], f[3][k], E, v[i][k])) return SM_PRED_UNRESOLVED;
+  return -fix_signw<4>(fix_det3(v, 0, 1, 2));
+}
+
+// insphere's sign convention (> 0: e inside the sphere of a positively
+// oriented abcd): the lifted determinant of a - e .. d - e expanded along
+// the lift column, -(-l0 D123 + l1 D023 - l2 D013 + l3 D012)
+SM_HD inline int insphere_fix(const double *a, const double *b, const double *c,
+                              const double *d, const double *e) {
+  const double *P[5] = {a, b, c, d, e};
+  FixNum f[5][3];
+  int E;
+  if (!fix_split_all<5, 3>(P, f, E)) return SM_PRED_UNRESOLVED;
+  int64_t v[4][3];
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 3; ++k)
+      if (!fix_diff(f[i][k], f[4][k], E, v[i][k])) return SM_PRED_UNRESOLVED;
+  fix_u128 l[4];
+  for (int i = 0; i < 4; ++i)
+    l[i] = (fix_u128)((fix_i128)v[i][0] * v[i][0]) + (fix_u128)((fix_i128)v[i][1] * v[i][1]) +
+           (fix_u128)((fix_i128)v[i][2] * v[i][2]);
+  Fix256 D[4] = {fix_det3(v, 1, 2, 3), fix_det3(v, 0, 2, 3), fix_det3(v, 0, 1, 3),
+                 fix_det3(v, 0, 1, 2)};
+  // det = -l0 D0 + l1 D1 - l2 D2 + l3 D3
+  Fix384 s = fix_mul384(l[1], D[1]);
+  fix_addw<6>(s, fix_mul384(l[3], D[3]));
+  Fix256 n0 = D[0], n2 = D[2];
+  fix_negate(n0.w, 4);
+  fix_negate(n2.w, 4);
+  fix_addw<6>(s, fix_mul384(l[0], n0));
+  fix_addw<6>(s, fix_mul384(l[2], n2));
+  return -fix_signw<6>(s);
+}
+
 // Host entry points: same filters, exact stage with room for any input a
 // double-precision coordinate set can produce in practice.
 inline int orient2d_h(const double *a, const double *b, const double *c) {
@@ -746,6 +868,8 @@ inline int incircle_h(const double *a, const double *b, const double *c, const d
 inline int orient3d_h(const double *a, const double *b, const double *c, const double *d) {
   int s = orient3d_filter(a, b, c, d);
   if (s != SM_PRED_UNRESOLVED) return s;
+  s = orient3d_fix(a, b, c, d);
+  if (s != SM_PRED_UNRESOLVED) return s;
   s = orient3d_exact<96>(a, b, c, d);
   if (s != SM_PRED_UNRESOLVED) return s;
   return orient3d_exact<2048>(a, b, c, d);
@@ -753,6 +877,8 @@ inline int orient3d_h(const double *a, const double *b, const double *c, const d
 inline int insphere_h(const double *a, const double *b, const double *c, const double *d,
                       const double *e) {
   int s = insphere_filter(a, b, c, d, e);
+  if (s != SM_PRED_UNRESOLVED) return s;
+  s = insphere_fix(a, b, c, d, e);
   if (s != SM_PRED_UNRESOLVED) return s;
   s = insphere_exact<160>(a, b, c, d, e);
   if (s != SM_PRED_UNRESOLVED) return s;

@@ -41,4 +41,11 @@ int h_orient2d_fix(const double *a, const double *b, const double *c) {
 int h_incircle_fix(const double *a, const double *b, const double *c, const double *d) {
   return smpred::incircle_fix(a, b, c, d);
 }
+int h_orient3d_fix(const double *a, const double *b, const double *c, const double *d) {
+  return smpred::orient3d_fix(a, b, c, d);
+}
+int h_insphere_fix(const double *a, const double *b, const double *c, const double *d,
+                   const double *e) {
+  return smpred::insphere_fix(a, b, c, d, e);
+}
 }

@@ -29,7 +29,8 @@ def lib(tmp_path_factory):
     L = ctypes.CDLL(out)
     for name in ("h_orient2d_exact", "h_incircle_exact96", "h_incircle_h", "h_orient3d_h",
                  "h_insphere_h", "h_insphere_exact160", "h_orient2d_if_exact",
-                 "h_incircle_if_exact", "h_orient2d_fix", "h_incircle_fix"):
+                 "h_incircle_if_exact", "h_orient2d_fix", "h_incircle_fix", "h_orient3d_fix",
+                 "h_insphere_fix"):
         getattr(L, name).restype = ctypes.c_int
     return L
 
@@ -215,3 +216,58 @@ def test_fixed_point_stage(lib):
     assert lib.h_orient2d_fix(_p(a), _p(b), _p(d)) == UNRES
     assert lib.h_orient2d_fix(_p(a), _p(b), _p(np.array([np.inf, 0.0]))) == UNRES
     assert lib.h_incircle_fix(_p(a), _p(b), _p(c), _p(np.array([np.nan, 0.0]))) == UNRES
+
+
+def test_fixed_point_stage_3d(lib):
+    """The 256/384-bit integer stage of orient3d and insphere answers
+    exactly or defers: cospherical cube corners of lattices (sign 0) at
+    the spacings of the 3D workloads, one-ulp perturbations of them (sign
+    decided by the last bit), points far from the origin, mixed signs
+    around the axes with zeros, and refusals (exponent spans too wide,
+    non-finite input).  A slip in a carry, a sign or a minor's order fails
+    the zeros or the perturbed cases."""
+    rng = np.random.default_rng(5)
+    cube = np.array([[x, y, z] for x in (0, 1) for y in (0, 1) for z in (0, 1)], float)
+    cases = [rng.random((5, 3)) for _ in range(100)]
+    for h in (1.0, 0.1, 0.015, 150.0, 1.0 / 3.0):
+        for off in (0.0, 0.25, -4200.0):
+            for _ in range(20):
+                i, j, k = rng.integers(-60, 60, 3)
+                pts = (cube + [i, j, k]) * h + off
+                q = pts[rng.permutation(8)[:5]]
+                cases.append(q.copy())
+                k2, c2 = rng.integers(0, 5), rng.integers(0, 3)
+                q[k2, c2] = np.nextafter(q[k2, c2], np.inf if rng.random() < 0.5 else -np.inf)
+                cases.append(q)
+    for _ in range(100):
+        base = rng.random(3) * 1e5
+        cases.append(base + rng.random((5, 3)) * 1e-5)
+    for _ in range(100):
+        q = rng.standard_normal((5, 3)) * 1e-2
+        q[rng.integers(0, 5), rng.integers(0, 3)] = 0.0
+        cases.append(q)
+    answered = {"o": 0, "i": 0, "i0": 0}
+    for q in cases:
+        a, b, c, d, e = q
+        o = P.orient3d(a, b, c, d)
+        got = lib.h_orient3d_fix(_p(a), _p(b), _p(c), _p(d))
+        assert got in (o, UNRES), (q, o, got)
+        answered["o"] += got != UNRES
+        if o == 0:
+            continue
+        if o < 0:
+            b, c = c, b
+        want = P.insphere(a, b, c, d, e)
+        got = lib.h_insphere_fix(_p(a), _p(b), _p(c), _p(d), _p(e))
+        assert got in (want, UNRES), (q, want, got)
+        answered["i"] += got != UNRES
+        answered["i0"] += got == 0
+        # an even permutation of the cell leaves the answer
+        got2 = lib.h_insphere_fix(_p(b), _p(c), _p(a), _p(d), _p(e))
+        assert got2 == got
+    assert answered["o"] > 0.8 * len(cases) and answered["i"] > 0.6 * len(cases), answered
+    assert answered["i0"] > 50, answered
+    a, b, c, d, e = np.array([[1.0, 0, 0], [0, 1.0, 0], [0, 0, 1.0], [0, 0, 0], [1e-30, 0, 0]])
+    assert lib.h_insphere_fix(_p(a), _p(b), _p(c), _p(d), _p(e)) == UNRES
+    assert lib.h_orient3d_fix(_p(a), _p(b), _p(c), _p(e)) == UNRES
+    assert lib.h_orient3d_fix(_p(a), _p(b), _p(c), _p(np.array([np.inf, 0, 0]))) == UNRES
