@@ -1004,10 +1004,13 @@ struct Builder3 {
           if (round == 0 && k >= S) break;
           for (;;) {
             if (fail.load(std::memory_order_relaxed)) return;
-            const size_t r = cursor[(size_t)k].fetch_add(1);
-            if (r >= nch[(size_t)k]) break;
+            const size_t r0 = cursor[(size_t)k].fetch_add(1);
+            if (r0 >= nch[(size_t)k]) break;
+            // odd slabs start half-way along their curve, so that the
+            // threads of neighbouring slabs do not meet at the boundary
+            const size_t r = (r0 + ((k & 1) ? nch[(size_t)k] / 2 : 0)) % nch[(size_t)k];
             const size_t b = r * chunk, e = std::min(slab[(size_t)k].size(), b + chunk);
-            if (k != prev_slab || r == 0) {
+            if (k != prev_slab || r0 == 0 || r == 0) {
               // in another slab, start near the chunk when that
               // tetrahedron still lives (the previous position stays as
               // the fallback); the next chunk of the same slab starts
