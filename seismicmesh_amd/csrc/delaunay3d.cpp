@@ -582,7 +582,12 @@ struct Builder3 {
   // from a dead slot (a stale hint) starts again from a live one.
   static constexpr int kDead = -2;
   static constexpr int kParallelMin = 4096;
-  static constexpr int kBlock = 64;
+  // fresh slots per block: 1024 (32 KiB of tetrahedra), starting on a
+  // 64-slot boundary, so that no two threads write the same cache line of
+  // a block and each thread's new tetrahedra stay together (at 590 k points
+  // on 16 threads: 0.224 s with blocks of 64, 0.20 s with 256, 0.177 s
+  // with 1024)
+  static constexpr int kBlock = 1024;
   std::vector<int64_t> slab_sizes;  // points per slab of the last parallel build
   std::vector<int> vtet;            // a tetrahedron made when each point went in
   int64_t retried = 0;              // points left to the single-thread pass
@@ -879,12 +884,15 @@ struct Builder3 {
     t0 = now_ms();
     // fixed capacity, fresh slots left unwritten until handed out; the
     // dead slots of the serial phase go to thread 0
-    cap_par = std::max<int64_t>((int64_t)T.size() + 64, 8 * (int64_t)nloc + 4096);
+    cap_par = std::max<int64_t>((int64_t)T.size() + 128, 8 * (int64_t)nloc + 4096) +
+              (int64_t)std::max(1, nthreads) * kBlock;
     if (cap_par > (int64_t)std::numeric_limits<int>::max())
       return sm_set_error(SM_ERR_CAPACITY, "host 3D Delaunay: too many points");
-    std::atomic<int64_t> nt_count{(int64_t)T.size()};
+    std::atomic<int64_t> nt_count{((int64_t)T.size() + 63) / 64 * 64};
     const size_t old_n = T.size();
     T.resize((size_t)cap_par);
+    // the slots skipped to reach the boundary are never handed out
+    for (int64_t t = (int64_t)old_n; t < nt_count.load(); ++t) T[(size_t)t].v[0] = kDead;
     for (int f : free_slots) T[f].v[0] = kDead;
     mark.assign((size_t)cap_par, 0);
     if (lockcap < (size_t)cap_par) {
