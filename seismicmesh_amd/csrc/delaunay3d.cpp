@@ -65,6 +65,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <functional>
 #include <mutex>
 #include <new>
 #include <thread>
@@ -136,6 +137,71 @@ inline uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
     for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
   return h;
 }
+
+// Worker threads kept between parallel sections: run(nt, fn) calls fn(0)
+// on the calling thread and fn(1 .. nt-1) on workers 1 .. nt-1 (started
+// on first use), and returns when all have finished.  Idle workers sleep
+// on a condition variable.  Sections do not nest.
+class WorkerPool {
+ public:
+  WorkerPool() = default;
+  WorkerPool(const WorkerPool &) = delete;
+  WorkerPool &operator=(const WorkerPool &) = delete;
+  ~WorkerPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    wake_.notify_all();
+    for (auto &t : th_) t.join();
+  }
+  void run(int nt, const std::function<void(int)> &fn) {
+    if (nt <= 1) {
+      fn(0);
+      return;
+    }
+    while ((int)th_.size() < nt - 1) {
+      const int w = (int)th_.size() + 1;
+      th_.emplace_back([this, w] { loop(w); });
+    }
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      fn_ = &fn;
+      nt_ = nt;
+      pending_ = nt - 1;
+      ++gen_;
+    }
+    wake_.notify_all();
+    fn(0);
+    std::unique_lock<std::mutex> lk(mu_);
+    done_.wait(lk, [this] { return pending_ == 0; });
+    fn_ = nullptr;
+  }
+
+ private:
+  void loop(int w) {
+    uint64_t seen = 0;
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+      wake_.wait(lk, [&] { return stop_ || gen_ != seen; });
+      if (stop_) return;
+      seen = gen_;
+      if (w >= nt_) continue;
+      const std::function<void(int)> *f = fn_;
+      lk.unlock();
+      (*f)(w);
+      lk.lock();
+      if (--pending_ == 0) done_.notify_one();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable wake_, done_;
+  const std::function<void(int)> *fn_ = nullptr;
+  int nt_ = 0, pending_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
 
 // kOther[a][b]: the two slots of a tetrahedron other than a and b (a != b)
 constexpr int kOther[4][4][2] = {
@@ -229,6 +295,8 @@ struct Builder3 {
   static constexpr int kInf = -1;
   std::vector<double> P;          // local points (n, 3)
   std::vector<int64_t> gid;       // local -> global id
+  std::vector<double> P2;         // scratch of the renumbering
+  std::vector<int64_t> gid2;
   std::vector<int64_t> dropped;   // global ids of skipped duplicates
   TetArray T;
   std::vector<int> free_slots;
@@ -491,13 +559,13 @@ struct Builder3 {
     return k;
   }
 
-  // fn(0 .. nt-1), on nt threads (this one included)
+  // fn(0 .. nt-1), on nt threads (this one included), on the pool's
+  // workers, which are kept between sections and builds (creating 15
+  // threads costs about 0.5 ms; a build has a dozen sections)
+  WorkerPool pool;
   template <class Fn>
-  static void on_threads(int nt, Fn &&fn) {
-    std::vector<std::thread> pool;
-    for (int w = 1; w < nt; ++w) pool.emplace_back([&fn, w] { fn(w); });
-    fn(0);
-    for (auto &t : pool) t.join();
+  void on_threads(int nt, Fn &&fn) {
+    pool.run(nt, std::function<void(int)>(std::forward<Fn>(fn)));
   }
 
   // Triangulates the nloc points of P in Hilbert order (on nthreads
@@ -510,35 +578,65 @@ struct Builder3 {
     last = 0;
     if (mark.size() < 64) mark.assign(64, 0);
     // Hilbert keys (30 bits) on the threads, then a stable two-pass radix
-    // sort: the order of std::sort on (key, id)
+    // sort on the threads (15-bit digits; per-thread histograms, each
+    // thread scattering its own range): the order of std::sort on (key, id)
     for (double &x : ph) x = 0.0;
     double t0 = now_ms();
     key.resize(n);
-    const int kt = std::max(1, std::min(nthreads, n / 32768 + 1));
+    const int kt = std::max(1, std::min(nthreads, n / 8192 + 1));
+    auto part = [&](int w) { return (int64_t)n * w / kt; };
     on_threads(kt, [&](int w) {
-      for (int i = (int)((int64_t)n * w / kt); i < (int)((int64_t)n * (w + 1) / kt); ++i) {
+      for (int64_t i = part(w); i < part(w + 1); ++i) {
         uint32_t g[3];
         for (int k = 0; k < 3; ++k)
-          g[k] = (uint32_t)std::min(1023.0, std::max(0.0, (pt(i)[k] - lo[k]) / ext * 1023.0));
-        key[i] = {hilbert3(g[0], g[1], g[2], 10), i};
+          g[k] = (uint32_t)std::min(1023.0, std::max(0.0, (pt((int)i)[k] - lo[k]) / ext * 1023.0));
+        key[(size_t)i] = {hilbert3(g[0], g[1], g[2], 10), (int)i};
       }
     });
     {
+      constexpr int kDigits = 1 << 15;
       std::vector<std::pair<uint64_t, int>> tmp((size_t)n);
-      std::vector<int> cnt(1 << 15);
+      std::vector<int64_t> cnt((size_t)kt * kDigits);
       for (int pass = 0; pass < 2; ++pass) {
         const int sh = 15 * pass;
-        std::fill(cnt.begin(), cnt.end(), 0);
-        for (int i = 0; i < n; ++i) ++cnt[(key[i].first >> sh) & 0x7FFF];
-        int s = 0;
-        for (int &c : cnt) {
-          const int x = c;
-          c = s;
-          s += x;
-        }
-        for (int i = 0; i < n; ++i) tmp[(size_t)cnt[(key[i].first >> sh) & 0x7FFF]++] = key[i];
+        on_threads(kt, [&](int w) {
+          int64_t *c = cnt.data() + (size_t)w * kDigits;
+          std::fill(c, c + kDigits, 0);
+          for (int64_t i = part(w); i < part(w + 1); ++i) ++c[(key[(size_t)i].first >> sh) & 0x7FFF];
+        });
+        // digit-major, thread-minor offsets keep the sort stable
+        int64_t run = 0;
+        for (int d = 0; d < kDigits; ++d)
+          for (int w = 0; w < kt; ++w) {
+            int64_t &c = cnt[(size_t)w * kDigits + d];
+            const int64_t x = c;
+            c = run;
+            run += x;
+          }
+        on_threads(kt, [&](int w) {
+          int64_t *c = cnt.data() + (size_t)w * kDigits;
+          for (int64_t i = part(w); i < part(w + 1); ++i)
+            tmp[(size_t)c[(key[(size_t)i].first >> sh) & 0x7FFF]++] = key[(size_t)i];
+        });
         key.swap(tmp);
       }
+    }
+    // local ids renumbered in curve order, so that the coordinates a walk
+    // or a cavity reads lie together in memory (the perturbation orders
+    // points by their coordinates, not their ids: the result is the same)
+    {
+      P2.resize(P.size());
+      gid2.resize(gid.size());
+      on_threads(kt, [&](int w) {
+        for (int64_t k = part(w); k < part(w + 1); ++k) {
+          const int i = key[(size_t)k].second;
+          for (int j = 0; j < 3; ++j) P2[3 * (size_t)k + j] = P[3 * (size_t)i + j];
+          gid2[(size_t)k] = gid[(size_t)i];
+          key[(size_t)k].second = (int)k;
+        }
+      });
+      P.swap(P2);
+      gid.swap(gid2);
     }
     ph[0] = now_ms() - t0;
     const size_t k0 = start(key);
@@ -594,7 +692,7 @@ struct Builder3 {
   double ph[8] = {};                // phase times of the last build (ms), traced
   struct TCtx {
     uint32_t me = 0;
-    std::vector<int> held, cav, stack, ids, freel, retry, recent;
+    std::vector<int> held, cav, stack, ids, freel, retry, retry_pred, recent;
     std::vector<BF> bnd;
     std::vector<Tet> nt;
     std::vector<Slot> table;
@@ -610,30 +708,30 @@ struct Builder3 {
   int64_t cap_par = 0;
   std::mutex *dup_mu = nullptr;
 
+  // a lock word holds its owner (me << 2) and, while held, the cavity mark
+  // of the tetrahedron (low two bits), so both share one cache line
   bool trylock(TCtx &c, int t) {
     std::atomic<uint32_t> &l = locks[t];
     uint32_t cur = l.load(std::memory_order_relaxed);
-    if (cur == c.me) return true;
+    if ((cur >> 2) == c.me) return true;
     if (cur != 0) return false;
-    if (!l.compare_exchange_strong(cur, c.me, std::memory_order_acquire)) return false;
+    if (!l.compare_exchange_strong(cur, c.me << 2, std::memory_order_acquire)) return false;
     c.held.push_back(t);
     return true;
+  }
+  int pmark(int t) const { return (int)(locks[t].load(std::memory_order_relaxed) & 3u); }
+  void set_pmark(const TCtx &c, int t, int m) {
+    locks[t].store((c.me << 2) | (uint32_t)m, std::memory_order_relaxed);
   }
   // lets go of every lock of c but the one on t
   void keep_only(TCtx &c, int t) {
     for (int h : c.held)
-      if (h != t) {
-        mark[h] = 0;
-        locks[h].store(0, std::memory_order_release);
-      }
+      if (h != t) locks[h].store(0, std::memory_order_release);
     c.held.clear();
     c.held.push_back(t);
   }
   void unlock_all(TCtx &c) {
-    for (int t : c.held) {
-      mark[t] = 0;
-      locks[t].store(0, std::memory_order_release);
-    }
+    for (int t : c.held) locks[t].store(0, std::memory_order_release);
     c.held.clear();
   }
   int par_new_slot(TCtx &c) {
@@ -743,7 +841,7 @@ struct Builder3 {
     c.cav.clear();
     c.bnd.clear();
     c.stack.assign(1, t0);
-    mark[t0] = 1;
+    set_pmark(c, t0, 1);
     while (!c.stack.empty()) {
       const int tt = c.stack.back();
       c.stack.pop_back();
@@ -754,16 +852,16 @@ struct Builder3 {
           unlock_all(c);
           return 2;
         }
-        if (nb >= 0 && mark[nb] == 1) continue;
+        if (nb >= 0 && pmark(nb) == 1) continue;
         bool in = false;
-        if (nb >= 0 && mark[nb] == 0) {
+        if (nb >= 0 && pmark(nb) == 0) {
           const int is = inf_slot(T[nb]);
           if (is >= 0 && !trylock(c, T[nb].n[is])) {
             unlock_all(c);
             return 2;
           }
           in = conflict(nb, q);
-          mark[nb] = in ? 1 : 2;
+          set_pmark(c, nb, in ? 1 : 2);
           if (in) c.stack.push_back(nb);
         }
         if (!in) {
@@ -829,7 +927,7 @@ struct Builder3 {
     }
     for (int h : c.used) c.table[h].key = ~0ull;
     c.last = c.ids[0];
-    vtet[qi] = c.ids[0];
+    __atomic_store_n(&vtet[(size_t)qi], c.ids[0], __ATOMIC_RELAXED);  // read by other threads
     c.recent.push_back(c.ids[0]);
     if (c.recent.size() > 64) c.recent.erase(c.recent.begin(), c.recent.begin() + 32);
     unlock_all(c);
@@ -894,7 +992,7 @@ struct Builder3 {
     // the slots skipped to reach the boundary are never handed out
     for (int64_t t = (int64_t)old_n; t < nt_count.load(); ++t) T[(size_t)t].v[0] = kDead;
     for (int f : free_slots) T[f].v[0] = kDead;
-    mark.assign((size_t)cap_par, 0);
+    // (the concurrent passes keep their cavity marks in the lock words)
     if (lockcap < (size_t)cap_par) {
       lockbuf.reset(new std::atomic<uint32_t>[(size_t)cap_par]);
       for (size_t i = 0; i < (size_t)cap_par; ++i) lockbuf[i].store(0, std::memory_order_relaxed);
@@ -923,24 +1021,34 @@ struct Builder3 {
     ph[4] = now_ms() - t0;
     t0 = now_ms();
     // points that met taken locks twice: more concurrent passes while there
-    // are many of them (contention is a matter of timing), then one thread
+    // are more than a few of them (contention is a matter of timing), then
+    // one thread (slow: the tetrahedra it visits sit in other cores' caches,
+    // about 12 us per point against 4 us in a pass)
     retried = 0;
     for (int w = 0; w < nthr; ++w) retried += (int64_t)ctx[w].retry.size();
-    for (int round = 0; round < 3 && rc == 0; ++round) {
-      std::vector<int> pend;
+    // every retried point starts its walk next to the point before it on
+    // the curve: vtet is refreshed first to a live tetrahedron of each
+    // vertex (the one made at its insertion may be gone)
+    if (retried > 0 && rc == 0) refresh_vtet(ctx);
+    for (int round = 0; round < 6 && rc == 0; ++round) {
+      std::vector<int> pend, ppred;
       for (auto &c : ctx) {
         pend.insert(pend.end(), c.retry.begin(), c.retry.end());
+        ppred.insert(ppred.end(), c.retry_pred.begin(), c.retry_pred.end());
         c.retry.clear();
+        c.retry_pred.clear();
       }
-      if (pend.size() < 2048) {
+      if (pend.size() < 64) {
         ctx[0].retry.swap(pend);
+        ctx[0].retry_pred.swap(ppred);
         break;
       }
-      rc = concurrent_pass(pend, pend, ctx, axis, nslabs, xlo, xhi, false);
+      rc = concurrent_pass(pend, ppred, ctx, axis, nslabs, xlo, xhi, false);
     }
     for (int w = 0; w < nthr && rc == 0; ++w)
-      for (int v : ctx[w].retry) {
-        const int r2 = par_insert(ctx[0], v);
+      for (size_t i = 0; i < ctx[w].retry.size(); ++i) {
+        hint_from(ctx[0], ctx[w].retry_pred[i]);
+        const int r2 = par_insert(ctx[0], ctx[w].retry[i]);
         if (r2 == 2 || r2 < 0) {
           rc = r2 < 0 ? r2 : SM_ERR_NUMERIC;
           break;
@@ -1023,14 +1131,15 @@ struct Builder3 {
               // tetrahedron still lives (the previous position stays as
               // the fallback); the next chunk of the same slab starts
               // where the last one ended
-              const int hv = vtet[spred[(size_t)k][b]];
+              const int hv = __atomic_load_n(&vtet[(size_t)spred[(size_t)k][b]], __ATOMIC_RELAXED);
               if (hv >= 0) {
                 c.recent.push_back(c.last);
                 c.last = hv;
               }
             }
             prev_slab = k;
-            if (int rc = insert_chunk(c, slab[(size_t)k], b, e, again)) {
+            if (int rc = insert_chunk(c, slab[(size_t)k], spred[(size_t)k], b, e, again,
+                                      !count)) {
               fail = rc;
               return;
             }
@@ -1042,21 +1151,61 @@ struct Builder3 {
   }
 
   // Inserts sp[b..e) on thread context c; points that meet a taken lock go
-  // once more at the end of the chunk, then to c.retry.
-  int insert_chunk(TCtx &c, const std::vector<int> &sp, size_t b, size_t e,
-                   std::vector<int> &again) {
+  // once more at the end of the chunk, then (with their entries of pred)
+  // to c.retry.  per_point: every point starts its walk from the
+  // tetrahedron of its pred (retried points lie far apart on the curve).
+  int insert_chunk(TCtx &c, const std::vector<int> &sp, const std::vector<int> &pred,
+                   size_t b, size_t e, std::vector<int> &again, bool per_point) {
     again.clear();
     for (size_t k = b; k < e; ++k) {
+      if (per_point) hint_from(c, pred[k]);
       const int rc = par_insert(c, sp[k]);
-      if (rc == 2) again.push_back(sp[k]);
+      if (rc == 2) again.push_back((int)k);
       else if (rc < 0) return rc;
     }
-    for (int v : again) {
-      const int rc = par_insert(c, v);
-      if (rc == 2) c.retry.push_back(v);
-      else if (rc < 0) return rc;
+    for (int k : again) {
+      if (per_point) hint_from(c, pred[(size_t)k]);
+      const int rc = par_insert(c, sp[(size_t)k]);
+      if (rc == 2) {
+        c.retry.push_back(sp[(size_t)k]);
+        c.retry_pred.push_back(pred[(size_t)k]);
+      } else if (rc < 0) {
+        return rc;
+      }
     }
     return 0;
+  }
+
+  // the next walk of c starts at the tetrahedron recorded for vertex u
+  // (when there is one; the current start stays as the fallback)
+  void hint_from(TCtx &c, int u) {
+    const int hv = u >= 0 ? __atomic_load_n(&vtet[(size_t)u], __ATOMIC_RELAXED) : -1;
+    if (hv >= 0 && hv != c.last) {
+      c.recent.push_back(c.last);
+      if (c.recent.size() > 64) c.recent.erase(c.recent.begin(), c.recent.begin() + 32);
+      c.last = hv;
+    }
+  }
+
+  // vtet[v] := a live tetrahedron with vertex v, for every inserted v
+  // (threads over ranges of slots; any of the writers of a vertex will do).
+  // The threads' blocks hold slots not yet handed out, never written: they
+  // are marked dead first (handing one out later writes it whole).
+  void refresh_vtet(std::vector<TCtx> &ctx) {
+    const int nthr = (int)ctx.size();
+    for (auto &c : ctx) {
+      for (int64_t t = c.blk; t < c.blk_end; ++t) T[(size_t)t].v[0] = kDead;
+      for (int t : c.skipped) T[(size_t)t].v[0] = kDead;
+    }
+    const int64_t nslot = std::min<int64_t>(ntet->load(), cap_par);
+    on_threads(nthr, [&](int w) {
+      for (int64_t t = nslot * w / nthr; t < nslot * (w + 1) / nthr; ++t) {
+        const Tet &r = T[(size_t)t];
+        if (r.v[0] == kDead) continue;
+        for (int j = 0; j < 4; ++j)
+          if (r.v[j] >= 0) __atomic_store_n(&vtet[(size_t)r.v[j]], (int)t, __ATOMIC_RELAXED);
+      }
+    });
   }
 };
 
@@ -1127,7 +1276,7 @@ int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, con
   };
   auto live = [&](const Tet &r) { return r.v[0] >= 0 && Builder3::inf_slot(r) < 0; };
   std::vector<int64_t> cnt((size_t)nthr * nthr, 0);
-  Builder3::on_threads(nthr, [&](int w) {
+  b.on_threads(nthr, [&](int w) {
     const int64_t s0 = nslot * w / nthr, s1 = nslot * (w + 1) / nthr;
     int64_t *c = cnt.data() + (size_t)w * nthr;
     for (int64_t t = s0; t < s1; ++t) {
@@ -1156,7 +1305,7 @@ int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, con
     sorted_.resize((size_t)m * 4);
     dst = sorted_.data();
   }
-  Builder3::on_threads(nthr, [&](int w) {
+  b.on_threads(nthr, [&](int w) {
     const int64_t s0 = nslot * w / nthr, s1 = nslot * (w + 1) / nthr;
     int64_t *p = pos.data() + (size_t)w * nthr;
     for (int64_t t = s0; t < s1; ++t) {
@@ -1171,7 +1320,7 @@ int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, con
   });
   start_.assign((size_t)n + 1, 0);
   auto *cells = reinterpret_cast<std::array<int32_t, 4> *>(dst);
-  Builder3::on_threads(nthr, [&](int w) {
+  b.on_threads(nthr, [&](int w) {
     // start_[i + 1] ends bucket i; this thread writes start_[a + 1 .. b] only
     const int64_t a = lo_id(w), bb = lo_id(w + 1), g0 = below[(size_t)w], g1 = below[(size_t)w + 1];
     std::vector<int64_t> fill((size_t)(bb - a), 0);
