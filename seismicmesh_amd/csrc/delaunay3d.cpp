@@ -112,29 +112,77 @@ class TetArray {
   size_t n_ = 0, cap_ = 0;
 };
 
-inline uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
-  // Skilling's transpose-form Hilbert index in 3D
-  uint32_t X[3] = {x, y, z};
-  uint32_t M = 1u << (bits - 1), P, Q, t;
-  for (Q = M; Q > 1; Q >>= 1) {
-    P = Q - 1;
-    for (int i = 0; i < 3; ++i) {
-      if (X[i] & Q) X[0] ^= P;
-      else {
-        t = (X[0] ^ X[i]) & P;
-        X[0] ^= t;
-        X[i] ^= t;
-      }
-    }
-  }
-  for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
-  t = 0;
-  for (Q = M; Q > 1; Q >>= 1)
-    if (X[2] & Q) t ^= Q - 1;
-  for (int i = 0; i < 3; ++i) X[i] ^= t;
+// 3D Hilbert index of a point of the 1024^3 grid, one octant (3 bits) per
+// level from a 24-state table: kHilbertDigit[s][o] is the curve position
+// of octant o = (x bit, y bit, z bit) of the cell in state s, and
+// kHilbertNext[s][o] the state of that octant's sub-curve.  The tables are
+// Skilling's transpose-form Hilbert curve ("Programming the Hilbert curve",
+// AIP Conf. Proc. 707, 2004) written as an automaton: each octant's
+// sub-curve is the parent curve under one of the 48 signed axis
+// permutations, found by matching the two-level curve, and the states are
+// the compositions reached from the identity (24 of them).  The order only
+// steers insertion, not the result.
+static const uint8_t kHilbertNext[24][8] = {
+    {1, 2, 3, 0, 4, 5, 6, 0},
+    {7, 8, 9, 10, 11, 2, 1, 1},
+    {6, 0, 12, 13, 14, 2, 1, 2},
+    {15, 16, 3, 3, 9, 10, 17, 0},
+    {18, 5, 4, 4, 15, 16, 9, 10},
+    {19, 5, 4, 5, 3, 0, 20, 13},
+    {9, 10, 17, 0, 7, 8, 6, 6},
+    {0, 21, 13, 9, 6, 7, 12, 7},
+    {22, 17, 10, 23, 8, 6, 8, 12},
+    {2, 15, 1, 9, 5, 7, 4, 9},
+    {16, 11, 10, 1, 8, 18, 10, 4},
+    {17, 6, 23, 12, 11, 14, 11, 1},
+    {23, 13, 21, 22, 12, 12, 7, 8},
+    {20, 13, 14, 2, 12, 13, 19, 5},
+    {21, 22, 7, 8, 14, 14, 11, 2},
+    {3, 15, 20, 15, 0, 21, 13, 9},
+    {16, 3, 16, 20, 22, 17, 10, 23},
+    {11, 1, 17, 3, 18, 4, 17, 6},
+    {18, 19, 18, 4, 17, 3, 23, 20},
+    {19, 19, 18, 5, 21, 22, 15, 16},
+    {20, 20, 15, 16, 23, 13, 21, 22},
+    {14, 21, 2, 15, 19, 21, 5, 7},
+    {22, 14, 16, 11, 22, 19, 8, 18},
+    {23, 20, 11, 14, 23, 12, 18, 19},
+};
+static const uint8_t kHilbertDigit[24][8] = {
+    {0, 1, 3, 2, 7, 6, 4, 5},
+    {0, 7, 1, 6, 3, 4, 2, 5},
+    {0, 1, 7, 6, 3, 2, 4, 5},
+    {6, 1, 5, 2, 7, 0, 4, 3},
+    {4, 3, 5, 2, 7, 0, 6, 1},
+    {4, 5, 3, 2, 7, 6, 0, 1},
+    {0, 7, 3, 4, 1, 6, 2, 5},
+    {0, 3, 7, 4, 1, 2, 6, 5},
+    {4, 7, 3, 0, 5, 6, 2, 1},
+    {0, 3, 1, 2, 7, 4, 6, 5},
+    {4, 7, 5, 6, 3, 0, 2, 1},
+    {6, 7, 1, 0, 5, 4, 2, 3},
+    {4, 3, 7, 0, 5, 2, 6, 1},
+    {4, 5, 7, 6, 3, 2, 0, 1},
+    {6, 1, 7, 0, 5, 2, 4, 3},
+    {6, 5, 1, 2, 7, 4, 0, 3},
+    {2, 1, 5, 6, 3, 0, 4, 7},
+    {6, 7, 5, 4, 1, 0, 2, 3},
+    {2, 3, 5, 4, 1, 0, 6, 7},
+    {2, 5, 3, 4, 1, 6, 0, 7},
+    {2, 5, 1, 6, 3, 4, 0, 7},
+    {6, 5, 7, 4, 1, 2, 0, 3},
+    {2, 1, 3, 0, 5, 6, 4, 7},
+    {2, 3, 1, 0, 5, 4, 6, 7},
+};
+
+inline uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z) {
   uint64_t h = 0;
-  for (int b = bits - 1; b >= 0; --b)
-    for (int i = 0; i < 3; ++i) h = (h << 1) | ((X[i] >> b) & 1u);
+  int s = 0;
+  for (int b = 9; b >= 0; --b) {
+    const int o = (int)(((x >> b) & 1u) << 2 | ((y >> b) & 1u) << 1 | ((z >> b) & 1u));
+    h = h << 3 | kHilbertDigit[s][o];
+    s = kHilbertNext[s][o];
+  }
   return h;
 }
 
@@ -301,7 +349,7 @@ struct Builder3 {
   TetArray T;
   std::vector<int> free_slots;
   std::vector<char> mark;
-  std::vector<std::pair<uint64_t, int>> key;
+  std::vector<std::pair<uint64_t, int>> key, tmpkey;
   std::vector<int> deferred;
   int last = 0;
   int nloc = 0;
@@ -577,8 +625,8 @@ struct Builder3 {
     const int n = nloc;
     last = 0;
     if (mark.size() < 64) mark.assign(64, 0);
-    // Hilbert keys (30 bits) on the threads, then a stable two-pass radix
-    // sort on the threads (15-bit digits; per-thread histograms, each
+    // Hilbert keys (30 bits) on the threads, then a stable three-pass radix
+    // sort on the threads (10-bit digits; per-thread histograms, each
     // thread scattering its own range): the order of std::sort on (key, id)
     for (double &x : ph) x = 0.0;
     double t0 = now_ms();
@@ -590,19 +638,20 @@ struct Builder3 {
         uint32_t g[3];
         for (int k = 0; k < 3; ++k)
           g[k] = (uint32_t)std::min(1023.0, std::max(0.0, (pt((int)i)[k] - lo[k]) / ext * 1023.0));
-        key[(size_t)i] = {hilbert3(g[0], g[1], g[2], 10), (int)i};
+        key[(size_t)i] = {hilbert3(g[0], g[1], g[2]), (int)i};
       }
     });
     {
-      constexpr int kDigits = 1 << 15;
-      std::vector<std::pair<uint64_t, int>> tmp((size_t)n);
+      constexpr int kBits = 10, kDigits = 1 << kBits;
+      tmpkey.resize((size_t)n);
+      std::vector<std::pair<uint64_t, int>> &tmp = tmpkey;
       std::vector<int64_t> cnt((size_t)kt * kDigits);
-      for (int pass = 0; pass < 2; ++pass) {
-        const int sh = 15 * pass;
+      for (int pass = 0; pass < 3; ++pass) {
+        const int sh = kBits * pass;
         on_threads(kt, [&](int w) {
           int64_t *c = cnt.data() + (size_t)w * kDigits;
           std::fill(c, c + kDigits, 0);
-          for (int64_t i = part(w); i < part(w + 1); ++i) ++c[(key[(size_t)i].first >> sh) & 0x7FFF];
+          for (int64_t i = part(w); i < part(w + 1); ++i) ++c[(key[(size_t)i].first >> sh) & (kDigits - 1)];
         });
         // digit-major, thread-minor offsets keep the sort stable
         int64_t run = 0;
@@ -616,7 +665,7 @@ struct Builder3 {
         on_threads(kt, [&](int w) {
           int64_t *c = cnt.data() + (size_t)w * kDigits;
           for (int64_t i = part(w); i < part(w + 1); ++i)
-            tmp[(size_t)c[(key[(size_t)i].first >> sh) & 0x7FFF]++] = key[(size_t)i];
+            tmp[(size_t)c[(key[(size_t)i].first >> sh) & (kDigits - 1)]++] = key[(size_t)i];
         });
         key.swap(tmp);
       }
