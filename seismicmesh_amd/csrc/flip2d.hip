@@ -25,6 +25,7 @@
 // with no flip and no inverted triangle, every edge is locally Delaunay and
 // the triangulation is the Delaunay triangulation of the points; otherwise
 // the host rebuilds it from scratch with exact predicates.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cstdio>
@@ -1149,9 +1150,13 @@ static int rebuild_flagged_domains(sm_mesher *m) {
   std::vector<int> todo;
   std::vector<int64_t> base;
   int64_t slots = 0;
-  for (int d = 0; d < m->ndom; ++d) {
-    if (!m->h_dfail[d]) continue;
-    todo.push_back(d);
+  for (int d = 0; d < m->ndom; ++d)
+    if (m->h_dfail[d]) todo.push_back(d);
+  // largest domains first, so that the worker threads finish together
+  std::stable_sort(todo.begin(), todo.end(), [&](int a, int b) {
+    return m->h_voff[a + 1] - m->h_voff[a] > m->h_voff[b + 1] - m->h_voff[b];
+  });
+  for (int d : todo) {
     base.push_back(slots);
     slots += m->h_tbase[d + 1] - m->h_tbase[d];
   }

@@ -67,14 +67,21 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 // The same network with the hidden layers' weights shared through LDS: a
 // workgroup of WAVES waves stages each k-step's eight weight fragments
-// (8 KiB) in LDS once -- double-buffered, the next k-step's fragments
-// loaded from global memory while this one's products run -- instead of
-// every wave streaming the whole layer from L2, and each wave multiplies
-// every fragment it reads into PT point tiles of 32.  The launch uses 8
-// waves of one tile each (132 KiB of activations + 16 KiB of fragments per
-// workgroup, two waves per SIMD to hide the fragment loads).  Measured on
-// MI355X at 4M points: 610 TFLOP/s, against 511 with every wave streaming
-// the weights from L2 and 453 with 4 waves of two tiles (one wave per SIMD).
+// (8 KiB) in LDS once -- three stages deep, each filled from a global load
+// issued two k-steps earlier -- instead of every wave streaming the whole
+// layer from L2.  Each wave reads the next k-step's fragments into a second
+// register set before it issues this k-step's eight products, so no product
+// waits on an LDS read.  The epilogue's biases and output weights are
+// staged in LDS as well.  The launch uses 8 waves of one 32-point tile each
+// (132 KiB of activations + 24 KiB of fragments + 3 KiB of epilogue
+// operands per workgroup, two waves per SIMD).  Measured on MI355X at 4M
+// points: 640 TFLOP/s (612 with two stages and the biases read from global
+// memory in the epilogue, 511 with every wave streaming the weights from
+// L2).  Counters put the limit in the vector ALU: the bias + SiLU epilogue
+// of 512 activations per lane and the FP32 input layer take ~23 k VALU
+// cycles per wave against ~12 k of matrix-core cycles, and both waves of a
+// SIMD reach their epilogues together (the k-step barriers keep them in
+// step), so the two do not overlap.
 constexpr int kEvalWaves = 8;
 constexpr int kEvalPT = 1;
 
@@ -83,7 +90,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_eval_lds(sm_mlp net, const d
                                                              const int32_t *cells, int64_t n,
                                                              float *out) {
   __shared__ __align__(16) __bf16 hs[WAVES][32 * PT][kRow];
-  __shared__ __align__(16) bf16x8 wsh[2][8 * 64];
+  __shared__ __align__(16) bf16x8 wsh[3][8 * 64];
+  // the epilogue's operands: each layer's biases (double-buffered: a wave
+  // may start the next layer while another still finishes this one) and
+  // the output weights, so that it never waits on a global load
+  __shared__ __align__(16) float bsh[2][kW];
+  __shared__ __align__(16) float osh[kW];
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int r = lane % 32, half = lane / 32;
   const int64_t p0 = ((int64_t)blockIdx.x * WAVES + wave) * (32 * PT);
@@ -115,23 +127,28 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_eval_lds(sm_mlp net, const d
     }
   }
   const bf16x8 *whid = reinterpret_cast<const bf16x8 *>(net.w_hid);
-  constexpr int kStep = 8 * 64;                       // fragments of one k-step
-  constexpr int kPer = kStep / (64 * WAVES);          // per thread and k-step
-  static_assert(kStep % (64 * WAVES) == 0, "k-step fragments per thread");
+  constexpr int kStep = 8 * 64;  // fragments of one k-step
+  constexpr int kSteps = kW / 16;
+  static_assert(kStep == 64 * WAVES, "one fragment per thread and k-step");
   float s[PT];
 #pragma unroll
   for (int pt = 0; pt < PT; ++pt) s[pt] = 0.0f;
+  for (int i = threadIdx.x; i < kW; i += 64 * WAVES) osh[i] = net.w_out[i];
   for (int l = 0; l < net.hidden; ++l) {
     const bf16x8 *W = whid + (size_t)l * (kW * kW / 8);
-    // k-step 0 of this layer into buffer 0, k-step 1 into registers
-    bf16x8 g[kPer];
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) g[j] = W[threadIdx.x + j * 64 * WAVES];
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) wsh[0][threadIdx.x + j * 64 * WAVES] = g[j];
-#pragma unroll
-    for (int j = 0; j < kPer; ++j) g[j] = W[kStep + threadIdx.x + j * 64 * WAVES];
-    __syncthreads();  // buffer 0 and the layer's input activations are in place
+    for (int i = threadIdx.x; i < kW; i += 64 * WAVES) bsh[l & 1][i] = net.b_hid[(size_t)l * kW + i];
+    // Software pipeline over the 16 k-steps.  Step ks multiplies the
+    // fragments read into registers during step ks - 1, while the reads of
+    // step ks + 1 are in flight; LDS holds three k-steps of weights (read
+    // ks + 1, written ks + 2 from a global load issued two steps before).
+    bf16x8 gq[2];
+    gq[0] = W[0 * kStep + threadIdx.x];
+    wsh[0][threadIdx.x] = gq[0];
+    gq[1] = W[1 * kStep + threadIdx.x];
+    wsh[1][threadIdx.x] = gq[1];
+    gq[0] = W[2 * kStep + threadIdx.x];
+    gq[1] = W[3 * kStep + threadIdx.x];
+    __syncthreads();  // stages 0 and 1 and the layer's input activations are in place
     floatx16 acc[PT][8];
 #pragma unroll
     for (int pt = 0; pt < PT; ++pt)
@@ -139,56 +156,85 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_eval_lds(sm_mlp net, const d
       for (int nb = 0; nb < 8; ++nb)
 #pragma unroll
         for (int i = 0; i < 16; ++i) acc[pt][nb][i] = 0.0f;
-    for (int ks = 0; ks < kW / 16; ++ks) {
-      const bf16x8 *buf = wsh[ks & 1];
-      bf16x8 a[PT];
+    bf16x8 wc[8], ac[PT];
 #pragma unroll
-      for (int pt = 0; pt < PT; ++pt)
-        a[pt] = *reinterpret_cast<const bf16x8 *>(&hs[wave][32 * pt + r][16 * ks + 8 * half]);
+    for (int nb = 0; nb < 8; ++nb) wc[nb] = wsh[0][nb * 64 + lane];
 #pragma unroll
-      for (int nb = 0; nb < 8; ++nb) {
-        const bf16x8 w = buf[nb * 64 + lane];
+    for (int pt = 0; pt < PT; ++pt)
+      ac[pt] = *reinterpret_cast<const bf16x8 *>(&hs[wave][32 * pt + r][8 * half]);
+#pragma unroll
+    for (int ks = 0; ks < kSteps; ++ks) {
+      bf16x8 wn[8], an[PT];
+      if (ks + 1 < kSteps) {
+        const bf16x8 *buf = wsh[(ks + 1) % 3];
+#pragma unroll
+        for (int nb = 0; nb < 8; ++nb) wn[nb] = buf[nb * 64 + lane];
 #pragma unroll
         for (int pt = 0; pt < PT; ++pt)
-          acc[pt][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, a[pt], acc[pt][nb], 0, 0, 0);
+          an[pt] = *reinterpret_cast<const bf16x8 *>(
+              &hs[wave][32 * pt + r][16 * (ks + 1) + 8 * half]);
       }
-      if (ks + 1 < kW / 16) {
-        // the other buffer was last read in k-step ks - 1, before the
-        // barrier that ended it
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int j = 0; j < kPer; ++j) wsh[(ks + 1) & 1][threadIdx.x + j * 64 * WAVES] = g[j];
-        if (ks + 2 < kW / 16) {
+      for (int nb = 0; nb < 8; ++nb)
 #pragma unroll
-          for (int j = 0; j < kPer; ++j)
-            g[j] = W[(ks + 2) * kStep + threadIdx.x + j * 64 * WAVES];
-        }
+        for (int pt = 0; pt < PT; ++pt)
+          acc[pt][nb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wc[nb], ac[pt], acc[pt][nb], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
+      if (ks + 2 < kSteps) {
+        // stage (ks + 2) % 3 last held k-step ks - 1, whose reads (step
+        // ks - 2's prefetch, or the prologue's) every wave completed
+        // before a barrier this wave has passed
+        wsh[(ks + 2) % 3][threadIdx.x] = gq[ks & 1];
+        if (ks + 4 < kSteps) gq[ks & 1] = W[(ks + 4) * kStep + threadIdx.x];
       }
       __syncthreads();
-    }
-    const float *bias = net.b_hid + (size_t)l * kW;
-    const bool last = l + 1 == net.hidden;
+      if (ks + 1 < kSteps) {
 #pragma unroll
-    for (int nb = 0; nb < 8; ++nb)
+        for (int nb = 0; nb < 8; ++nb) wc[nb] = wn[nb];
 #pragma unroll
-      for (int g4 = 0; g4 < 4; ++g4) {
-        const int f0 = 32 * nb + 8 * g4 + 4 * half;
-        const float4 b4 = *reinterpret_cast<const float4 *>(bias + f0);
-        const float bb[4] = {b4.x, b4.y, b4.z, b4.w};
-#pragma unroll
-        for (int pt = 0; pt < PT; ++pt) {
-          bf16x4 v;
-#pragma unroll
-          for (int e = 0; e < 4; ++e) v[e] = (__bf16)silu_fast(acc[pt][nb][4 * g4 + e] + bb[e]);
-          if (last) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) s[pt] = fmaf((float)v[e], net.w_out[f0 + e], s[pt]);
-          } else {
-            // every wave has finished reading this layer's input (the
-            // barrier of the last k-step), so the output may overwrite it
-            *reinterpret_cast<bf16x4 *>(&hs[wave][32 * pt + r][f0]) = v;
-          }
-        }
+        for (int pt = 0; pt < PT; ++pt) ac[pt] = an[pt];
       }
+    }
+    const float *bias = bsh[l & 1];
+    const bool last = l + 1 == net.hidden;
+    // bias + SiLU of every accumulator; straight-line code in each branch
+    // (the branch is uniform), so the LDS reads of the operands batch
+    const auto act = [&](int pt, int nb, int g4, int f0) {
+      const float4 b4 = *reinterpret_cast<const float4 *>(bias + f0);
+      const float bb[4] = {b4.x, b4.y, b4.z, b4.w};
+      bf16x4 v;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[e] = (__bf16)silu_fast(acc[pt][nb][4 * g4 + e] + bb[e]);
+      return v;
+    };
+    if (!last) {
+      // every wave has finished reading this layer's input (the barrier of
+      // the last k-step), so the output may overwrite it
+#pragma unroll
+      for (int nb = 0; nb < 8; ++nb)
+#pragma unroll
+        for (int g4 = 0; g4 < 4; ++g4)
+#pragma unroll
+          for (int pt = 0; pt < PT; ++pt) {
+            const int f0 = 32 * nb + 8 * g4 + 4 * half;
+            *reinterpret_cast<bf16x4 *>(&hs[wave][32 * pt + r][f0]) = act(pt, nb, g4, f0);
+          }
+    } else {
+#pragma unroll
+      for (int nb = 0; nb < 8; ++nb)
+#pragma unroll
+        for (int g4 = 0; g4 < 4; ++g4)
+#pragma unroll
+          for (int pt = 0; pt < PT; ++pt) {
+            const int f0 = 32 * nb + 8 * g4 + 4 * half;
+            const bf16x4 v = act(pt, nb, g4, f0);
+            const float4 o4 = *reinterpret_cast<const float4 *>(&osh[f0]);
+            const float oo[4] = {o4.x, o4.y, o4.z, o4.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) s[pt] = fmaf((float)v[e], oo[e], s[pt]);
+          }
+    }
   }
   if (net.hidden == 0) {
     __syncthreads();
