@@ -565,6 +565,8 @@ int repair_on(sm_mesher *m, const double *pts, int max_rounds) {
 // points in d_pvalid) and move the points towards their current positions
 // in sub-steps, repairing by flips after each; a failed sub-step is retried
 // from the last good state with half the step.  Returns true on success.
+static bool trace_flips();
+
 bool kinetic_repair(sm_mesher *m) {
   hipStream_t s = m->stream;
   const int64_t n2 = m->NA * 2;
@@ -608,8 +610,12 @@ bool kinetic_repair(sm_mesher *m) {
                            m->d_pvalid, m->d_p, s1, m->d_pstage);
       pts = m->d_pstage;
     }
+    const int64_t r0 = m->stats[1];
     int ok = repair_on(m, pts, 64);
     ++stages;
+    if (trace_flips())
+      std::fprintf(stderr, "kinetic_repair: stage %d, %.4f -> %.4f: %s after %lld rounds\n",
+                   stages, s0, s1, ok == 1 ? "ok" : "failed", (long long)(m->stats[1] - r0));
     if (ok == 1) {
       s0 = s1;
       if (s0 < 1.0) save();
@@ -617,7 +623,9 @@ bool kinetic_repair(sm_mesher *m) {
     } else {
       if (ok < 0 || step < 1.0 / 512 || stages > 200) return false;
       restore();
-      step *= 0.5;
+      // half the sub-step just tried (the step may exceed what was left
+      // before the end, and halving it alone would try the same one again)
+      step = 0.5 * (s1 - s0);
     }
   }
   m->stats[6] += 1;
