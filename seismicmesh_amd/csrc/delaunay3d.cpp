@@ -1113,10 +1113,20 @@ struct Builder3 {
     // back to the serial layout: dead slots become free slots
     const int64_t used_n = std::min<int64_t>(nt_count.load(), cap_par);
     T.resize((size_t)std::max<int64_t>(used_n, (int64_t)old_n));
-    free_slots.clear();
-    for (size_t t = 0; t < T.size(); ++t)
-      if (T[t].v[0] == kDead) free_slots.push_back((int)t);
-    mark.assign(T.size() + 64, 0);
+    {
+      // (on the threads: a scan of every slot; the list stays ascending)
+      std::vector<std::vector<int>> dead((size_t)nthr);
+      const int64_t ns = (int64_t)T.size();
+      on_threads(nthr, [&](int w) {
+        for (int64_t t = ns * w / nthr; t < ns * (w + 1) / nthr; ++t)
+          if (T[(size_t)t].v[0] == kDead) dead[(size_t)w].push_back((int)t);
+      });
+      free_slots.clear();
+      for (auto &d : dead) free_slots.insert(free_slots.end(), d.begin(), d.end());
+    }
+    // the serial marks are all clear (the concurrent passes kept theirs in
+    // the lock words)
+    if (mark.size() < T.size() + 64) mark.resize(T.size() + 64, 0);
     locks = nullptr;
     ntet = nullptr;
     dup_mu = nullptr;
