@@ -19,7 +19,9 @@ int insphere_perturbed(const double *a, const double *b, const double *c, const 
 void canonical_tet(int32_t v[4]);
 
 // The tetrahedralisation of a point set, built on one or more threads
-// (delaunay3d.cpp); keeps its scratch between calls.
+// (delaunay3d.cpp); keeps its scratch, and the worker threads of its
+// parallel builds (idle between calls, joined on destruction), between
+// calls.  One build at a time per object.
 struct DT3 {
   // Called on the calling thread with the finite tetrahedra (canonical, in
   // increasing order) in nslabs consecutive pieces (piece index first),
