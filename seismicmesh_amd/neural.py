@@ -368,7 +368,7 @@ class NeuralSDF:
                              "targets: train longer or truncate wider" % (out, self.trunc))
         return out, err
 
-    def calibrate_bands(self, domain, band, res=16, nval=1000000, near=None, safety=2.0,
+    def calibrate_bands(self, domain, band, res=64, nval=8000000, near=None, safety=2.0,
                         seed=54321):
         """Per-region bands for Mesher.set_surrogate: the largest |d_nn - d|
         over validation points (half uniform in the box, half within `near`
@@ -377,7 +377,11 @@ class NeuralSDF:
         cell without validation points gets the largest error overall.  For
         a network trained on a truncated SDF both values are clipped to
         +-trunc first, and a band of trunc or more becomes 1e30 (never
-        skipped: the network's output cannot get past it).
+        skipped: the network's output cannot get past it).  Fine regions
+        keep one bad spot of the network from closing a large block to it
+        (on a 76-primitive CSG: 1 % of the interior decided without the
+        exact program at res = 16, 27 % at 64, 38 % at 128 with 32 M
+        validation points); keep ~100 validation points per region.
         Returns (bands (res,)*dim float32, lo, hi, largest error)."""
         if near is None:
             near = band
