@@ -1321,11 +1321,10 @@ int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, con
   const int nthr = std::max(1, std::min(nthreads, (int)(nslot / 65536 + 1)));
   // Thread w owns slots [nslot w / nthr, ...) for the extraction and the
   // range of lowest ids [n r / nthr, n (r + 1) / nthr) with r = w for the
-  // sort.  Pass 1 counts each thread's cells per range, pass 2 writes them
-  // canonical into the staging array grouped by range (a range's group
-  // spans exactly the positions its cells take in the output), pass 3
-  // counting-sorts each group by lowest id into the output and sorts every
-  // bucket.
+  // sort.  Pass 1 appends each thread's cells, canonical, to its buffer
+  // for their range (buffers kept between calls); pass 2 counting-sorts
+  // each range's cells by lowest id into the output (reading the buffers
+  // of threads 0, 1, ... in turn) and sorts every bucket.
   auto lo_id = [&](int w) { return n * w / nthr; };
   auto range_of = [&](int32_t v0) {
     int rg = (int)((int64_t)v0 * nthr / std::max<int64_t>(n, 1));
@@ -1334,56 +1333,44 @@ int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, con
     return rg;
   };
   auto live = [&](const Tet &r) { return r.v[0] >= 0 && Builder3::inf_slot(r) < 0; };
-  std::vector<int64_t> cnt((size_t)nthr * nthr, 0);
+  if ((int)part_.size() != nthr * nthr) part_.assign((size_t)nthr * nthr, {});
   b.on_threads(nthr, [&](int w) {
     const int64_t s0 = nslot * w / nthr, s1 = nslot * (w + 1) / nthr;
-    int64_t *c = cnt.data() + (size_t)w * nthr;
+    std::vector<int32_t> *row = part_.data() + (size_t)w * nthr;
+    for (int r = 0; r < nthr; ++r) row[r].clear();
     for (int64_t t = s0; t < s1; ++t) {
       const Tet &r = b.T[t];
       if (!live(r)) continue;  // dead or infinite
-      int64_t lo = b.gid[r.v[0]];
-      for (int j = 1; j < 4; ++j) lo = std::min(lo, b.gid[r.v[j]]);
-      ++c[range_of((int32_t)lo)];
+      int32_t v[4];
+      for (int j = 0; j < 4; ++j) v[j] = (int32_t)b.gid[r.v[j]];
+      canonical_tet(v);
+      std::vector<int32_t> &dstv = row[range_of(v[0])];
+      dstv.insert(dstv.end(), v, v + 4);
     }
   });
-  // below[r]: cells with lowest id below range r; pos[w][r]: where thread
-  // w's cells of range r start
-  std::vector<int64_t> below((size_t)nthr + 1, 0), pos((size_t)nthr * nthr);
+  // below[r]: cells with lowest id below range r
+  std::vector<int64_t> below((size_t)nthr + 1, 0);
   for (int r = 0; r < nthr; ++r) {
-    int64_t o = below[(size_t)r];
-    for (int w = 0; w < nthr; ++w) {
-      pos[(size_t)w * nthr + r] = o;
-      o += cnt[(size_t)w * nthr + r];
-    }
-    below[(size_t)r + 1] = o;
+    int64_t c = 0;
+    for (int w = 0; w < nthr; ++w) c += (int64_t)part_[(size_t)w * nthr + r].size() / 4;
+    below[(size_t)r + 1] = below[(size_t)r] + c;
   }
   const int64_t m = below[(size_t)nthr];
-  stage_.resize((size_t)m * 4);
   int32_t *dst = out;
   if (!out || out_cap < m) {
     sorted_.resize((size_t)m * 4);
     dst = sorted_.data();
   }
-  b.on_threads(nthr, [&](int w) {
-    const int64_t s0 = nslot * w / nthr, s1 = nslot * (w + 1) / nthr;
-    int64_t *p = pos.data() + (size_t)w * nthr;
-    for (int64_t t = s0; t < s1; ++t) {
-      const Tet &r = b.T[t];
-      if (!live(r)) continue;
-      int32_t v[4];
-      for (int j = 0; j < 4; ++j) v[j] = (int32_t)b.gid[r.v[j]];
-      canonical_tet(v);
-      const int64_t o = p[range_of(v[0])]++;
-      std::memcpy(&stage_[(size_t)o * 4], v, sizeof(v));
-    }
-  });
   start_.assign((size_t)n + 1, 0);
   auto *cells = reinterpret_cast<std::array<int32_t, 4> *>(dst);
   b.on_threads(nthr, [&](int w) {
     // start_[i + 1] ends bucket i; this thread writes start_[a + 1 .. b] only
-    const int64_t a = lo_id(w), bb = lo_id(w + 1), g0 = below[(size_t)w], g1 = below[(size_t)w + 1];
+    const int64_t a = lo_id(w), bb = lo_id(w + 1), g0 = below[(size_t)w];
     std::vector<int64_t> fill((size_t)(bb - a), 0);
-    for (int64_t k = g0; k < g1; ++k) ++fill[(size_t)(stage_[(size_t)k * 4] - a)];
+    for (int u = 0; u < nthr; ++u) {
+      const std::vector<int32_t> &src = part_[(size_t)u * nthr + w];
+      for (size_t k = 0; k < src.size(); k += 4) ++fill[(size_t)(src[k] - a)];
+    }
     int64_t run = g0;
     for (int64_t i = a; i < bb; ++i) {
       const int64_t c = fill[(size_t)(i - a)];
@@ -1391,10 +1378,12 @@ int DT3::run(const double *p, int64_t n, int axis, int nslabs, int nthreads, con
       run += c;
       start_[(size_t)i + 1] = run;
     }
-    for (int64_t k = g0; k < g1; ++k) {
-      const int32_t *v = &stage_[(size_t)k * 4];
-      const int64_t o = fill[(size_t)(v[0] - a)]++;
-      std::memcpy(dst + 4 * o, v, 4 * sizeof(int32_t));
+    for (int u = 0; u < nthr; ++u) {
+      const std::vector<int32_t> &src = part_[(size_t)u * nthr + w];
+      for (size_t k = 0; k < src.size(); k += 4) {
+        const int64_t o = fill[(size_t)(src[k] - a)]++;
+        std::memcpy(dst + 4 * o, &src[k], 4 * sizeof(int32_t));
+      }
     }
     for (int64_t i = a; i < bb; ++i) {
       const int64_t c0 = i == a ? g0 : start_[(size_t)i];

@@ -60,7 +60,8 @@ struct DT3 {
 
   std::unique_ptr<Builder3> B;
   std::vector<int64_t> start_;
-  std::vector<int32_t> sorted_, stage_;
+  std::vector<int32_t> sorted_;
+  std::vector<std::vector<int32_t>> part_;  // (threads^2) cells per thread and range
   // points binned on a uniform grid (verify)
   double blo_[3] = {0, 0, 0}, bhi_[3] = {0, 0, 0};
   int64_t nb_[3] = {1, 1, 1};
