@@ -255,6 +255,8 @@ def cpu_baseline_many(sts, iters, reps=3, min_points=50000):
 
 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+# the surrogate network 3 -> 256 -> 256 -> 256 -> 256 -> 1: FLOP per evaluation
+MLP_FLOP_PER_POINT = 2 * (3 * 256 + 3 * 256 * 256 + 256)
 
 
 def mlp_measure(npts=1 << 22, reps=10):
@@ -289,7 +291,7 @@ def mlp_measure(npts=1 << 22, reps=10):
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / reps
-    flop = 2.0 * (3 * 256 + 3 * 256 * 256 + 256) * npts
+    flop = float(MLP_FLOP_PER_POINT) * npts
     tf = flop / (ms * 1e-3) / 1e12
     return {"bound": "mfma", "kernel": "k_mlp_eval_lds", "points": npts,
             "flop_per_point": int(flop / npts), "avg_launch_us": round(ms * 1e3, 2),
@@ -438,11 +440,29 @@ def bench_3d(args):
                 "launches_per_iteration": b["launches_per_iteration"],
                 "streaming": streaming,
                 "phases_ms": {k: round(v[0], 3) for k, v in prof.items() if v[0] > 0}}
-    if best in ("centroid_filter", "vertex_update") and neural is not None:
-        roofline["note"] = ("the phase is arithmetic (the FP64 SDF program of the CSG and the "
-                            "BF16 network at every centroid or point), not HBM traffic; the "
-                            "network's own roofline is mlp_roofline")
-    dev_ms = sum(v[0] for v in prof.values()) / args.steps
+    if neural is not None and prof["centroid_network"][1] > 0:
+        # the dominant kernel of the fused iteration is the network at the
+        # centroids (one launch per iteration over every finite tetrahedron
+        # of the build): its matrix-core roofline, timed by the same events
+        ms_net, n_net = prof["centroid_network"]
+        tets = s1["tets_last_build"]
+        flops = tets * MLP_FLOP_PER_POINT
+        tfs = flops / (ms_net / n_net * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "kernel": "k_mlp_eval_lds (centroids)",
+                    "achieved": round(tfs, 1), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(tfs / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
+                    "flop_per_launch": flops, "ms_per_launch": round(ms_net / n_net, 4),
+                    "launches_per_iteration": round(n_net / args.steps, 2),
+                    "note": "the network at every centroid is the largest kernel of the fused "
+                            "iteration; the centroid filter around it (network + exact program "
+                            "where the network does not decide) and the streaming kernels are "
+                            "in streaming",
+                    "streaming": streaming,
+                    "phases_ms": {k: round(v[0], 3) for k, v in prof.items() if v[0] > 0}}
+    # (flip_mark and centroid_network are timed inside flip_round and
+    # centroid_filter)
+    dev_ms = sum(v[0] for k, v in prof.items()
+                 if k not in ("flip_mark", "centroid_network")) / args.steps
 
     cpu = None
     if not args.no_cpu:

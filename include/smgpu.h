@@ -337,7 +337,8 @@ int64_t sm_mesher_get_triangulation(sm_mesher *m, int32_t *tri, int32_t *adj, in
  * not settle (each forces a host rebuild), [8] vertices taken out and put
  * back on the host to untangle what flips could not, [9] 3D builds on
  * several threads, [10] points the threads left to the final single-thread
- * pass (their cells were locked by another thread twice), [11] 0, [12]
+ * pass (their cells were locked by another thread twice), [11] finite
+ * tetrahedra of the last 3D build (0 in 2D), [12]
  * microseconds spent in the host 3D tetrahedralisation (wall clock, copies
  * and overlapped filters included). */
 int sm_mesher_stats(sm_mesher *m, int64_t *stats, int nstats);
@@ -352,9 +353,11 @@ int sm_mesher_set_slabs(sm_mesher *m, int axis, int nslabs, int nthreads);
 /* Per-phase kernel timing with HIP events recorded on the handle's stream.
  * Phases: 0 flip rounds, 1 centroid filter, 2 unique edges + edge lists,
  * 3 bar lengths and sizing, 4 L0 sums, 5 bar forces, 6 gather + update +
- * projection, 7 the flip-marking kernel alone (inside phase 0).  sm_mesher_profile(m, 1) clears and enables;
- * sm_mesher_profile_read fills ms[i] (total) and launches[i]. */
-#define SM_NPHASES 8
+ * projection, 7 the flip-marking kernel alone (inside phase 0), 8 the
+ * surrogate network at the cell centroids (inside phase 1).
+ * sm_mesher_profile(m, 1) clears and enables; sm_mesher_profile_read fills
+ * ms[i] (total) and launches[i]. */
+#define SM_NPHASES 9
 int sm_mesher_profile(sm_mesher *m, int enable);
 int sm_mesher_profile_read(sm_mesher *m, double *ms, int64_t *launches, int nphases);
 /* Average duration (ms) of `reps` back-to-back launches of one phase's

@@ -706,8 +706,10 @@ int centroid_keep_range(sm_mesher *m, int64_t off, int64_t T, bool *done) {
   *done = true;
   if (T <= 0) return SM_OK;
   if (m->surrogate) {
+    const int pb = prof_begin(m);
     int rc = mlp_eval_cells(&m->net, m->d_tri + (m->dim + 1) * off, T, m->d_p, m->d_dnn + off,
                             m->stream);
+    prof_end(m, 8, pb);
     if (rc) return rc;
     if (m->dim == 2) {
       if (m->deep_sdf) launch_keep_surrogate<2, kSdfDeep>(m, off, T);

@@ -244,6 +244,7 @@ int host_rebuild3d(sm_mesher *m, bool *filtered) {
            (long long)m->capT);
   m->T = off;
   m->stats[3] += 1;
+  m->stats[11] = off;
   m->stats[9] += m->dt3.stats.parallel;
   m->stats[10] += m->dt3.stats.retried;
   m->have_tri = true;

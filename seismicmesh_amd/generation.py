@@ -266,7 +266,7 @@ class Mesher:
         return out
 
     PHASES = ("flip_round", "centroid_filter", "unique_edges", "edge_geom", "l0_sums",
-              "edge_force", "vertex_update", "flip_mark")
+              "edge_force", "vertex_update", "flip_mark", "centroid_network")
 
     def set_graph(self, enable=True):
         """Run FP32 iterations as one captured graph launch each (default)."""
@@ -277,9 +277,10 @@ class Mesher:
         _lib.check(_lib.lib.sm_mesher_profile(self.h, int(enable)), "sm_mesher_profile")
 
     def profile_read(self):
-        ms = (ctypes.c_double * 8)()
-        n = (ctypes.c_int64 * 8)()
-        _lib.check(_lib.lib.sm_mesher_profile_read(self.h, ms, n, 8), "sm_mesher_profile_read")
+        k = len(self.PHASES)
+        ms = (ctypes.c_double * k)()
+        n = (ctypes.c_int64 * k)()
+        _lib.check(_lib.lib.sm_mesher_profile_read(self.h, ms, n, k), "sm_mesher_profile_read")
         return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.PHASES)}
 
     def set_surrogate(self, net, band, bands=None):
@@ -357,7 +358,7 @@ class Mesher:
 
     STATS = ("iterations", "flip_rounds", "flips", "host_rebuilds", "exact_predicates",
              "rollbacks", "kinetic_repairs", "unresolved", "relocations", "parallel_builds",
-             "retried_points", "reserved", "host_tet_us")
+             "retried_points", "tets_last_build", "host_tet_us")
 
     def stats(self):
         k = len(self.STATS)
