@@ -106,12 +106,19 @@ __global__ __launch_bounds__(64 * WAVES) void k_mlp_eval_lds(sm_mlp net, const d
     float xin[IN];
     {
       double q[3] = {0.0, 0.0, 0.0};
+      // both halves of the wave take point r; for cells only the lower half
+      // gathers the vertices and hands the scaled input to the upper one
       valid[pt] = pi < n;
-      if (valid[pt]) valid[pt] = mlp_input<IN, CELLS>(x, cells, pi, q);
+      if (valid[pt] && (!CELLS || half == 0)) valid[pt] = mlp_input<IN, CELLS>(x, cells, pi, q);
       if (!valid[pt] && pi < n && half == 0) out[pi] = INFINITY;
 #pragma unroll
       for (int c = 0; c < IN; ++c)
         xin[c] = valid[pt] ? (float)((q[c] - net.shift[c]) * net.scale[c]) : 0.0f;
+      if (CELLS) {
+#pragma unroll
+        for (int c = 0; c < IN; ++c) xin[c] = __shfl(xin[c], r);
+        valid[pt] = __shfl((int)valid[pt], r) != 0;
+      }
     }
     for (int j0 = 0; j0 < kW / 2; j0 += 8) {
       bf16x8 v;
