@@ -118,11 +118,29 @@ def get_boundary_facets(t):
     return unq[cnt == 1]
 
 
+def _boundary_facets_any_order(t):
+    # the facets of one cell only, in no particular order: each sorted facet
+    # as one int64 key, sorted, the keys seen once kept -- several times
+    # faster than the unique-row view on millions of facets
+    f = np.sort(get_facets(t), axis=1)
+    nv = int(f.max()) + 1 if f.size else 0
+    if nv == 0 or nv > (1 << 21):  # nv^3 must fit 64 bits
+        return get_boundary_facets(t)
+    w = np.uint64(nv)
+    k = (f[:, 0].astype(np.uint64) * w + f[:, 1].astype(np.uint64)) * w + f[:, 2].astype(np.uint64)
+    k.sort()
+    new = np.ones(len(k), dtype=bool)
+    new[1:] = k[1:] != k[:-1]
+    start = np.flatnonzero(new)
+    once = k[start[np.diff(np.append(start, len(k))) == 1]]
+    return np.stack([once // (w * w), (once // w) % w, once % w], axis=1).astype(f.dtype)
+
+
 def get_boundary_vertices(t, dim=2):
     if dim == 2:
         b = get_boundary_edges(t)
     elif dim == 3:
-        b = get_boundary_facets(t)
+        b = _boundary_facets_any_order(t)
     else:
         raise ValueError("Dimension not supported.")
     return np.unique(b.reshape(-1))

@@ -135,3 +135,20 @@ def test_simp_qual_closed_forms():
     rt = np.array([[0, 0], [1, 0], [0, 1.0]])
     # right isosceles: 2 r / R = 2 (2 - sqrt 2) / 2 / (sqrt 2 / 2)
     assert np.allclose(og.simp_qual(rt, np.array([[0, 1, 2]])), 2 * (1 - 1 / math.sqrt(2)) / (math.sqrt(2) / 2))
+
+
+def test_boundary_vertices_match_the_oracle():
+    """The package's boundary vertices of tetrahedra (facets keyed as one
+    integer each) equal the oracle's (unique rows of the sorted facets) on
+    random cell sets, including shared and repeated facets."""
+    from seismicmesh_amd import mesh_utils as mu
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        n = int(rng.integers(5, 3000))
+        t = rng.integers(0, n, (int(rng.integers(1, 5000)), 4)).astype(np.int32)
+        s = np.sort(t, 1)
+        t = t[(s[:, 1:] != s[:, :-1]).all(1)]
+        if len(t) == 0:
+            continue
+        want = np.unique(og.get_boundary_facets(t).reshape(-1))
+        assert np.array_equal(mu.get_boundary_vertices(t, 3), want)
