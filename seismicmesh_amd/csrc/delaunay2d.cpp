@@ -239,7 +239,27 @@ int64_t sm_host_delaunay2d(const double *p, int64_t n, int32_t *tri, int32_t *ad
     uint32_t y = (uint32_t)std::min(65535.0, (p[2 * i + 1] - lo[1]) / ext * 65535.0);
     key[i] = {hilbert_d(65536, x, y), (int)i};
   }
-  std::sort(key.begin(), key.end());
+  // stable two-pass radix sort on the 32-bit keys (ids ascending within a
+  // key): the order of std::sort on (key, id); the keys are signed ints
+  // (the curve index wraps past 2^31), so the sign bit is flipped
+  {
+    std::vector<std::pair<int, int>> tmp((size_t)n);
+    std::vector<int64_t> cnt(1 << 16);
+    for (int pass = 0; pass < 2; ++pass) {
+      const int sh = 16 * pass;
+      std::fill(cnt.begin(), cnt.end(), 0);
+      for (int64_t i = 0; i < n; ++i) ++cnt[(((uint32_t)key[i].first ^ 0x80000000u) >> sh) & 0xFFFF];
+      int64_t run = 0;
+      for (int64_t &c : cnt) {
+        const int64_t x = c;
+        c = run;
+        run += x;
+      }
+      for (int64_t i = 0; i < n; ++i)
+        tmp[(size_t)cnt[(((uint32_t)key[i].first ^ 0x80000000u) >> sh) & 0xFFFF]++] = key[i];
+      key.swap(tmp);
+    }
+  }
   std::vector<int> order(n);
   for (int64_t i = 0; i < n; ++i) order[i] = key[i].second;
   // exact duplicates share a curve cell, where the lowest id comes first:
