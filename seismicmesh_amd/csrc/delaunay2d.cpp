@@ -9,8 +9,9 @@
 // counter-clockwise order, with neighbour j opposite vertex j.
 //
 // Points are inserted along a Hilbert curve, coarse to fine (every 4096th
-// point of the curve, then every 64th, then the rest); point location walks
-// from the last created triangle; of exact duplicates the lowest id is kept.
+// point of the curve, then every 64th, then the rest), from a copy stored
+// in curve order; point location walks from the last created triangle; of
+// exact duplicates the lowest id is kept.
 // Cavities use the strict in-circle test, so the
 // triangulation of cocircular configurations depends on insertion order
 // (exactly as for any incremental method); for points in general position the
@@ -279,30 +280,32 @@ int64_t sm_host_delaunay2d(const double *p, int64_t n, int32_t *tri, int32_t *ad
     a = e;
   }
 
+  // the points in curve order: the builder works on positions along the
+  // curve (local ids), so that points near each other sit together in memory
+  std::vector<double> L(2 * (size_t)n);
+  for (int64_t k = 0; k < n; ++k) {
+    L[2 * k] = p[2 * (int64_t)order[k]];
+    L[2 * k + 1] = p[2 * (int64_t)order[k] + 1];
+  }
+  const double *lp = L.data();
   Builder B;
-  B.P = p;
+  B.P = lp;
   // first triangle: two distinct points and a third not collinear with them
-  int i0 = order[0], i1 = -1, i2 = -1;
-  size_t k1 = 0, k2 = 0;
-  for (size_t k = 1; k < (size_t)n; ++k) {
-    int c = order[k];
-    if (p[2 * c] != p[2 * i0] || p[2 * c + 1] != p[2 * i0 + 1]) {
-      i1 = c;
-      k1 = k;
+  int i0 = 0, i1 = -1, i2 = -1;
+  for (int64_t k = 1; k < n; ++k)
+    if (lp[2 * k] != lp[0] || lp[2 * k + 1] != lp[1]) {
+      i1 = (int)k;
       break;
     }
-  }
   if (i1 < 0) {
-    inserted[i0] = 1;
+    inserted[order[0]] = 1;
     return 0;
   }
   int o = 0;
-  for (size_t k = k1 + 1; k < (size_t)n; ++k) {
-    int c = order[k];
-    o = smpred::orient2d_h(p + 2 * i0, p + 2 * i1, p + 2 * c);
+  for (int64_t k = i1 + 1; k < n; ++k) {
+    o = smpred::orient2d_h(lp + 2 * i0, lp + 2 * i1, lp + 2 * k);
     if (o != 0) {
-      i2 = c;
-      k2 = k;
+      i2 = (int)k;
       break;
     }
   }
@@ -310,7 +313,7 @@ int64_t sm_host_delaunay2d(const double *p, int64_t n, int32_t *tri, int32_t *ad
     // all points collinear: no triangles (CGAL reports no finite faces)
     return 0;
   }
-  if (o < 0) std::swap(i1, i2), std::swap(k1, k2);
+  if (o < 0) std::swap(i1, i2);
   // real triangle 0 = (i0, i1, i2); ghosts 1..3 on its three edges
   B.T.resize(4);
   B.T[0] = Tri{{i0, i1, i2}, {2, 3, 1}};
@@ -320,15 +323,15 @@ int64_t sm_host_delaunay2d(const double *p, int64_t n, int32_t *tri, int32_t *ad
   B.T[2] = Tri{{i2, i1, INF}, {1, 3, 0}};
   // ghost across edge (i2,i0) [opposite i1]: (i0, i2, INF)
   B.T[3] = Tri{{i0, i2, INF}, {2, 1, 0}};
-  inserted[i0] = inserted[i1] = inserted[i2] = 1;
+  inserted[order[i0]] = inserted[order[i1]] = inserted[order[i2]] = 1;
   // coarse to fine along the curve: every 4096th point, then every 64th,
   // then the rest (shorter walks and fewer cache misses than one sweep)
   for (const size_t stride : {(size_t)4096, (size_t)64, (size_t)1})
     for (size_t k = 0; k < (size_t)n; ++k) {
       if (k % stride != 0 || (stride < 4096 && k % (stride * 64) == 0)) continue;
-      const int q = order[k];
+      const int q = (int)k;
       if (dupl[k] || q == i0 || q == i1 || q == i2) continue;
-      if (B.insert(q)) inserted[q] = 1;
+      if (B.insert(q)) inserted[order[k]] = 1;
     }
   // compact
   std::vector<int> remap(B.T.size(), -1);
@@ -342,7 +345,8 @@ int64_t sm_host_delaunay2d(const double *p, int64_t n, int32_t *tri, int32_t *ad
     if (dead[t]) continue;
     int64_t r = remap[t];
     for (int k = 0; k < 3; ++k) {
-      tri[3 * r + k] = B.T[t].v[k];
+      const int v = B.T[t].v[k];
+      tri[3 * r + k] = v >= 0 ? order[v] : v;  // back to the caller's ids
       adj[3 * r + k] = remap[B.T[t].n[k]];
     }
   }
