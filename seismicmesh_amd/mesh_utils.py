@@ -69,6 +69,19 @@ def _unique_row_view(data):
     return u.view(data.dtype).reshape(-1, data.shape[1]), cnts
 
 
+def _used_ids(t):
+    # np.unique(t, return_inverse=True) for integer ids: the ids present,
+    # sorted, and each entry's rank among them -- by counting when the ids
+    # are non-negative (linear time), the same arrays
+    flat = np.asarray(t).reshape(-1)
+    if flat.size == 0 or not np.issubdtype(flat.dtype, np.integer) or flat.min() < 0:
+        pix, _, jx = np.unique(t, return_index=True, return_inverse=True)
+        return pix, jx
+    present = np.bincount(flat) > 0
+    rank = np.cumsum(present, dtype=np.intp) - 1
+    return np.flatnonzero(present).astype(flat.dtype), rank[t]
+
+
 def fix_mesh(p, t, ptol=2e-13, dim=2, delete_unused=False, fix_orientation=True):
     """Merge duplicate vertices/cells, drop unused vertices, orient cells."""
     snap = (p.max(0) - p.min(0)).max() * ptol
@@ -78,7 +91,7 @@ def fix_mesh(p, t, ptol=2e-13, dim=2, delete_unused=False, fix_orientation=True)
     t = np.sort(t, axis=1)
     t, _ = _unique_row_view(t)
     if delete_unused:
-        pix, _, jx = np.unique(t, return_index=True, return_inverse=True)
+        pix, jx = _used_ids(t)
         t = np.reshape(jx, t.shape)
         p = p[pix, :]
     if fix_orientation:

@@ -152,3 +152,17 @@ def test_boundary_vertices_match_the_oracle():
             continue
         want = np.unique(og.get_boundary_facets(t).reshape(-1))
         assert np.array_equal(mu.get_boundary_vertices(t, 3), want)
+
+
+def test_used_ids_match_numpy_unique():
+    """fix_mesh's compaction of the vertex ids (by counting) returns exactly
+    np.unique(t, return_inverse=True): values, dtypes and shapes."""
+    from seismicmesh_amd import mesh_utils as mu
+    rng = np.random.default_rng(2)
+    for trial in range(30):
+        shape = (int(rng.integers(0, 3000)), int(rng.integers(1, 5)))
+        t = rng.integers(0, int(rng.integers(1, 10000)), shape).astype([np.int32, np.int64][trial % 2])
+        a_p, _, a_j = np.unique(t, return_index=True, return_inverse=True)
+        b_p, b_j = mu._used_ids(t)
+        assert a_p.dtype == b_p.dtype and np.array_equal(a_p, b_p)
+        assert a_j.dtype == b_j.dtype and a_j.shape == b_j.shape and np.array_equal(a_j, b_j)
